@@ -1,0 +1,140 @@
+// engine_types.hpp — device-visible layout of a world set.
+//
+// A world set is any number of independent worlds stored back to back in one segmented SoA.  A
+// plain simulation is a world set of one world; the batch and the server use the same kernels.
+// Every index stored in a world's arrays (bond targets, cell slots, last absorbed cell, particle
+// slots) is local to that world, so a world's ranges can be copied between sets unchanged.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace alien {
+
+constexpr int kMaxBonds = 6;
+constexpr int kColors = 7;
+constexpr int kParticleBin = 8;  // edge of the energy-particle bins, in world units
+constexpr int kNone = -1;
+constexpr int kClaimFree = 0x7fffffff;
+
+// cell flags word
+constexpr uint32_t kNumConnMask = 0xFu;        // bits 0..3
+constexpr uint32_t kMaxConnShift = 4;          // bits 4..7
+constexpr uint32_t kColorShift = 8;            // bits 8..15
+constexpr uint32_t kBarrierBit = 1u << 16;
+constexpr uint32_t kDetachedBit = 1u << 17;
+// particle flags byte
+constexpr uint8_t kParticleColorMask = 0x7F;
+constexpr uint8_t kParticleDead = 0x80;
+
+__host__ __device__ inline int flags_num_conn(uint32_t f) { return int(f & kNumConnMask); }
+__host__ __device__ inline int flags_max_conn(uint32_t f) { return int((f >> kMaxConnShift) & 0xFu); }
+__host__ __device__ inline int flags_color(uint32_t f) { return int((f >> kColorShift) & 0xFFu); }
+__host__ __device__ inline bool flags_barrier(uint32_t f) { return (f & kBarrierBit) != 0; }
+__host__ __device__ inline int flags_detached(uint32_t f) { return (f & kDetachedBit) ? 1 : 0; }
+__host__ __device__ inline uint32_t flags_with_num_conn(uint32_t f, int n) {
+    return (f & ~kNumConnMask) | (uint32_t(n) & kNumConnMask);
+}
+
+// per-world simulation parameters (physics subset of SimulationParameters)
+struct DevParams {
+    float dt;
+    int motion;  // 0 fluid, 1 collision
+    float smoothing, viscosity, pressure;
+    float collisionDistance, repulsion;
+    float friction, maxVelocity, minDistance, fusionVelocity, maxBindingEnergy;
+    float maxBindingDistance[kColors];
+    float absorption[kColors];
+};
+
+// per-world geometry and placement in the set
+struct DevWorld {
+    int W, H;
+    int binSize, nbx, nby;  // cell bins (row-major)
+    int pnbx, pnby;         // particle bins of kParticleBin units
+    int cellOff, numCells;  // hot and cold cell arrays
+    int binOff;             // first cell bin in the combined bin arrays
+    int pbinOff;            // first particle bin in the combined bin arrays
+    int partOff, partCap;   // particle arrays
+};
+
+// per-world device counters
+struct DevCounters {
+    unsigned long long timestep;
+    int numParticles;  // live particles at [partOff, partOff + numParticles)
+    int pad;
+};
+
+// global counters of one step (reset at the start of every step)
+struct StepCounters {
+    int staged;      // cells of the staged (bonded or barrier) list
+    int candidates;  // fusion candidate pairs
+    int errors;      // bit 0: candidate list overflow
+    int pad;
+};
+
+// hot per-cell / per-particle state; two of these exist and alternate every step
+struct HotBuffers {
+    float2* pos;
+    float2* vel;
+    float* dens;
+    uint32_t* flags;
+    int* slot;    // local cold slot of the cell (stable identity)
+    int* btgt;    // [kMaxBonds][cellCap] local logical index of the bond partner
+    float* brest; // [kMaxBonds][cellCap]
+    float* bang;  // [kMaxBonds][cellCap]
+    float2* ppos;
+    float2* pvel;
+    float* pen;
+    int* plast;   // local cold slot of the last absorbing cell, kNone if none
+    int* pslot;   // local particle slot (stable identity)
+    uint8_t* pflags;
+};
+
+struct Ctx {
+    int numWorlds;
+    int cellCap;  // total cells (= sum of numCells)
+    int partCap;  // total particle slots
+    int numBins;  // cell bins + particle bins of all worlds
+    const DevWorld* worlds;
+    const DevParams* params;
+    DevCounters* ctr;
+    StepCounters* sc;
+    HotBuffers A;  // state at the start of the step (read)
+    HotBuffers B;  // state at the end of the step (written)
+    float* energy;  // cold [cellCap], indexed by cellOff + slot
+    float* stiff;   // cold [cellCap]
+    int* binOf;     // [cellCap] global bin of A-cell i
+    int* rank;      // [cellCap]
+    int* pbinOf;    // [partCap]
+    int* prank;     // [partCap]
+    uint32_t* count;  // [numBins]
+    uint32_t* start;  // [numBins + 1] exclusive scan
+    uint32_t* blockSums;
+    int* sorted;   // [cellCap] new index -> A index
+    int* newidx;   // [cellCap] A index -> new index
+    int* psorted;  // [partCap]
+    int2* staged;  // [cellCap] (new index, world)
+    float2* own;   // [cellCap]
+    float2* s1;
+    float2* s2;
+    float2* out1;  // [kMaxBonds][cellCap]
+    float2* out2;
+    uint32_t* valid;  // [cellCap]
+    uint8_t* delFlag;
+    uint8_t* mark;
+    unsigned long long* claim;  // [cellCap]
+    int4* cand;                 // [candCap] (a, b, world, live)
+    int candCap;
+    int* want;        // [partCap]
+    int* fclaim;      // [partCap]
+    int* absorbCell;  // [partCap]
+    int* cclaim;      // [cellCap]
+    const int2* cellBlocks;  // per block: (world, first global cell index)
+    const int2* binBlocks;   // per block: (world, first global bin index) over cell bins
+    const int2* pbinBlocks;  // per block: (world, first global bin index) over particle bins
+    const int2* partBlocks;  // per block: (world, first global particle index)
+};
+
+}  // namespace alien

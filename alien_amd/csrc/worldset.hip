@@ -1,0 +1,861 @@
+// worldset.hip — device memory, layout, transfer-object conversion and the step loop of a world
+// set.  See worldset.hpp.
+#include "worldset.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <numeric>
+
+namespace alien {
+
+void check_hip(hipError_t e, const char* what) {
+    if (e != hipSuccess) {
+        if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation)
+            throw Error(ALIEN_E_OOM, std::string(what) + ": " + hipGetErrorString(e));
+        throw Error(ALIEN_E_HIP, std::string(what) + ": " + hipGetErrorString(e));
+    }
+}
+
+int bin_size_for(const alien_params& p) {
+    float r = p.motionType == ALIEN_MOTION_FLUID ? p.smoothingLength * 2.0f : p.cellMaxCollisionDistance;
+    int s = int(std::ceil(r));
+    return s < 2 ? 2 : s;
+}
+
+void validate_params(const alien_params& p) {
+    if (p.motionType != ALIEN_MOTION_FLUID && p.motionType != ALIEN_MOTION_COLLISION)
+        throw Error(ALIEN_E_ARG, "motionType must be ALIEN_MOTION_FLUID or ALIEN_MOTION_COLLISION");
+    if (p.innerFriction != 0.0f)
+        throw Error(ALIEN_E_ARG, "innerFriction must be 0 (inner friction is not part of this engine)");
+    if (p.cellMaxForceDecayProb != 0.0f)
+        throw Error(ALIEN_E_ARG, "cellMaxForceDecayProb must be 0 (random bond decay is not part of this engine)");
+    if (!(p.timestepSize > 0.0f)) throw Error(ALIEN_E_ARG, "timestepSize must be positive");
+    if (p.motionType == ALIEN_MOTION_FLUID && !(p.smoothingLength > 0.0f))
+        throw Error(ALIEN_E_ARG, "smoothingLength must be positive");
+    if (p.motionType == ALIEN_MOTION_COLLISION && !(p.cellMaxCollisionDistance > 0.0f))
+        throw Error(ALIEN_E_ARG, "cellMaxCollisionDistance must be positive");
+    float r = p.motionType == ALIEN_MOTION_FLUID ? p.smoothingLength * 2.0f : p.cellMaxCollisionDistance;
+    if (r > 64.0f) throw Error(ALIEN_E_ARG, "interaction radius above 64 is not supported");
+}
+
+void validate_world(const alien_world_desc& d, const alien_params& p) {
+    validate_params(p);
+    int s = bin_size_for(p);
+    if (d.worldSizeX < 3 * s || d.worldSizeY < 3 * s)
+        throw Error(ALIEN_E_ARG, "world must span at least three interaction bins per axis");
+    if (int64_t(d.worldSizeX) * int64_t(d.worldSizeY) > (int64_t(1) << 30))
+        throw Error(ALIEN_E_ARG, "world area above 2^30 is not supported");
+}
+
+DevParams to_dev_params(const alien_params& p) {
+    DevParams d{};
+    d.dt = p.timestepSize;
+    d.motion = p.motionType == ALIEN_MOTION_FLUID ? 0 : 1;
+    d.smoothing = p.smoothingLength;
+    d.viscosity = p.viscosityStrength;
+    d.pressure = p.pressureStrength;
+    d.collisionDistance = p.cellMaxCollisionDistance;
+    d.repulsion = p.cellRepulsionStrength;
+    d.friction = p.friction;
+    d.maxVelocity = p.cellMaxVelocity;
+    d.minDistance = p.cellMinDistance;
+    d.fusionVelocity = p.cellFusionVelocity;
+    d.maxBindingEnergy = p.cellMaxBindingEnergy;
+    for (int c = 0; c < kColors; ++c) {
+        d.maxBindingDistance[c] = p.cellMaxBindingDistance[c];
+        d.absorption[c] = p.radiationAbsorption[c];
+    }
+    return d;
+}
+
+// BaseMap::correctPosition(float2&) (Map.cuh:20-26) on the host, same operations as the device
+static inline int host_floor_int(float v) {
+    int r = static_cast<int>(v);
+    if (static_cast<float>(r) > v) --r;
+    return r;
+}
+static inline void host_wrap(float& x, float& y, int W, int H) {
+    int ix = host_floor_int(x), iy = host_floor_int(y);
+    float fx = x - static_cast<float>(ix), fy = y - static_cast<float>(iy);
+    ix = ((ix % W) + W) % W;
+    iy = ((iy % H) + H) % H;
+    x = static_cast<float>(ix) + fx;
+    y = static_cast<float>(iy) + fy;
+}
+
+// ---------------------------------------------------------------------------------------------
+// WorldImage <-> transfer objects
+// ---------------------------------------------------------------------------------------------
+WorldImage WorldImage::fromTO(const alien_world_desc& desc, const alien_params& params, const alien_cell_to* cells,
+                              uint64_t numCells, const alien_particle_to* particles, uint64_t numParticles) {
+    validate_world(desc, params);
+    if (numCells > uint64_t(0x3fffffff) || numParticles > uint64_t(0x3fffffff))
+        throw Error(ALIEN_E_ARG, "too many cells or particles for one world");
+    if ((numCells && !cells) || (numParticles && !particles)) throw Error(ALIEN_E_ARG, "null transfer array");
+    WorldImage img;
+    const int n = int(numCells), np = int(numParticles);
+    const int W = desc.worldSizeX, H = desc.worldSizeY;
+    HostWorld& h = img.host;
+    h.desc = desc;
+    h.params = params;
+    h.numCells = n;
+    h.particleCap = np;
+    h.cellId.resize(n);
+    h.age.resize(n);
+    h.creatureId.resize(n);
+    h.mutationId.resize(n);
+    h.activationTime.resize(n);
+    h.livingState.resize(n);
+    h.cellFunction.resize(n);
+    h.ancestorMutationId.resize(n);
+    h.selected.resize(n);
+    h.genomeComplexity.resize(n);
+    img.pos.resize(n);
+    img.vel.resize(n);
+    img.dens.assign(n, 1.0f);
+    img.flags.resize(n);
+    img.slot.resize(n);
+    img.btgt.assign(size_t(kMaxBonds) * n, 0);
+    img.brest.assign(size_t(kMaxBonds) * n, 0.0f);
+    img.bang.assign(size_t(kMaxBonds) * n, 0.0f);
+    img.energy.resize(n);
+    img.stiff.resize(n);
+    for (int i = 0; i < n; ++i) {
+        const alien_cell_to& c = cells[i];
+        if (c.numConnections > kMaxBonds) throw Error(ALIEN_E_ARG, "numConnections above 6");
+        float x = c.pos[0], y = c.pos[1];
+        if (!std::isfinite(x) || !std::isfinite(y)) throw Error(ALIEN_E_ARG, "non-finite cell position");
+        host_wrap(x, y, W, H);
+        img.pos[i] = make_float2(x, y);
+        img.vel[i] = make_float2(c.vel[0], c.vel[1]);
+        uint32_t maxConn = c.maxConnections > 15 ? 15u : c.maxConnections;
+        img.flags[i] = uint32_t(c.numConnections) | (maxConn << kMaxConnShift) | (uint32_t(c.color) << kColorShift) |
+                       (c.barrier ? kBarrierBit : 0u);
+        img.slot[i] = i;
+        for (int b = 0; b < c.numConnections; ++b) {
+            int t = c.connections[b].cellIndex;
+            if (t < 0 || t >= n) throw Error(ALIEN_E_ARG, "bond cellIndex out of range");
+            img.btgt[size_t(b) * n + i] = t;
+            img.brest[size_t(b) * n + i] = c.connections[b].distance;
+            img.bang[size_t(b) * n + i] = c.connections[b].angleFromPrevious;
+        }
+        img.energy[i] = c.energy;
+        img.stiff[i] = c.stiffness;
+        h.cellId[i] = c.id;
+        h.age[i] = c.age;
+        h.creatureId[i] = c.creatureId;
+        h.mutationId[i] = c.mutationId;
+        h.activationTime[i] = c.activationTime;
+        h.livingState[i] = c.livingState;
+        h.cellFunction[i] = c.cellFunction;
+        h.ancestorMutationId[i] = c.ancestorMutationId;
+        h.selected[i] = c.selected;
+        h.genomeComplexity[i] = c.genomeComplexity;
+    }
+    img.ppos.resize(np);
+    img.pvel.resize(np);
+    img.pen.resize(np);
+    img.plast.assign(np, kNone);
+    img.pslot.resize(np);
+    img.pflags.resize(np);
+    h.particleId.resize(np);
+    h.particleSelected.resize(np);
+    for (int i = 0; i < np; ++i) {
+        const alien_particle_to& p = particles[i];
+        float x = p.pos[0], y = p.pos[1];
+        if (!std::isfinite(x) || !std::isfinite(y)) throw Error(ALIEN_E_ARG, "non-finite particle position");
+        host_wrap(x, y, W, H);
+        img.ppos[i] = make_float2(x, y);
+        img.pvel[i] = make_float2(p.vel[0], p.vel[1]);
+        img.pen[i] = p.energy;
+        img.pslot[i] = i;
+        img.pflags[i] = uint8_t(p.color & kParticleColorMask);
+        h.particleId[i] = p.id;
+        h.particleSelected[i] = p.selected;
+    }
+    img.numParticles = np;
+    return img;
+}
+
+void WorldImage::toTO(std::vector<alien_cell_to>& cells, std::vector<alien_particle_to>& particles) const {
+    const HostWorld& h = host;
+    const int n = h.numCells;
+    std::vector<int> order(n);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int a, int b) { return h.cellId[slot[a]] < h.cellId[slot[b]]; });
+    std::vector<int> posOf(n);
+    for (int k = 0; k < n; ++k) posOf[order[k]] = k;
+    cells.assign(n, alien_cell_to{});
+    const uint64_t s = h.stepsSinceSet;
+    for (int k = 0; k < n; ++k) {
+        const int i = order[k];
+        const int sl = slot[i];
+        alien_cell_to& c = cells[k];
+        const uint32_t f = flags[i];
+        c.id = h.cellId[sl];
+        c.pos[0] = pos[i].x;
+        c.pos[1] = pos[i].y;
+        c.vel[0] = vel[i].x;
+        c.vel[1] = vel[i].y;
+        c.energy = energy[sl];
+        c.stiffness = stiff[sl];
+        c.color = uint8_t(flags_color(f));
+        c.maxConnections = uint8_t(flags_max_conn(f));
+        c.numConnections = uint8_t(flags_num_conn(f));
+        c.barrier = flags_barrier(f) ? 1 : 0;
+        for (int b = 0; b < c.numConnections; ++b) {
+            c.connections[b].cellIndex = posOf[btgt[size_t(b) * n + i]];
+            c.connections[b].distance = brest[size_t(b) * n + i];
+            c.connections[b].angleFromPrevious = bang[size_t(b) * n + i];
+        }
+        // aging and living-state transitions (CellProcessor.cuh:561-676) evaluated in closed form
+        // for the steps since the data was set: every step ages non-barrier cells by one, counts a
+        // Ready cell's activationTime down, and turns Activating/Detaching/Reviving into Ready
+        uint8_t ls = h.livingState[sl];
+        uint32_t at = h.activationTime[sl];
+        uint32_t age = h.age[sl];
+        if (s > 0) {
+            if (c.barrier) {
+                ls = 0;
+            } else {
+                age = uint32_t(uint64_t(age) + s);
+                if (ls == 0) {
+                    at = uint64_t(at) > s ? uint32_t(at - s) : 0u;
+                } else if (ls == 2 || ls == 3 || ls == 4) {
+                    uint64_t d = s - 1;
+                    at = uint64_t(at) > d ? uint32_t(at - d) : 0u;
+                    ls = 0;
+                }
+            }
+        }
+        c.age = age;
+        c.livingState = ls;
+        c.activationTime = at;
+        c.cellFunction = h.cellFunction[sl];
+        c.ancestorMutationId = h.ancestorMutationId[sl];
+        c.selected = h.selected[sl];
+        c.creatureId = h.creatureId[sl];
+        c.mutationId = h.mutationId[sl];
+        c.genomeComplexity = h.genomeComplexity[sl];
+    }
+    const int np = numParticles;
+    std::vector<int> po(np);
+    std::iota(po.begin(), po.end(), 0);
+    std::stable_sort(po.begin(), po.end(),
+                     [&](int a, int b) { return h.particleId[pslot[a]] < h.particleId[pslot[b]]; });
+    particles.assign(np, alien_particle_to{});
+    for (int k = 0; k < np; ++k) {
+        const int i = po[k];
+        alien_particle_to& p = particles[k];
+        p.id = h.particleId[pslot[i]];
+        p.energy = pen[i];
+        p.pos[0] = ppos[i].x;
+        p.pos[1] = ppos[i].y;
+        p.vel[0] = pvel[i].x;
+        p.vel[1] = pvel[i].y;
+        p.color = uint8_t(pflags[i] & kParticleColorMask);
+        p.selected = h.particleSelected[pslot[i]];
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// WorldSet
+// ---------------------------------------------------------------------------------------------
+WorldSet::WorldSet() { check_hip(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate"); }
+
+WorldSet::~WorldSet() {
+    invalidateGraphs();
+    freeAll();
+    for (auto& a : tableAllocs_) (void)hipFree(a.p);
+    tableAllocs_.clear();
+    if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+void* WorldSet::alloc(size_t bytes) {
+    if (bytes == 0) bytes = 64;
+    bytes = (bytes + 255) & ~size_t(255);
+    void* p = nullptr;
+    check_hip(hipMalloc(&p, bytes), "hipMalloc");
+    allocs_.push_back({p, bytes});
+    deviceBytes_ += bytes;
+    return p;
+}
+
+void WorldSet::freeAll() {
+    for (auto& a : allocs_) (void)hipFree(a.p);
+    allocs_.clear();
+    deviceBytes_ = 0;
+}
+
+void WorldSet::invalidateGraphs() {
+    for (auto& g : graph_)
+        if (g) {
+            (void)hipGraphExecDestroy(g);
+            g = nullptr;
+        }
+}
+
+void WorldSet::synchronize() { check_hip(hipStreamSynchronize(stream_), "hipStreamSynchronize"); }
+
+uint64_t WorldSet::estimateBytes(uint64_t cells, uint64_t particles, int W, int H, int binSize) {
+    const uint64_t perCell = 2 * (8 + 8 + 4 + 4 + 4 + kMaxBonds * 12) + 8 + 4 * 4 + 8 + 3 * 8 + 2 * kMaxBonds * 8 + 4 +
+                             2 + 8 + 4 * 16 + 4;
+    const uint64_t perParticle = 2 * (8 + 8 + 4 + 4 + 4 + 1) + 6 * 4;
+    const uint64_t bins = uint64_t(std::max(1, W / binSize)) * uint64_t(std::max(1, H / binSize)) +
+                          uint64_t(std::max(1, W / kParticleBin)) * uint64_t(std::max(1, H / kParticleBin));
+    return cells * perCell + particles * perParticle + bins * 8 + (1u << 20);
+}
+
+void WorldSet::clearAll() {
+    invalidateGraphs();
+    freeAll();
+    hosts_.clear();
+    worlds_.clear();
+    params_.clear();
+    densityPending_.clear();
+    cellCap_ = partCap_ = numBins_ = candCap_ = 0;
+    parity_ = 0;
+}
+
+void WorldSet::build(const std::vector<Source>& sources) {
+    // collect the new layout first (sources may live in this very set's old buffers only via a
+    // different WorldSet object)
+    std::vector<HostWorld> hosts;
+    std::vector<DevWorld> worlds;
+    std::vector<DevParams> params;
+    std::vector<uint64_t> timesteps;
+    std::vector<int> liveParticles;
+    std::vector<uint8_t> pending;
+    int cellOff = 0, partOff = 0, bins = 0;
+    for (const auto& src : sources) {
+        HostWorld h;
+        int live = 0;
+        uint64_t ts = 0;
+        bool pend = false;
+        if (src.image) {
+            h = src.image->host;
+            live = src.image->numParticles;
+            ts = src.image->timestep;
+            pend = src.image->densityResetPending;
+        } else {
+            h = src.from->hosts_.at(src.fromIndex);
+            live = src.from->numParticles(src.fromIndex);
+            ts = src.from->timestep(src.fromIndex);
+            pend = src.from->densityPending_.at(src.fromIndex) != 0;
+        }
+        validate_world(h.desc, h.params);
+        h.particleCap = live;
+        DevWorld w{};
+        w.W = h.desc.worldSizeX;
+        w.H = h.desc.worldSizeY;
+        w.binSize = bin_size_for(h.params);
+        w.nbx = w.W / w.binSize;
+        w.nby = w.H / w.binSize;
+        w.pnbx = std::max(1, w.W / kParticleBin);
+        w.pnby = std::max(1, w.H / kParticleBin);
+        w.cellOff = cellOff;
+        w.numCells = h.numCells;
+        w.binOff = bins;
+        w.partOff = partOff;
+        w.partCap = live;
+        cellOff += h.numCells;
+        partOff += live;
+        bins += w.nbx * w.nby;
+        hosts.push_back(std::move(h));
+        worlds.push_back(w);
+        params.push_back(to_dev_params(hosts.back().params));
+        timesteps.push_back(ts);
+        liveParticles.push_back(live);
+        pending.push_back(pend ? 1 : 0);
+    }
+    for (auto& w : worlds) {
+        w.pbinOff = bins;
+        bins += w.pnbx * w.pnby;
+    }
+
+    // allocate the new buffers into a fresh set of allocations, copy, then swap in
+    WorldSet& me = *this;
+    std::vector<Arr> oldAllocs;
+    oldAllocs.swap(allocs_);
+    size_t oldBytes = deviceBytes_;
+    deviceBytes_ = 0;
+    HotBuffers oldBuf[2] = {buf_[0], buf_[1]};
+    (void)oldBuf;
+    (void)oldBytes;
+    invalidateGraphs();
+
+    const int NC = std::max(cellOff, 1), NP = std::max(partOff, 1), NB = std::max(bins, 1);
+    cellCap_ = NC;
+    partCap_ = NP;
+    numBins_ = NB;
+    candCap_ = std::max(4 * NC, 4096);
+    auto A = [&](size_t bytes) { return me.alloc(bytes); };
+    dWorlds_ = static_cast<DevWorld*>(A(sizeof(DevWorld) * std::max<size_t>(worlds.size(), 1)));
+    dParams_ = static_cast<DevParams*>(A(sizeof(DevParams) * std::max<size_t>(worlds.size(), 1)));
+    dCtr_ = static_cast<DevCounters*>(A(sizeof(DevCounters) * std::max<size_t>(worlds.size(), 1)));
+    dSc_ = static_cast<StepCounters*>(A(sizeof(StepCounters)));
+    for (int b = 0; b < 2; ++b) {
+        HotBuffers& h = buf_[b];
+        h.pos = static_cast<float2*>(A(sizeof(float2) * NC));
+        h.vel = static_cast<float2*>(A(sizeof(float2) * NC));
+        h.dens = static_cast<float*>(A(sizeof(float) * NC));
+        h.flags = static_cast<uint32_t*>(A(sizeof(uint32_t) * NC));
+        h.slot = static_cast<int*>(A(sizeof(int) * NC));
+        h.btgt = static_cast<int*>(A(sizeof(int) * kMaxBonds * size_t(NC)));
+        h.brest = static_cast<float*>(A(sizeof(float) * kMaxBonds * size_t(NC)));
+        h.bang = static_cast<float*>(A(sizeof(float) * kMaxBonds * size_t(NC)));
+        h.ppos = static_cast<float2*>(A(sizeof(float2) * NP));
+        h.pvel = static_cast<float2*>(A(sizeof(float2) * NP));
+        h.pen = static_cast<float*>(A(sizeof(float) * NP));
+        h.plast = static_cast<int*>(A(sizeof(int) * NP));
+        h.pslot = static_cast<int*>(A(sizeof(int) * NP));
+        h.pflags = static_cast<uint8_t*>(A(NP));
+    }
+    energy_ = static_cast<float*>(A(sizeof(float) * NC));
+    stiff_ = static_cast<float*>(A(sizeof(float) * NC));
+    binOf_ = static_cast<int*>(A(sizeof(int) * NC));
+    rank_ = static_cast<int*>(A(sizeof(int) * NC));
+    pbinOf_ = static_cast<int*>(A(sizeof(int) * NP));
+    prank_ = static_cast<int*>(A(sizeof(int) * NP));
+    count_ = static_cast<uint32_t*>(A(sizeof(uint32_t) * NB));
+    start_ = static_cast<uint32_t*>(A(sizeof(uint32_t) * (size_t(NB) + 1)));
+    blockSums_ = static_cast<uint32_t*>(A(sizeof(uint32_t) * (size_t(NB) / 2048 + 2)));
+    sorted_ = static_cast<int*>(A(sizeof(int) * NC));
+    newidx_ = static_cast<int*>(A(sizeof(int) * NC));
+    psorted_ = static_cast<int*>(A(sizeof(int) * NP));
+    staged_ = static_cast<int2*>(A(sizeof(int2) * NC));
+    own_ = static_cast<float2*>(A(sizeof(float2) * NC));
+    s1_ = static_cast<float2*>(A(sizeof(float2) * NC));
+    s2_ = static_cast<float2*>(A(sizeof(float2) * NC));
+    out1_ = static_cast<float2*>(A(sizeof(float2) * kMaxBonds * size_t(NC)));
+    out2_ = static_cast<float2*>(A(sizeof(float2) * kMaxBonds * size_t(NC)));
+    valid_ = static_cast<uint32_t*>(A(sizeof(uint32_t) * NC));
+    delFlag_ = static_cast<uint8_t*>(A(NC));
+    mark_ = static_cast<uint8_t*>(A(NC));
+    claim_ = static_cast<unsigned long long*>(A(sizeof(unsigned long long) * NC));
+    cand_ = static_cast<int4*>(A(sizeof(int4) * size_t(candCap_)));
+    want_ = static_cast<int*>(A(sizeof(int) * NP));
+    fclaim_ = static_cast<int*>(A(sizeof(int) * NP));
+    absorbCell_ = static_cast<int*>(A(sizeof(int) * NP));
+    cclaim_ = static_cast<int*>(A(sizeof(int) * NC));
+
+    // copy world states into buffer 0
+    HotBuffers& D = buf_[0];
+    for (size_t wi = 0; wi < sources.size(); ++wi) {
+        const auto& src = sources[wi];
+        const DevWorld& w = worlds[wi];
+        const int n = w.numCells, np = liveParticles[wi];
+        auto h2d = [&](void* dst, const void* s, size_t bytes) {
+            if (bytes) check_hip(hipMemcpyAsync(dst, s, bytes, hipMemcpyHostToDevice, stream_), "hipMemcpy H2D");
+        };
+        auto d2d = [&](void* dst, const void* s, size_t bytes) {
+            if (bytes) check_hip(hipMemcpyAsync(dst, s, bytes, hipMemcpyDeviceToDevice, stream_), "hipMemcpy D2D");
+        };
+        if (src.image) {
+            const WorldImage& im = *src.image;
+            h2d(D.pos + w.cellOff, im.pos.data(), sizeof(float2) * n);
+            h2d(D.vel + w.cellOff, im.vel.data(), sizeof(float2) * n);
+            h2d(D.dens + w.cellOff, im.dens.data(), sizeof(float) * n);
+            h2d(D.flags + w.cellOff, im.flags.data(), sizeof(uint32_t) * n);
+            h2d(D.slot + w.cellOff, im.slot.data(), sizeof(int) * n);
+            for (int b = 0; b < kMaxBonds; ++b) {
+                h2d(D.btgt + size_t(b) * NC + w.cellOff, im.btgt.data() + size_t(b) * n, sizeof(int) * n);
+                h2d(D.brest + size_t(b) * NC + w.cellOff, im.brest.data() + size_t(b) * n, sizeof(float) * n);
+                h2d(D.bang + size_t(b) * NC + w.cellOff, im.bang.data() + size_t(b) * n, sizeof(float) * n);
+            }
+            h2d(energy_ + w.cellOff, im.energy.data(), sizeof(float) * n);
+            h2d(stiff_ + w.cellOff, im.stiff.data(), sizeof(float) * n);
+            h2d(D.ppos + w.partOff, im.ppos.data(), sizeof(float2) * np);
+            h2d(D.pvel + w.partOff, im.pvel.data(), sizeof(float2) * np);
+            h2d(D.pen + w.partOff, im.pen.data(), sizeof(float) * np);
+            h2d(D.plast + w.partOff, im.plast.data(), sizeof(int) * np);
+            h2d(D.pslot + w.partOff, im.pslot.data(), sizeof(int) * np);
+            h2d(D.pflags + w.partOff, im.pflags.data(), np);
+            // the staging vectors must outlive the async copies
+            check_hip(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+        } else {
+            WorldSet& o = *src.from;
+            o.synchronize();
+            const DevWorld& ow = o.worlds_.at(src.fromIndex);
+            const HotBuffers& S = o.buf_[o.parity_];
+            const int oNC = o.cellCap_;
+            d2d(D.pos + w.cellOff, S.pos + ow.cellOff, sizeof(float2) * n);
+            d2d(D.vel + w.cellOff, S.vel + ow.cellOff, sizeof(float2) * n);
+            d2d(D.dens + w.cellOff, S.dens + ow.cellOff, sizeof(float) * n);
+            d2d(D.flags + w.cellOff, S.flags + ow.cellOff, sizeof(uint32_t) * n);
+            d2d(D.slot + w.cellOff, S.slot + ow.cellOff, sizeof(int) * n);
+            for (int b = 0; b < kMaxBonds; ++b) {
+                d2d(D.btgt + size_t(b) * NC + w.cellOff, S.btgt + size_t(b) * oNC + ow.cellOff, sizeof(int) * n);
+                d2d(D.brest + size_t(b) * NC + w.cellOff, S.brest + size_t(b) * oNC + ow.cellOff, sizeof(float) * n);
+                d2d(D.bang + size_t(b) * NC + w.cellOff, S.bang + size_t(b) * oNC + ow.cellOff, sizeof(float) * n);
+            }
+            d2d(energy_ + w.cellOff, o.energy_ + ow.cellOff, sizeof(float) * n);
+            d2d(stiff_ + w.cellOff, o.stiff_ + ow.cellOff, sizeof(float) * n);
+            d2d(D.ppos + w.partOff, S.ppos + ow.partOff, sizeof(float2) * np);
+            d2d(D.pvel + w.partOff, S.pvel + ow.partOff, sizeof(float2) * np);
+            d2d(D.pen + w.partOff, S.pen + ow.partOff, sizeof(float) * np);
+            d2d(D.plast + w.partOff, S.plast + ow.partOff, sizeof(int) * np);
+            d2d(D.pslot + w.partOff, S.pslot + ow.partOff, sizeof(int) * np);
+            d2d(D.pflags + w.partOff, S.pflags + ow.partOff, np);
+        }
+    }
+    check_hip(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+    // release the previous buffers of this set (sources from other sets were copied already)
+    for (auto& a : oldAllocs) (void)hipFree(a.p);
+
+    hosts_ = std::move(hosts);
+    worlds_ = std::move(worlds);
+    params_ = std::move(params);
+    densityPending_ = std::move(pending);
+    parity_ = 0;
+
+    std::vector<DevCounters> ctr(worlds_.size());
+    for (size_t i = 0; i < worlds_.size(); ++i) {
+        ctr[i].timestep = timesteps[i];
+        ctr[i].numParticles = liveParticles[i];
+        ctr[i].pad = 0;
+    }
+    if (!worlds_.empty()) {
+        check_hip(hipMemcpy(dWorlds_, worlds_.data(), sizeof(DevWorld) * worlds_.size(), hipMemcpyHostToDevice), "H2D");
+        check_hip(hipMemcpy(dParams_, params_.data(), sizeof(DevParams) * params_.size(), hipMemcpyHostToDevice), "H2D");
+        check_hip(hipMemcpy(dCtr_, ctr.data(), sizeof(DevCounters) * ctr.size(), hipMemcpyHostToDevice), "H2D");
+    }
+    StepCounters sc{};
+    check_hip(hipMemcpy(dSc_, &sc, sizeof(sc), hipMemcpyHostToDevice), "H2D");
+    launch_fill_u32(count_, 0u, size_t(NB), stream_);
+    launch_fill_u32(reinterpret_cast<uint32_t*>(fclaim_), uint32_t(kClaimFree), size_t(NP), stream_);
+    launch_fill_u32(reinterpret_cast<uint32_t*>(cclaim_), uint32_t(kClaimFree), size_t(NC), stream_);
+    launch_fill_u32(reinterpret_cast<uint32_t*>(claim_), 0xffffffffu, size_t(NC) * 2, stream_);
+    launch_fill_u32(reinterpret_cast<uint32_t*>(delFlag_), 0u, (size_t(NC) + 3) / 4, stream_);
+    launch_fill_u32(reinterpret_cast<uint32_t*>(mark_), 0u, (size_t(NC) + 3) / 4, stream_);
+    launch_fill_u32(reinterpret_cast<uint32_t*>(buf_[1].pflags), 0u, (size_t(NP) + 3) / 4, stream_);
+    check_hip(hipGetLastError(), "fill kernels");
+    buildTables();
+    synchronize();
+}
+
+void WorldSet::buildTables() {
+    for (auto& a : tableAllocs_) (void)hipFree(a.p);
+    tableAllocs_.clear();
+    std::vector<int2> cells, bins, pbins, parts;
+    const int ct = shape_.cellThreads, pt = shape_.partThreads, bt = shape_.binThreads;
+    for (int wi = 0; wi < int(worlds_.size()); ++wi) {
+        const DevWorld& w = worlds_[wi];
+        for (int s = 0; s < w.numCells; s += ct) cells.push_back(make_int2(wi, w.cellOff + s));
+        for (int s = 0; s < w.partCap; s += pt) parts.push_back(make_int2(wi, w.partOff + s));
+        for (int s = 0; s < w.nbx * w.nby; s += bt) bins.push_back(make_int2(wi, w.binOff + s));
+        for (int s = 0; s < w.pnbx * w.pnby; s += bt) pbins.push_back(make_int2(wi, w.pbinOff + s));
+    }
+    auto up = [&](const std::vector<int2>& v) -> int2* {
+        if (v.empty()) return nullptr;
+        void* p = nullptr;
+        check_hip(hipMalloc(&p, sizeof(int2) * v.size()), "hipMalloc");
+        tableAllocs_.push_back({p, sizeof(int2) * v.size()});
+        check_hip(hipMemcpy(p, v.data(), sizeof(int2) * v.size(), hipMemcpyHostToDevice), "H2D");
+        return static_cast<int2*>(p);
+    };
+    cellBlocks_ = up(cells);
+    binBlocks_ = up(bins);
+    pbinBlocks_ = up(pbins);
+    partBlocks_ = up(parts);
+    shape_.cellBlocks = int(cells.size());
+    shape_.partBlocks = int(parts.size());
+    shape_.binBlocks = int(bins.size());
+    shape_.pbinBlocks = int(pbins.size());
+    invalidateGraphs();
+}
+
+void WorldSet::setBlockSize(int threads) {
+    if (threads < 64 || threads > 256 || threads % 64 != 0)
+        throw Error(ALIEN_E_ARG, "block size must be 64, 128, 192 or 256");
+    shape_.cellThreads = threads;
+    shape_.partThreads = threads;
+    shape_.binThreads = threads;
+    if (!worlds_.empty()) buildTables();
+}
+
+void WorldSet::setUseGraph(bool on) {
+    useGraph_ = on;
+    invalidateGraphs();
+}
+
+Ctx WorldSet::makeCtx(int parity) const {
+    Ctx c{};
+    c.numWorlds = int(worlds_.size());
+    c.cellCap = cellCap_;
+    c.partCap = partCap_;
+    c.numBins = numBins_;
+    c.worlds = dWorlds_;
+    c.params = dParams_;
+    c.ctr = dCtr_;
+    c.sc = dSc_;
+    c.A = buf_[parity];
+    c.B = buf_[parity ^ 1];
+    c.energy = energy_;
+    c.stiff = stiff_;
+    c.binOf = binOf_;
+    c.rank = rank_;
+    c.pbinOf = pbinOf_;
+    c.prank = prank_;
+    c.count = count_;
+    c.start = start_;
+    c.blockSums = blockSums_;
+    c.sorted = sorted_;
+    c.newidx = newidx_;
+    c.psorted = psorted_;
+    c.staged = staged_;
+    c.own = own_;
+    c.s1 = s1_;
+    c.s2 = s2_;
+    c.out1 = out1_;
+    c.out2 = out2_;
+    c.valid = valid_;
+    c.delFlag = delFlag_;
+    c.mark = mark_;
+    c.claim = claim_;
+    c.cand = cand_;
+    c.candCap = candCap_;
+    c.want = want_;
+    c.fclaim = fclaim_;
+    c.absorbCell = absorbCell_;
+    c.cclaim = cclaim_;
+    c.cellBlocks = cellBlocks_;
+    c.binBlocks = binBlocks_;
+    c.pbinBlocks = pbinBlocks_;
+    c.partBlocks = partBlocks_;
+    return c;
+}
+
+int WorldSet::numParticles(int w) {
+    DevCounters c{};
+    check_hip(hipMemcpy(&c, dCtr_ + w, sizeof(c), hipMemcpyDeviceToHost), "D2H");
+    return c.numParticles;
+}
+
+uint64_t WorldSet::timestep(int w) {
+    DevCounters c{};
+    check_hip(hipMemcpy(&c, dCtr_ + w, sizeof(c), hipMemcpyDeviceToHost), "D2H");
+    return c.timestep;
+}
+
+void WorldSet::setTimestep(int w, uint64_t t) {
+    synchronize();
+    DevCounters c{};
+    check_hip(hipMemcpy(&c, dCtr_ + w, sizeof(c), hipMemcpyDeviceToHost), "D2H");
+    c.timestep = t;
+    check_hip(hipMemcpy(dCtr_ + w, &c, sizeof(c), hipMemcpyHostToDevice), "H2D");
+}
+
+void WorldSet::setParams(int w, const alien_params& p) {
+    validate_world(hosts_.at(w).desc, p);
+    const bool geometryChanges = bin_size_for(p) != bin_size_for(hosts_.at(w).params);
+    if (geometryChanges) {
+        // bin grid changes: rebuild the layout around the new parameters
+        std::vector<WorldImage> imgs(worlds_.size());
+        for (int i = 0; i < numWorlds(); ++i) extract(i, imgs[i]);
+        imgs[w].host.params = p;
+        imgs[w].densityResetPending = true;
+        std::vector<Source> src(imgs.size());
+        for (size_t i = 0; i < imgs.size(); ++i) src[i].image = &imgs[i];
+        build(src);
+        return;
+    }
+    hosts_.at(w).params = p;
+    params_.at(w) = to_dev_params(p);
+    check_hip(hipMemcpy(dParams_ + w, &params_[w], sizeof(DevParams), hipMemcpyHostToDevice), "H2D");
+    densityPending_.at(w) = 1;  // prepareForSimulationParametersChanges, SimulationCudaFacade.cu:667-669
+}
+
+void WorldSet::checkErrors() {
+    StepCounters sc{};
+    check_hip(hipMemcpy(&sc, dSc_, sizeof(sc), hipMemcpyDeviceToHost), "D2H");
+    if (sc.errors & 1) {
+        sc.errors = 0;
+        check_hip(hipMemcpy(dSc_, &sc, sizeof(sc), hipMemcpyHostToDevice), "H2D");
+        throw Error(ALIEN_E_CAPACITY, "fusion candidate list overflow");
+    }
+}
+
+void WorldSet::step(uint64_t n) {
+    if (worlds_.empty() || n == 0) return;
+    // density reset requested by a parameter change, applied before the next step
+    for (int w = 0; w < numWorlds(); ++w) {
+        if (!densityPending_[w]) continue;
+        const DevWorld& dw = worlds_[w];
+        launch_fill_u32(reinterpret_cast<uint32_t*>(buf_[parity_].dens + dw.cellOff), 0x3f800000u, size_t(dw.numCells),
+                        stream_);
+        densityPending_[w] = 0;
+    }
+    for (uint64_t s = 0; s < n; ++s) {
+        if (useGraph_) {
+            hipGraphExec_t& g = graph_[parity_];
+            if (!g) {
+                hipGraph_t graph = nullptr;
+                check_hip(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+                launch_step(makeCtx(parity_), shape_, stream_);
+                check_hip(hipStreamEndCapture(stream_, &graph), "hipStreamEndCapture");
+                check_hip(hipGraphInstantiate(&g, graph, nullptr, nullptr, 0), "hipGraphInstantiate");
+                (void)hipGraphDestroy(graph);
+            }
+            check_hip(hipGraphLaunch(g, stream_), "hipGraphLaunch");
+        } else {
+            launch_step(makeCtx(parity_), shape_, stream_);
+            check_hip(hipGetLastError(), "kernel launch");
+        }
+        parity_ ^= 1;
+    }
+    for (auto& h : hosts_) h.stepsSinceSet += n;
+    synchronize();
+    checkErrors();
+}
+
+void WorldSet::extract(int wi, WorldImage& img) {
+    synchronize();
+    const DevWorld& w = worlds_.at(wi);
+    DevCounters ctr{};
+    check_hip(hipMemcpy(&ctr, dCtr_ + wi, sizeof(ctr), hipMemcpyDeviceToHost), "D2H");
+    const int n = w.numCells, np = ctr.numParticles;
+    img.host = hosts_.at(wi);
+    img.timestep = ctr.timestep;
+    img.numParticles = np;
+    img.densityResetPending = densityPending_.at(wi) != 0;
+    const HotBuffers& S = buf_[parity_];
+    auto d2h = [&](void* dst, const void* src, size_t bytes) {
+        if (bytes) check_hip(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost), "D2H");
+    };
+    img.pos.resize(n);
+    img.vel.resize(n);
+    img.dens.resize(n);
+    img.flags.resize(n);
+    img.slot.resize(n);
+    img.btgt.resize(size_t(kMaxBonds) * n);
+    img.brest.resize(size_t(kMaxBonds) * n);
+    img.bang.resize(size_t(kMaxBonds) * n);
+    img.energy.resize(n);
+    img.stiff.resize(n);
+    d2h(img.pos.data(), S.pos + w.cellOff, sizeof(float2) * n);
+    d2h(img.vel.data(), S.vel + w.cellOff, sizeof(float2) * n);
+    d2h(img.dens.data(), S.dens + w.cellOff, sizeof(float) * n);
+    d2h(img.flags.data(), S.flags + w.cellOff, sizeof(uint32_t) * n);
+    d2h(img.slot.data(), S.slot + w.cellOff, sizeof(int) * n);
+    for (int b = 0; b < kMaxBonds; ++b) {
+        d2h(img.btgt.data() + size_t(b) * n, S.btgt + size_t(b) * cellCap_ + w.cellOff, sizeof(int) * n);
+        d2h(img.brest.data() + size_t(b) * n, S.brest + size_t(b) * cellCap_ + w.cellOff, sizeof(float) * n);
+        d2h(img.bang.data() + size_t(b) * n, S.bang + size_t(b) * cellCap_ + w.cellOff, sizeof(float) * n);
+    }
+    d2h(img.energy.data(), energy_ + w.cellOff, sizeof(float) * n);
+    d2h(img.stiff.data(), stiff_ + w.cellOff, sizeof(float) * n);
+    img.ppos.resize(np);
+    img.pvel.resize(np);
+    img.pen.resize(np);
+    img.plast.resize(np);
+    img.pslot.resize(np);
+    img.pflags.resize(np);
+    d2h(img.ppos.data(), S.ppos + w.partOff, sizeof(float2) * np);
+    d2h(img.pvel.data(), S.pvel + w.partOff, sizeof(float2) * np);
+    d2h(img.pen.data(), S.pen + w.partOff, sizeof(float) * np);
+    d2h(img.plast.data(), S.plast + w.partOff, sizeof(int) * np);
+    d2h(img.pslot.data(), S.pslot + w.partOff, sizeof(int) * np);
+    d2h(img.pflags.data(), S.pflags + w.partOff, np);
+    // particles killed in the last step are still in the live range until the next sort
+    int keep = 0;
+    for (int i = 0; i < np; ++i) {
+        if (img.pflags[i] & kParticleDead) continue;
+        img.ppos[keep] = img.ppos[i];
+        img.pvel[keep] = img.pvel[i];
+        img.pen[keep] = img.pen[i];
+        img.plast[keep] = img.plast[i];
+        img.pslot[keep] = img.pslot[i];
+        img.pflags[keep] = img.pflags[i];
+        ++keep;
+    }
+    img.ppos.resize(keep);
+    img.pvel.resize(keep);
+    img.pen.resize(keep);
+    img.plast.resize(keep);
+    img.pslot.resize(keep);
+    img.pflags.resize(keep);
+    img.numParticles = keep;
+}
+
+alien_totals WorldSet::totals(int wi) {
+    WorldImage img;
+    extract(wi, img);
+    alien_totals t{};
+    for (int i = 0; i < img.host.numCells; ++i) {
+        t.cellEnergy += double(img.energy[img.slot[i]]);
+        t.momentum[0] += double(img.vel[i].x);
+        t.momentum[1] += double(img.vel[i].y);
+        t.kinetic += 0.5 * (double(img.vel[i].x) * double(img.vel[i].x) + double(img.vel[i].y) * double(img.vel[i].y));
+    }
+    for (int i = 0; i < img.numParticles; ++i) t.particleEnergy += double(img.pen[i]);
+    t.numCells = uint64_t(img.host.numCells);
+    t.numParticles = uint64_t(img.numParticles);
+    return t;
+}
+
+void WorldSet::cellMap(int wi, std::vector<int>& bins, std::vector<int>& slots) {
+    synchronize();
+    const Ctx c = makeCtx(parity_);
+    StepCounters sc{};
+    check_hip(hipMemcpy(dSc_, &sc, sizeof(sc), hipMemcpyHostToDevice), "H2D");
+    launch_sort(c, shape_, stream_);
+    check_hip(hipGetLastError(), "sort");
+    synchronize();
+    const DevWorld& w = worlds_.at(wi);
+    const int n = w.numCells;
+    std::vector<int> sorted(n), binOfAll(cellCap_), slotAll(cellCap_);
+    if (n) {
+        check_hip(hipMemcpy(sorted.data(), sorted_ + w.cellOff, sizeof(int) * n, hipMemcpyDeviceToHost), "D2H");
+        check_hip(hipMemcpy(binOfAll.data(), binOf_, sizeof(int) * cellCap_, hipMemcpyDeviceToHost), "D2H");
+        check_hip(hipMemcpy(slotAll.data(), buf_[parity_].slot, sizeof(int) * cellCap_, hipMemcpyDeviceToHost), "D2H");
+    }
+    bins.resize(n);
+    slots.resize(n);
+    for (int k = 0; k < n; ++k) {
+        bins[k] = binOfAll[sorted[k]] - w.binOff;
+        slots[k] = slotAll[sorted[k]];
+    }
+}
+
+void WorldSet::neighbors(int wi, std::vector<std::vector<int>>& bySlot) {
+    synchronize();
+    const Ctx c = makeCtx(parity_);
+    launch_sort(c, shape_, stream_);
+    int* counts = nullptr;
+    int* out = nullptr;
+    uint64_t* offs = nullptr;
+    check_hip(hipMalloc(&counts, sizeof(int) * cellCap_), "hipMalloc");
+    launch_debug_neighbors(c, shape_, nullptr, nullptr, counts, stream_);
+    check_hip(hipGetLastError(), "debug neighbours");
+    synchronize();
+    std::vector<int> cnt(cellCap_);
+    check_hip(hipMemcpy(cnt.data(), counts, sizeof(int) * cellCap_, hipMemcpyDeviceToHost), "D2H");
+    std::vector<uint64_t> off(size_t(cellCap_) + 1, 0);
+    for (int i = 0; i < cellCap_; ++i) off[i + 1] = off[i] + uint64_t(std::max(cnt[i], 0));
+    check_hip(hipMalloc(&offs, sizeof(uint64_t) * off.size()), "hipMalloc");
+    check_hip(hipMalloc(&out, sizeof(int) * std::max<uint64_t>(off.back(), 1)), "hipMalloc");
+    check_hip(hipMemcpy(offs, off.data(), sizeof(uint64_t) * off.size(), hipMemcpyHostToDevice), "H2D");
+    launch_debug_neighbors(c, shape_, offs, out, nullptr, stream_);
+    check_hip(hipGetLastError(), "debug neighbours");
+    synchronize();
+    std::vector<int> ids(off.back());
+    if (!ids.empty())
+        check_hip(hipMemcpy(ids.data(), out, sizeof(int) * ids.size(), hipMemcpyDeviceToHost), "D2H");
+    std::vector<int> slotAll(cellCap_);
+    check_hip(hipMemcpy(slotAll.data(), buf_[parity_].slot, sizeof(int) * cellCap_, hipMemcpyDeviceToHost), "D2H");
+    (void)hipFree(counts);
+    (void)hipFree(out);
+    (void)hipFree(offs);
+    const DevWorld& w = worlds_.at(wi);
+    bySlot.assign(w.numCells, {});
+    for (int i = w.cellOff; i < w.cellOff + w.numCells; ++i) {
+        auto& v = bySlot[slotAll[i]];
+        for (uint64_t q = off[i]; q < off[i + 1]; ++q) v.push_back(slotAll[ids[q]]);
+    }
+}
+
+}  // namespace alien

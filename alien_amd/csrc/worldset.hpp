@@ -1,0 +1,168 @@
+// worldset.hpp — host engine of a set of independent worlds resident in device memory.
+//
+// Plays the role of ALIEN's device seam _SimulationCudaFacade (SimulationCudaFacade.cuh:27-139)
+// and its SimulationData (SimulationData.cuh:17-54): owns device state, converts transfer objects
+// to the device layout and back (DataAccessKernels.cu, DescriptionConverter.cpp), and runs the
+// timestep loop (SimulationCudaFacade.cu:128-160).  Many worlds share one set and one launch
+// sequence; a single simulation is a set of one world.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/alien_hip.h"
+#include "engine_types.hpp"
+#include "kernels.hpp"
+
+namespace alien {
+
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+void check_hip(hipError_t e, const char* what);
+
+int bin_size_for(const alien_params& p);
+void validate_params(const alien_params& p);
+void validate_world(const alien_world_desc& d, const alien_params& p);
+DevParams to_dev_params(const alien_params& p);
+
+// host-side identity and pass-through data of one world (fields the physics path never touches)
+struct HostWorld {
+    alien_world_desc desc{};
+    alien_params params{};
+    int numCells = 0;
+    int particleCap = 0;
+    uint64_t stepsSinceSet = 0;
+    std::vector<uint64_t> cellId;
+    std::vector<uint32_t> age, creatureId, mutationId, activationTime;
+    std::vector<uint8_t> livingState, cellFunction, ancestorMutationId, selected;
+    std::vector<float> genomeComplexity;
+    std::vector<uint64_t> particleId;
+    std::vector<uint8_t> particleSelected;
+};
+
+// complete state of one world in the device layout, staged in host memory
+struct WorldImage {
+    HostWorld host;
+    uint64_t timestep = 0;
+    int numParticles = 0;
+    bool densityResetPending = false;
+    std::vector<float2> pos, vel;
+    std::vector<float> dens;
+    std::vector<uint32_t> flags;
+    std::vector<int> slot;
+    std::vector<int> btgt;  // [kMaxBonds][numCells]
+    std::vector<float> brest, bang;
+    std::vector<float> energy, stiff;  // by slot
+    std::vector<float2> ppos, pvel;
+    std::vector<float> pen;
+    std::vector<int> plast, pslot;
+    std::vector<uint8_t> pflags;
+
+    // from transfer objects (ObjectFactory.cuh:46-86: positions wrapped, density 1)
+    static WorldImage fromTO(const alien_world_desc& desc, const alien_params& params, const alien_cell_to* cells,
+                             uint64_t numCells, const alien_particle_to* particles, uint64_t numParticles);
+    // to transfer objects, cells and particles in ascending id order
+    void toTO(std::vector<alien_cell_to>& cells, std::vector<alien_particle_to>& particles) const;
+};
+
+class WorldSet {
+public:
+    WorldSet();
+    ~WorldSet();
+    WorldSet(const WorldSet&) = delete;
+    WorldSet& operator=(const WorldSet&) = delete;
+
+    // A world of the new layout comes either from a host image or from a world of another set
+    // (copied device to device).
+    struct Source {
+        const WorldImage* image = nullptr;
+        WorldSet* from = nullptr;
+        int fromIndex = -1;
+    };
+    void build(const std::vector<Source>& sources);
+    void clearAll();
+
+    int numWorlds() const { return int(hosts_.size()); }
+    HostWorld& host(int w) { return hosts_.at(w); }
+    const HostWorld& host(int w) const { return hosts_.at(w); }
+    int numCells(int w) const { return worlds_.at(w).numCells; }
+    int numParticles(int w);
+    uint64_t timestep(int w);
+    void setTimestep(int w, uint64_t t);
+    void setParams(int w, const alien_params& p);
+
+    // advance every world by n timesteps
+    void step(uint64_t n);
+    void extract(int w, WorldImage& out);
+    alien_totals totals(int w);
+    // cell map of the current state: bin of each cell in map order, and its slot
+    void cellMap(int w, std::vector<int>& bins, std::vector<int>& slots);
+    // neighbour lists under the step predicate: per slot, neighbour slots in visiting order
+    void neighbors(int w, std::vector<std::vector<int>>& bySlot);
+
+    void setBlockSize(int threads);
+    int blockSize() const { return shape_.cellThreads; }
+    void setUseGraph(bool on);
+    size_t deviceBytes() const { return deviceBytes_; }
+    static uint64_t estimateBytes(uint64_t cells, uint64_t particles, int W, int H, int binSize);
+    void synchronize();
+    hipStream_t stream() const { return stream_; }
+
+private:
+    struct Arr {
+        void* p = nullptr;
+        size_t bytes = 0;
+    };
+    void* alloc(size_t bytes);
+    void freeAll();
+    void buildTables();
+    void invalidateGraphs();
+    Ctx makeCtx(int parity) const;
+    void ensureSorted();
+    void checkErrors();
+
+    hipStream_t stream_ = nullptr;
+    std::vector<HostWorld> hosts_;
+    std::vector<DevWorld> worlds_;
+    std::vector<DevParams> params_;
+    std::vector<uint8_t> densityPending_;
+    std::vector<Arr> allocs_;
+    size_t deviceBytes_ = 0;
+    int cellCap_ = 0, partCap_ = 0, numBins_ = 0, candCap_ = 0;
+    int parity_ = 0;  // index of the buffer holding the current state
+    LaunchShape shape_;
+    bool useGraph_ = true;
+
+    // device memory
+    DevWorld* dWorlds_ = nullptr;
+    DevParams* dParams_ = nullptr;
+    DevCounters* dCtr_ = nullptr;
+    StepCounters* dSc_ = nullptr;
+    HotBuffers buf_[2]{};
+    float* energy_ = nullptr;
+    float* stiff_ = nullptr;
+    int *binOf_ = nullptr, *rank_ = nullptr, *pbinOf_ = nullptr, *prank_ = nullptr;
+    uint32_t *count_ = nullptr, *start_ = nullptr, *blockSums_ = nullptr;
+    int *sorted_ = nullptr, *newidx_ = nullptr, *psorted_ = nullptr;
+    int2* staged_ = nullptr;
+    float2 *own_ = nullptr, *s1_ = nullptr, *s2_ = nullptr, *out1_ = nullptr, *out2_ = nullptr;
+    uint32_t* valid_ = nullptr;
+    uint8_t *delFlag_ = nullptr, *mark_ = nullptr;
+    unsigned long long* claim_ = nullptr;
+    int4* cand_ = nullptr;
+    int *want_ = nullptr, *fclaim_ = nullptr, *absorbCell_ = nullptr, *cclaim_ = nullptr;
+    int2 *cellBlocks_ = nullptr, *binBlocks_ = nullptr, *pbinBlocks_ = nullptr, *partBlocks_ = nullptr;
+    std::vector<Arr> tableAllocs_;
+
+    hipGraphExec_t graph_[2] = {nullptr, nullptr};
+};
+
+}  // namespace alien

@@ -1,0 +1,113 @@
+// pymodule.cpp — Python binding (pybind11) of the C++ facade, module alien_amd._facade.
+#include <pybind11/numpy.h>
+#include <pybind11/pybind11.h>
+#include <pybind11/stl.h>
+
+#include <cstring>
+
+#include "simulation_facade.hpp"
+
+namespace py = pybind11;
+using alien::SimulationFacade;
+
+namespace {
+template <class T> std::vector<T> fromArray(py::array a) {
+    py::buffer_info bi = a.request();
+    if (bi.ndim != 1 || size_t(bi.itemsize) != sizeof(T))
+        throw std::invalid_argument("array does not have the transfer-object layout");
+    std::vector<T> v(size_t(bi.shape[0]));
+    if (!v.empty()) std::memcpy(v.data(), bi.ptr, v.size() * sizeof(T));
+    return v;
+}
+template <class T> py::bytes toBytes(const std::vector<T>& v) {
+    return py::bytes(reinterpret_cast<const char*>(v.data()), v.size() * sizeof(T));
+}
+alien_params paramsFromBytes(py::bytes b) {
+    std::string s = b;
+    if (s.size() != sizeof(alien_params)) throw std::invalid_argument("params must be an alien_params blob");
+    alien_params p;
+    std::memcpy(&p, s.data(), sizeof(p));
+    return p;
+}
+}  // namespace
+
+PYBIND11_MODULE(_facade, m) {
+    m.doc() = "C++ SimulationFacade (physics subset of ALIEN's _SimulationFacade) over the HIP C ABI";
+    py::register_exception<alien::MemoryAllocationException>(m, "MemoryAllocationException", PyExc_MemoryError);
+    py::class_<SimulationFacade>(m, "SimulationFacade")
+        .def(py::init<>())
+        .def("newSimulation",
+             [](SimulationFacade& f, uint64_t ts, int sx, int sy, py::bytes params) {
+                 alien::GeneralSettings g;
+                 g.worldSizeX = sx;
+                 g.worldSizeY = sy;
+                 py::gil_scoped_release rel;
+                 f.newSimulation(ts, g, paramsFromBytes(params));
+             })
+        .def("setSimulationData",
+             [](SimulationFacade& f, py::array cells, py::array particles) {
+                 alien::SimulationData d;
+                 d.cells = fromArray<alien_cell_to>(cells);
+                 d.particles = fromArray<alien_particle_to>(particles);
+                 py::gil_scoped_release rel;
+                 f.setSimulationData(d);
+             })
+        .def("getSimulationData",
+             [](SimulationFacade& f) {
+                 alien::SimulationData d;
+                 {
+                     py::gil_scoped_release rel;
+                     d = f.getSimulationData();
+                 }
+                 return py::make_tuple(toBytes(d.cells), toBytes(d.particles));
+             })
+        .def("calcTimesteps",
+             [](SimulationFacade& f, uint64_t n) {
+                 py::gil_scoped_release rel;
+                 f.calcTimesteps(n);
+             })
+        .def("runSimulation", &SimulationFacade::runSimulation)
+        .def("pauseSimulation", &SimulationFacade::pauseSimulation, py::call_guard<py::gil_scoped_release>())
+        .def("isSimulationRunning", &SimulationFacade::isSimulationRunning)
+        .def("closeSimulation", &SimulationFacade::closeSimulation, py::call_guard<py::gil_scoped_release>())
+        .def("clear", &SimulationFacade::clear, py::call_guard<py::gil_scoped_release>())
+        .def("getCurrentTimestep", &SimulationFacade::getCurrentTimestep, py::call_guard<py::gil_scoped_release>())
+        .def("setCurrentTimestep", &SimulationFacade::setCurrentTimestep, py::call_guard<py::gil_scoped_release>())
+        .def("getSimulationParameters",
+             [](SimulationFacade& f) {
+                 alien_params p = f.getSimulationParameters();
+                 return py::bytes(reinterpret_cast<const char*>(&p), sizeof(p));
+             })
+        .def("setSimulationParameters",
+             [](SimulationFacade& f, py::bytes b) {
+                 alien_params p = paramsFromBytes(b);
+                 py::gil_scoped_release rel;
+                 f.setSimulationParameters(p);
+             })
+        .def("getWorldSize",
+             [](SimulationFacade& f) {
+                 auto s = f.getWorldSize();
+                 return py::make_tuple(s.x, s.y);
+             })
+        .def("getGpuName", &SimulationFacade::getGpuName)
+        .def("getTps", &SimulationFacade::getTps)
+        .def("getRawStatistics",
+             [](SimulationFacade& f) {
+                 alien_totals t;
+                 {
+                     py::gil_scoped_release rel;
+                     t = f.getRawStatistics();
+                 }
+                 py::dict d;
+                 d["cellEnergy"] = t.cellEnergy;
+                 d["particleEnergy"] = t.particleEnergy;
+                 d["momentum"] = py::make_tuple(t.momentum[0], t.momentum[1]);
+                 d["kinetic"] = t.kinetic;
+                 d["numCells"] = t.numCells;
+                 d["numParticles"] = t.numParticles;
+                 return d;
+             })
+        .def("testOnly_failNextWorkerStep", &SimulationFacade::testOnly_failNextWorkerStep)
+        .def("testOnly_setGuardTimeoutMs",
+             [](SimulationFacade& f, int ms) { f.testOnly_setGuardTimeout(std::chrono::milliseconds(ms)); });
+}

@@ -1,0 +1,232 @@
+// simulation_facade.cpp — see simulation_facade.hpp.
+#include "simulation_facade.hpp"
+
+#include <cstring>
+
+namespace alien {
+
+void SimulationFacade::check(int rc) {
+    if (rc == ALIEN_OK) return;
+    std::string msg = alien_last_error();
+    if (rc == ALIEN_E_OOM) throw MemoryAllocationException(msg);
+    throw std::runtime_error(msg);
+}
+
+// Parks the worker thread for the duration of an API call (EngineWorkerGuard, EngineWorker.cpp:
+// 608-632).  Throws if the worker does not answer within the timeout.
+class SimulationFacade::Guard {
+public:
+    explicit Guard(SimulationFacade& f) : _f(f), _lock(f._accessMutex) {
+        std::unique_lock<std::mutex> lk(_f._stateMutex);
+        _f._accessState = 1;
+        _f._stateCv.notify_all();
+        if (!_f._stateCv.wait_for(lk, _f._guardTimeout, [&] { return _f._accessState == 2 || !_f._worker.joinable(); })) {
+            _f._accessState = 0;
+            throw std::runtime_error("GPU worker thread is not reachable.");
+        }
+    }
+    ~Guard() {
+        std::lock_guard<std::mutex> lk(_f._stateMutex);
+        _f._accessState = 0;
+        _f._stateCv.notify_all();
+    }
+
+private:
+    SimulationFacade& _f;
+    std::unique_lock<std::mutex> _lock;
+};
+
+SimulationFacade::SimulationFacade() {}
+
+SimulationFacade::~SimulationFacade() { closeSimulation(); }
+
+void SimulationFacade::workerLoop() {
+    auto last = std::chrono::steady_clock::now();
+    uint64_t stepsSince = 0;
+    while (true) {
+        {
+            std::unique_lock<std::mutex> lk(_stateMutex);
+            _stateCv.wait(lk, [&] { return _stop || _accessState == 1 || (_running && !_workerError); });
+            if (_stop) return;
+            if (_accessState == 1) {
+                _accessState = 2;  // parked: the caller has exclusive access
+                _stateCv.notify_all();
+                _stateCv.wait(lk, [&] { return _stop || _accessState == 0; });
+                continue;
+            }
+        }
+        // running mode: one timestep at a time (EngineWorker.cpp:450-476)
+        try {
+            if (auto m = takeInjectedFailure()) throw std::runtime_error(*m);
+            check(alien_step(_world, 1));
+            ++stepsSince;
+            auto now = std::chrono::steady_clock::now();
+            double dt = std::chrono::duration<double>(now - last).count();
+            if (dt > 0.5) {
+                _tps = float(double(stepsSince) / dt);
+                stepsSince = 0;
+                last = now;
+            }
+        } catch (...) {
+            std::lock_guard<std::mutex> lk(_stateMutex);
+            _workerError = std::current_exception();
+            _running = false;
+        }
+    }
+}
+
+std::optional<std::string> SimulationFacade::takeInjectedFailure() {
+    std::lock_guard<std::mutex> lk(_stateMutex);
+    auto m = _injectedFailure;
+    _injectedFailure.reset();
+    return m;
+}
+
+void SimulationFacade::throwIfWorkerFailed() {
+    std::lock_guard<std::mutex> lk(_stateMutex);
+    if (_workerError) throw std::runtime_error("GPU worker thread is in an invalid state.");
+}
+
+void SimulationFacade::newSimulation(uint64_t timestep, GeneralSettings const& generalSettings,
+                                     alien_params const& parameters) {
+    closeSimulation();
+    alien_world_desc desc{};
+    desc.worldSizeX = generalSettings.worldSizeX;
+    desc.worldSizeY = generalSettings.worldSizeY;
+    alien_world* w = nullptr;
+    check(alien_world_create(&desc, &parameters, &w));
+    _world = w;
+    check(alien_set_timestep(_world, timestep));
+    _settings = generalSettings;
+    _params = parameters;
+    _stop = false;
+    _running = false;
+    _workerError = nullptr;
+    _accessState = 0;
+    _worker = std::thread([this] { workerLoop(); });
+}
+
+void SimulationFacade::closeSimulation() {
+    if (_worker.joinable()) {
+        {
+            std::lock_guard<std::mutex> lk(_stateMutex);
+            _stop = true;
+        }
+        _stateCv.notify_all();
+        _worker.join();
+    }
+    if (_world) {
+        alien_world_destroy(_world);
+        _world = nullptr;
+    }
+}
+
+void SimulationFacade::clear() {
+    Guard g(*this);
+    throwIfWorkerFailed();
+    check(alien_clear(_world));
+}
+
+std::string SimulationFacade::getGpuName() const {
+    char buf[256];
+    check(alien_get_device_name(buf, sizeof(buf)));
+    return buf;
+}
+
+SimulationData SimulationFacade::getSimulationData() {
+    Guard g(*this);
+    throwIfWorkerFailed();
+    SimulationData d;
+    uint64_t n = 0, p = 0;
+    check(alien_get_counts(_world, &n, &p));
+    d.cells.resize(n);
+    d.particles.resize(p);
+    check(alien_get_data(_world, d.cells.data(), n, &n, d.particles.data(), p, &p));
+    d.cells.resize(n);
+    d.particles.resize(p);
+    return d;
+}
+
+void SimulationFacade::setSimulationData(SimulationData const& data) {
+    Guard g(*this);
+    check(alien_set_data(_world, data.cells.data(), data.cells.size(), data.particles.data(), data.particles.size()));
+    std::lock_guard<std::mutex> lk(_stateMutex);
+    _workerError = nullptr;  // fresh data makes the engine usable again
+}
+
+void SimulationFacade::calcTimesteps(uint64_t timesteps) {
+    Guard g(*this);
+    throwIfWorkerFailed();
+    try {
+        if (auto m = takeInjectedFailure()) throw std::runtime_error(*m);
+        check(alien_step(_world, timesteps));
+    } catch (...) {
+        // a failed step leaves the engine unusable until new data is set (EngineWorker.cpp:472-475)
+        std::lock_guard<std::mutex> lk(_stateMutex);
+        _workerError = std::current_exception();
+        throw;
+    }
+}
+
+void SimulationFacade::runSimulation() {
+    throwIfWorkerFailed();
+    std::lock_guard<std::mutex> lk(_stateMutex);
+    _running = true;
+    _stateCv.notify_all();
+}
+
+void SimulationFacade::pauseSimulation() {
+    Guard g(*this);
+    std::lock_guard<std::mutex> lk(_stateMutex);
+    _running = false;
+}
+
+bool SimulationFacade::isSimulationRunning() const {
+    auto& self = const_cast<SimulationFacade&>(*this);
+    std::lock_guard<std::mutex> lk(self._stateMutex);
+    return _running;
+}
+
+uint64_t SimulationFacade::getCurrentTimestep() {
+    Guard g(*this);
+    throwIfWorkerFailed();
+    return alien_get_timestep(_world);
+}
+
+void SimulationFacade::setCurrentTimestep(uint64_t value) {
+    Guard g(*this);
+    throwIfWorkerFailed();
+    check(alien_set_timestep(_world, value));
+}
+
+alien_params SimulationFacade::getSimulationParameters() const { return _params; }
+
+void SimulationFacade::setSimulationParameters(alien_params const& parameters) {
+    Guard g(*this);
+    throwIfWorkerFailed();
+    check(alien_set_params(_world, &parameters));
+    _params = parameters;
+}
+
+GeneralSettings SimulationFacade::getGeneralSettings() const { return _settings; }
+
+IntVector2D SimulationFacade::getWorldSize() const { return {_settings.worldSizeX, _settings.worldSizeY}; }
+
+alien_totals SimulationFacade::getRawStatistics() {
+    Guard g(*this);
+    throwIfWorkerFailed();
+    alien_totals t{};
+    check(alien_get_totals(_world, &t));
+    return t;
+}
+
+float SimulationFacade::getTps() const { return _tps.load(); }
+
+void SimulationFacade::testOnly_failNextWorkerStep(std::string const& message) {
+    std::lock_guard<std::mutex> lk(_stateMutex);
+    _injectedFailure = message;
+}
+
+void SimulationFacade::testOnly_setGuardTimeout(std::chrono::milliseconds timeout) { _guardTimeout = timeout; }
+
+}  // namespace alien

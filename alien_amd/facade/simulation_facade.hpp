@@ -1,0 +1,109 @@
+// simulation_facade.hpp — C++ host engine with ALIEN's facade surface for the physics path.
+//
+// Mirrors the physics subset of _SimulationFacade (source/EngineInterface/SimulationFacade.h:13-110)
+// and the threading model of _SimulationFacadeImpl / EngineWorker
+// (source/EngineImpl/SimulationFacadeImpl.cpp, EngineWorker.cpp): one worker thread owns the
+// device world; API calls take access through a guard that parks the worker (timeout 7 s,
+// EngineWorker.cpp:608-632); an exception thrown on the worker thread is rethrown on the next
+// access as "GPU worker thread is in an invalid state" (EngineWorker.cpp:645-651).  Device work
+// goes only through the C ABI of include/alien_hip.h.
+#pragma once
+
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <cstdint>
+#include <exception>
+#include <mutex>
+#include <optional>
+#include <stdexcept>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../include/alien_hip.h"
+
+namespace alien {
+
+// GeneralSettings (source/EngineInterface/GeneralSettings.h)
+struct GeneralSettings {
+    int worldSizeX = 1000;
+    int worldSizeY = 1000;
+};
+
+struct IntVector2D {
+    int x = 0, y = 0;
+};
+
+// DataTO-shaped host data (TOs.cuh:195-236) in vectors
+struct SimulationData {
+    std::vector<alien_cell_to> cells;
+    std::vector<alien_particle_to> particles;
+};
+
+// CudaMemoryAllocationException analog (Base/Exceptions.h:15-20)
+struct MemoryAllocationException : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+class SimulationFacade {
+public:
+    SimulationFacade();
+    ~SimulationFacade();
+    SimulationFacade(const SimulationFacade&) = delete;
+    SimulationFacade& operator=(const SimulationFacade&) = delete;
+
+    void newSimulation(uint64_t timestep, GeneralSettings const& generalSettings, alien_params const& parameters);
+    void clear();
+    std::string getGpuName() const;
+
+    SimulationData getSimulationData();
+    void setSimulationData(SimulationData const& data);
+
+    void calcTimesteps(uint64_t timesteps);
+    void runSimulation();
+    void pauseSimulation();
+    bool isSimulationRunning() const;
+    void closeSimulation();
+
+    uint64_t getCurrentTimestep();
+    void setCurrentTimestep(uint64_t value);
+
+    alien_params getSimulationParameters() const;
+    void setSimulationParameters(alien_params const& parameters);
+
+    GeneralSettings getGeneralSettings() const;
+    IntVector2D getWorldSize() const;
+    alien_totals getRawStatistics();
+    float getTps() const;
+
+    // test hooks: make the worker thread fail on its next step (failure detection path), and set
+    // the guard timeout
+    void testOnly_failNextWorkerStep(std::string const& message);
+    void testOnly_setGuardTimeout(std::chrono::milliseconds timeout);
+
+private:
+    class Guard;
+    void workerLoop();
+    void throwIfWorkerFailed();
+    std::optional<std::string> takeInjectedFailure();
+    static void check(int rc);
+
+    alien_world* _world = nullptr;
+    GeneralSettings _settings;
+    alien_params _params{};
+
+    std::thread _worker;
+    mutable std::mutex _accessMutex;  // serializes API callers (EngineWorkerGuard)
+    std::mutex _stateMutex;
+    std::condition_variable _stateCv;
+    int _accessState = 0;  // 0 worker free, 1 access requested, 2 worker parked
+    bool _running = false;
+    bool _stop = false;
+    std::exception_ptr _workerError;
+    std::optional<std::string> _injectedFailure;
+    std::chrono::milliseconds _guardTimeout{7000};
+    std::atomic<float> _tps{0.0f};
+};
+
+}  // namespace alien
