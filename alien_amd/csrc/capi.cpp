@@ -291,6 +291,36 @@ int alien_debug_neighbors(alien_world* w, uint64_t* offsets, uint64_t offsetsCap
     });
 }
 
+static void copyTimes(WorldSet& set, alien_kernel_time* out, int32_t cap, int32_t* count) {
+    auto times = set.takeKernelTimes();
+    int32_t n = 0;
+    for (auto& kv : times) {
+        if (n < cap && out) {
+            std::memset(&out[n], 0, sizeof(alien_kernel_time));
+            std::strncpy(out[n].name, kv.first.c_str(), sizeof(out[n].name) - 1);
+            out[n].launches = kv.second.launches;
+            out[n].totalMs = kv.second.totalMs;
+        }
+        ++n;
+    }
+    if (count) *count = n;
+    if (n > cap) throw Error(ALIEN_E_CAPACITY, "timing buffer too small (count was written)");
+}
+
+int alien_set_kernel_timing(alien_world* w, int32_t enable) {
+    return guarded([&] {
+        requireHealthy(w);
+        w->set->setKernelTiming(enable != 0);
+    });
+}
+
+int alien_get_kernel_timing(alien_world* w, alien_kernel_time* out, int32_t cap, int32_t* count) {
+    return guarded([&] {
+        requireHealthy(w);
+        copyTimes(*w->set, out, cap, count);
+    });
+}
+
 // ---- batched worlds ---------------------------------------------------------------------------
 }  // extern "C"
 
@@ -300,11 +330,13 @@ struct alien_batch {
     int resident = 0;
     int blockSize = 256;
     bool useGraph = true;
+    bool timing = false;
     void flush() {
         if (pending.empty()) return;
         auto fresh = std::make_unique<WorldSet>();
         fresh->setBlockSize(blockSize);
         fresh->setUseGraph(useGraph);
+        fresh->setKernelTiming(timing);
         std::vector<WorldSet::Source> src;
         for (int i = 0; i < resident; ++i) {
             WorldSet::Source s;
@@ -398,6 +430,25 @@ int alien_batch_set_block_size(alien_batch* b, int32_t threads) {
             throw Error(ALIEN_E_ARG, "block size must be 64, 128, 192 or 256");
         b->blockSize = threads;
         if (b->set) b->set->setBlockSize(threads);
+    });
+}
+
+int alien_batch_set_kernel_timing(alien_batch* b, int32_t enable) {
+    return guarded([&] {
+        require(b != nullptr, "null batch");
+        b->timing = enable != 0;
+        if (b->set) b->set->setKernelTiming(b->timing);
+    });
+}
+
+int alien_batch_get_kernel_timing(alien_batch* b, alien_kernel_time* out, int32_t cap, int32_t* count) {
+    return guarded([&] {
+        require(b != nullptr, "null batch");
+        if (!b->set) {
+            if (count) *count = 0;
+            return;
+        }
+        copyTimes(*b->set, out, cap, count);
     });
 }
 

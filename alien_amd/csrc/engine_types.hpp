@@ -16,6 +16,10 @@ constexpr int kMaxBonds = 6;
 constexpr int kColors = 7;
 constexpr int kParticleBin = 8;  // edge of the energy-particle bins, in world units
 constexpr int kNone = -1;
+// neighbour-pass tiles: threads per workgroup, cells staged in LDS, halo bins per tile
+constexpr int kTileThreads = 128;
+constexpr int kTileCap = 512;
+constexpr int kMaxHaloBins = 512;
 constexpr int kClaimFree = 0x7fffffff;
 
 // cell flags word
@@ -57,6 +61,8 @@ struct DevWorld {
     int binOff;             // first cell bin in the combined bin arrays
     int pbinOff;            // first particle bin in the combined bin arrays
     int partOff, partCap;   // particle arrays
+    int tileX, tileY;       // bins per neighbour-pass tile (launch shape only; never changes results)
+    int ntx, nty;           // tiles per axis
 };
 
 // per-world device counters
@@ -106,11 +112,10 @@ struct Ctx {
     float* energy;  // cold [cellCap], indexed by cellOff + slot
     float* stiff;   // cold [cellCap]
     int* binOf;     // [cellCap] global bin of A-cell i
-    int* rank;      // [cellCap]
-    int* pbinOf;    // [partCap]
-    int* prank;     // [partCap]
-    uint32_t* count;  // [numBins]
-    uint32_t* start;  // [numBins + 1] exclusive scan
+    int* pbinOf;    // [partCap] global bin of A-particle i, -1 if dead
+    uint32_t* count;      // [numBins] bin sizes of the start-of-step state; scatter cursor
+    uint32_t* countNext;  // [numBins] bin sizes of the end-of-step state (next step's `count`)
+    uint32_t* start;      // [numBins + 1] exclusive scan
     uint32_t* blockSums;
     int* sorted;   // [cellCap] new index -> A index
     int* newidx;   // [cellCap] A index -> new index
@@ -135,6 +140,8 @@ struct Ctx {
     const int2* binBlocks;   // per block: (world, first global bin index) over cell bins
     const int2* pbinBlocks;  // per block: (world, first global bin index) over particle bins
     const int2* partBlocks;  // per block: (world, first global particle index)
+    const int2* tileBlocks;  // per block: (world, tile index), -1 for padding blocks
+    int numPartBlocks;
 };
 
 }  // namespace alien

@@ -1,0 +1,445 @@
+// k_bonds.hip — bonded cells and structural operations.
+//
+// Connection forces (CellProcessor::calcConnectionForces, source/EngineGpuKernels/CellProcessor.cuh:
+// 408-491) run twice per step around the Verlet position update (:525-544) and before the velocity
+// update (:546-559), over the staged list (cells with bonds, and barrier cells).  Angle forces that
+// the reference adds to the neighbours with float atomics are written by the owner into per-bond
+// slots and gathered by the receiver in its own bond order (R7).  Structural operations follow
+// CellConnectionProcessor (source/EngineGpuKernels/CellConnectionProcessor.cuh): fusion pairs
+// claim their cells and bond neighbours and apply only when every claim holds (R5); bond deletion
+// works from per-cell flags (R6).
+#include "kernels_common.hpp"
+#include "kernels.hpp"
+
+namespace alien {
+
+#define FOR_STAGED(q) for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x)
+
+// bond targets of the staged cells from start-of-step to end-of-step indices
+__global__ void k_remap(Ctx c) {
+    const int n = c.sc->staged;
+    const int cap = c.cellCap;
+    FOR_STAGED(q) {
+        const int2 ent = c.staged[q];
+        const int x = ent.x;
+        const DevWorld& w = c.worlds[ent.y];
+        const int nc = flags_num_conn(c.B.flags[x]);
+        for (int b = 0; b < nc; ++b) {
+            const int t = c.B.btgt[b * cap + x];
+            c.B.btgt[b * cap + x] = c.newidx[w.cellOff + t] - w.cellOff;
+        }
+    }
+}
+
+// connection forces of the staged cells: own force plus per-bond angle forces for the partners;
+// pass 1 also runs checkConnections (CellProcessor.cuh:493-523) on the new positions
+__global__ void k_conn_own(Ctx c, int pass) {
+    const int n = c.sc->staged;
+    const int cap = c.cellCap;
+    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
+        const int2 ent = c.staged[q];
+        const int x = ent.x, wi = ent.y;
+        const DevWorld& w = c.worlds[wi];
+        const DevParams& P = c.params[wi];
+        const uint32_t fx = c.B.flags[x];
+        const int nc = flags_num_conn(fx);
+        if (nc == 0 || flags_barrier(fx)) {
+            c.valid[x] = 0u;
+            continue;
+        }
+        const float Wf = float(w.W), Hf = float(w.H);
+        const bool considerAngles = (c.ctr[wi].timestep % 3ull) == 0ull;
+        const f2 px = to_f2(c.B.pos[x]);
+        const float sx = c.stiff[w.cellOff + c.B.slot[x]];
+        const float kx2 = sx * sx;
+        int tg[kMaxBonds];
+#pragma unroll
+        for (int b = 0; b < kMaxBonds; ++b) tg[b] = b < nc ? w.cellOff + c.B.btgt[b * cap + x] : -1;
+        f2 prevDisp = torus_dir(sub(to_f2(c.B.pos[tg[nc - 1]]), px), Wf, Hf);
+        f2 force = {0.0f, 0.0f};
+        uint32_t v = 0u;
+        bool tooLong = false;
+        const float maxBind = P.maxBindingDistance[flags_color(fx) % kColors];
+        for (int b = 0; b < nc; ++b) {
+            const int o = tg[b];
+            const float so = c.stiff[w.cellOff + c.B.slot[o]];
+            const float kc2 = so * so;
+            const f2 disp = torus_dir(sub(to_f2(c.B.pos[o]), px), Wf, Hf);
+            const float ad = len2(disp);
+            if (pass == 1 && ad > maxBind) tooLong = true;  // checkConnections on new positions
+            const float dev = ad - c.B.brest[b * cap + x];
+            force = add(force, dvd(mul(mul(unit(disp), dev), kx2 + kc2), 6.0f));
+            if (considerAngles && (nc > 2 || (nc == 2 && b == 0))) {
+                const int last = (b + nc - 1) % nc;
+                const int lc = tg[last];
+                const uint32_t fo = c.B.flags[o];
+                const int no = flags_num_conn(fo);
+                bool tri = false;
+                for (int j = 0; j < no; ++j)
+                    if (w.cellOff + c.B.btgt[j * cap + o] == lc) {
+                        tri = true;
+                        break;
+                    }
+                if (!tri) {
+                    const float angle = angle_of(disp);
+                    const float prevAngle = angle_of(prevDisp);
+                    const float act = sub_angle(angle, prevAngle);
+                    if (act < 0.0f) continue;
+                    const float ref = c.B.bang[b * cap + x];
+                    const float strength = fabsf(ref - act) / 2000.0f * kx2;
+                    f2 f1 = mul(dvd(unit(disp), fmaxf(len2(disp), P.minDistance)), strength);
+                    rot_cw(f1);
+                    f2 fb = mul(dvd(unit(prevDisp), fmaxf(len2(prevDisp), P.minDistance)), strength);
+                    rot_ccw(fb);
+                    if (ref < act) {
+                        f1 = mul(f1, -1.0f);
+                        fb = mul(fb, -1.0f);
+                    }
+                    if (!flags_barrier(fo)) {
+                        c.out1[b * cap + x] = to_float2(f1);
+                        v |= 1u << b;
+                    }
+                    if (!flags_barrier(c.B.flags[lc])) {
+                        c.out2[b * cap + x] = to_float2(fb);
+                        v |= 1u << (6 + b);
+                    }
+                    force = sub(force, add(f1, fb));
+                }
+            }
+            prevDisp = disp;
+        }
+        c.own[x] = to_float2(force);
+        c.valid[x] = v;
+        if (tooLong) c.delFlag[x] = 1;
+    }
+}
+
+__device__ __forceinline__ f2 gather_conn(const Ctx& c, const DevWorld& w, int x, int nc) {
+    const int cap = c.cellCap;
+    f2 acc = {0.0f, 0.0f};
+    acc = add(acc, to_f2(c.own[x]));
+    const int xl = x - w.cellOff;
+    for (int q = 0; q < nc; ++q) {
+        const int p = w.cellOff + c.B.btgt[q * cap + x];
+        const int np = flags_num_conn(c.B.flags[p]);
+        int b = -1;
+        for (int kk = 0; kk < np; ++kk)
+            if (c.B.btgt[kk * cap + p] == xl) {
+                b = kk;
+                break;
+            }
+        if (b < 0) continue;
+        const uint32_t vp = c.valid[p];
+        if (vp & (1u << b)) acc = add(acc, to_f2(c.out1[b * cap + p]));
+        const int b2 = (b + 1) % np;
+        if (vp & (1u << (6 + b2))) acc = add(acc, to_f2(c.out2[b2 * cap + p]));
+    }
+    return acc;
+}
+
+__global__ void k_conn_verlet_pos(Ctx c) {
+    const int n = c.sc->staged;
+    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
+        const int2 ent = c.staged[q];
+        const int x = ent.x, wi = ent.y;
+        const DevWorld& w = c.worlds[wi];
+        const DevParams& P = c.params[wi];
+        const uint32_t fx = c.B.flags[x];
+        const float dt = P.dt;
+        const f2 p = to_f2(c.B.pos[x]);
+        const f2 v = to_f2(c.B.vel[x]);
+        if (flags_barrier(fx)) {
+            const f2 p2 = torus_pos(add(p, mul(v, dt)), w.W, w.H);
+            c.B.pos[x] = to_float2(p2);
+            bin_for_next_step(c, w, x, p2, c.countNext);
+            continue;
+        }
+        const f2 s1 = gather_conn(c, w, x, flags_num_conn(fx));
+        const f2 p2 = torus_pos(add(p, add(mul(v, dt), dvd(mul(mul(s1, dt), dt), 2.0f))), w.W, w.H);
+        c.B.pos[x] = to_float2(p2);
+        c.s2[x] = to_float2(s1);
+        bin_for_next_step(c, w, x, p2, c.countNext);
+    }
+}
+
+__global__ void k_conn_verlet_vel(Ctx c) {
+    const int n = c.sc->staged;
+    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
+        const int2 ent = c.staged[q];
+        const int x = ent.x, wi = ent.y;
+        const DevWorld& w = c.worlds[wi];
+        const DevParams& P = c.params[wi];
+        const uint32_t fx = c.B.flags[x];
+        if (flags_barrier(fx)) continue;
+        const f2 s1 = gather_conn(c, w, x, flags_num_conn(fx));
+        const f2 acc = dvd(add(s1, to_f2(c.s2[x])), 2.0f);
+        f2 v = to_f2(c.B.vel[x]);
+        v = add(v, mul(acc, P.dt));
+        v = mul(v, 1.0f - P.friction);
+        c.B.vel[x] = to_float2(v);
+        if (c.energy[w.cellOff + c.B.slot[x]] > P.maxBindingEnergy) c.delFlag[x] = 1;  // decay, :786-790
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// structural operations (CellConnectionProcessor.cuh), R5/R6
+// ---------------------------------------------------------------------------------------------
+__global__ void k_mark(Ctx c) {
+    const int n = c.sc->staged;
+    const int cap = c.cellCap;
+    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
+        const int2 ent = c.staged[q];
+        const int x = ent.x;
+        const DevWorld& w = c.worlds[ent.y];
+        uint8_t m = c.delFlag[x];
+        const int nc = flags_num_conn(c.B.flags[x]);
+        for (int b = 0; b < nc && !m; ++b)
+            if (c.delFlag[w.cellOff + c.B.btgt[b * cap + x]]) m = 1;
+        c.mark[x] = m;
+    }
+}
+
+template <class F> __device__ __forceinline__ void for_claim_set(const Ctx& c, const DevWorld& w, int a, int b, F&& f) {
+    const int cap = c.cellCap;
+    f(a);
+    int na = flags_num_conn(c.B.flags[a]);
+    for (int k = 0; k < na; ++k) f(w.cellOff + c.B.btgt[k * cap + a]);
+    f(b);
+    int nb = flags_num_conn(c.B.flags[b]);
+    for (int k = 0; k < nb; ++k) f(w.cellOff + c.B.btgt[k * cap + b]);
+}
+
+__device__ __forceinline__ unsigned long long claim_key(const DevWorld& w, int a, int b) {
+    return (static_cast<unsigned long long>(uint32_t(a - w.cellOff)) << 32) | uint32_t(b - w.cellOff);
+}
+
+__global__ void k_fuse_claim(Ctx c) {
+    const int n = min(c.sc->candidates, c.candCap);
+    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
+        int4 cd = c.cand[q];
+        const DevWorld& w = c.worlds[cd.z];
+        if (c.mark[cd.x] || c.mark[cd.y]) {
+            c.cand[q].w = 0;
+            continue;
+        }
+        unsigned long long key = claim_key(w, cd.x, cd.y);
+        for_claim_set(c, w, cd.x, cd.y, [&](int x) { atomicMin(&c.claim[x], key); });
+    }
+}
+
+// tryAddConnectionOneWay (CellConnectionProcessor.cuh:274-427) without angle alignment
+__device__ bool would_overlap(const Ctx& c, const DevWorld& w, int c1, f2 otherPos) {
+    const int cap = c.cellCap;
+    const int n = flags_num_conn(c.B.flags[c1]);
+    if (n < 2) return false;
+    const f2 p1 = to_f2(c.B.pos[c1]);
+    for (int i = 0; i < n; ++i) {
+        const int cc = w.cellOff + c.B.btgt[i * cap + c1];
+        const int nc = w.cellOff + c.B.btgt[((i + 1) % n) * cap + c1];
+        const int ncl = nc - w.cellOff;
+        const int ncc = flags_num_conn(c.B.flags[cc]);
+        bool both = false;
+        for (int j = 0; j < ncc; ++j)
+            if (c.B.btgt[j * cap + cc] == ncl) {
+                both = true;
+                break;
+            }
+        if (!both) continue;
+        if (segments_cross(p1, otherPos, to_f2(c.B.pos[cc]), to_f2(c.B.pos[nc]))) return true;
+    }
+    return false;
+}
+
+__device__ bool add_one_way(const Ctx& c, const DevWorld& w, int c1, int c2, f2 posDelta) {
+    const int cap = c.cellCap;
+    const float Wf = float(w.W), Hf = float(w.H);
+    uint32_t f = c.B.flags[c1];
+    const int n = flags_num_conn(f);
+    if (n == kMaxBonds) return false;
+    if (would_overlap(c, w, c1, to_f2(c.B.pos[c2]))) return false;
+    const float newAngle = angle_of(posDelta);
+    const float dist = len2(posDelta);
+    const int c2l = c2 - w.cellOff;
+    const f2 p1 = to_f2(c.B.pos[c1]);
+    if (n == 0) {
+        c.B.btgt[c1] = c2l;
+        c.B.brest[c1] = dist;
+        c.B.bang[c1] = 360.0f;
+        c.B.flags[c1] = flags_with_num_conn(f, 1);
+        return true;
+    }
+    if (n == 1) {
+        const f2 cd = torus_dir(sub(to_f2(c.B.pos[w.cellOff + c.B.btgt[c1]]), p1), Wf, Hf);
+        const float prevAngle = angle_of(cd);
+        float angleDiff = newAngle - prevAngle;
+        if (angleDiff < 0.0f) angleDiff += 360.0f;
+        if (fabsf(angleDiff) < kNearZero || fabsf(angleDiff - 360.0f) < kNearZero ||
+            fabsf(angleDiff + 360.0f) < kNearZero)
+            return false;
+        c.B.bang[cap + c1] = angleDiff;
+        c.B.bang[c1] = 360.0f - angleDiff;
+        c.B.btgt[cap + c1] = c2l;
+        c.B.brest[cap + c1] = dist;
+        c.B.flags[c1] = flags_with_num_conn(f, 2);
+        return true;
+    }
+    int index = 0;
+    float prevAngle = 0.0f, nextAngle = 0.0f;
+    bool found = false;
+    for (; index < n; ++index) {
+        const int prevIndex = (index + n - 1) % n;
+        prevAngle = angle_of(torus_dir(sub(to_f2(c.B.pos[w.cellOff + c.B.btgt[prevIndex * cap + c1]]), p1), Wf, Hf));
+        nextAngle = angle_of(torus_dir(sub(to_f2(c.B.pos[w.cellOff + c.B.btgt[index * cap + c1]]), p1), Wf, Hf));
+        if (angle_between(prevAngle, nextAngle, newAngle) || prevIndex == index) {
+            found = true;
+            break;
+        }
+    }
+    if (!found) return false;
+    const float refAngle = c.B.bang[index * cap + c1];
+    const float angleDiff1 = sub_angle(newAngle, prevAngle);
+    const float angleDiff2 = sub_angle(nextAngle, prevAngle);
+    const float frac = angleDiff2 != 0.0f ? angleDiff1 / angleDiff2 : 0.5f;
+    float afp = refAngle * frac;
+    afp = fminf(afp, refAngle);
+    if (afp < kNearZero) return false;
+    if (index == 0) index = n;
+    for (int j = n; j > index; --j) {
+        c.B.btgt[j * cap + c1] = c.B.btgt[(j - 1) * cap + c1];
+        c.B.brest[j * cap + c1] = c.B.brest[(j - 1) * cap + c1];
+        c.B.bang[j * cap + c1] = c.B.bang[(j - 1) * cap + c1];
+    }
+    c.B.btgt[index * cap + c1] = c2l;
+    c.B.brest[index * cap + c1] = dist;
+    c.B.bang[index * cap + c1] = afp;
+    c.B.bang[((index + 1) % (n + 1)) * cap + c1] = refAngle - afp;
+    c.B.flags[c1] = flags_with_num_conn(f, n + 1);
+    return true;
+}
+
+__global__ void k_fuse_apply(Ctx c) {
+    const int n = min(c.sc->candidates, c.candCap);
+    const int cap = c.cellCap;
+    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
+        int4 cd = c.cand[q];
+        if (!cd.w) continue;
+        const DevWorld& w = c.worlds[cd.z];
+        const int a = cd.x, b = cd.y;
+        unsigned long long key = claim_key(w, a, b);
+        bool ok = true;
+        for_claim_set(c, w, a, b, [&](int x) { ok = ok && (c.claim[x] == key); });
+        if (!ok) continue;
+        // lockAndtryAddConnections (CellConnectionProcessor.cuh:249-272)
+        const uint32_t fa = c.B.flags[a], fb = c.B.flags[b];
+        const int na = flags_num_conn(fa), nb = flags_num_conn(fb);
+        bool already = false;
+        for (int k = 0; k < na; ++k) already = already || (c.B.btgt[k * cap + a] == b - w.cellOff);
+        if (already) continue;
+        if (!(na < flags_max_conn(fa) && nb < flags_max_conn(fb))) continue;
+        // tryAddConnections (:211-240)
+        const f2 posDelta = torus_dir(sub(to_f2(c.B.pos[b]), to_f2(c.B.pos[a])), float(w.W), float(w.H));
+        int st[kMaxBonds];
+        float sr[kMaxBonds], sa[kMaxBonds];
+        for (int k = 0; k < kMaxBonds; ++k) {
+            st[k] = c.B.btgt[k * cap + a];
+            sr[k] = c.B.brest[k * cap + a];
+            sa[k] = c.B.bang[k * cap + a];
+        }
+        if (!add_one_way(c, w, a, b, posDelta)) continue;
+        if (!add_one_way(c, w, b, a, mul(posDelta, -1.0f))) {
+            c.B.flags[a] = fa;
+            for (int k = 0; k < kMaxBonds; ++k) {
+                c.B.btgt[k * cap + a] = st[k];
+                c.B.brest[k * cap + a] = sr[k];
+                c.B.bang[k * cap + a] = sa[k];
+            }
+        }
+    }
+}
+
+__global__ void k_fuse_reset(Ctx c) {
+    const int n = min(c.sc->candidates, c.candCap);
+    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
+        int4 cd = c.cand[q];
+        if (!cd.w) continue;
+        const DevWorld& w = c.worlds[cd.z];
+        for_claim_set(c, w, cd.x, cd.y, [&](int x) { c.claim[x] = ~0ull; });
+    }
+}
+
+// deleteConnectionOneWay (CellConnectionProcessor.cuh:429-448)
+__device__ void delete_one_way(const Ctx& c, int c1, int targetLocal) {
+    const int cap = c.cellCap;
+    uint32_t f = c.B.flags[c1];
+    const int n = flags_num_conn(f);
+    for (int i = 0; i < n; ++i) {
+        if (c.B.btgt[i * cap + c1] == targetLocal) {
+            const float angleToAdd = c.B.bang[i * cap + c1];
+            for (int j = i; j < n - 1; ++j) {
+                c.B.btgt[j * cap + c1] = c.B.btgt[(j + 1) * cap + c1];
+                c.B.brest[j * cap + c1] = c.B.brest[(j + 1) * cap + c1];
+                c.B.bang[j * cap + c1] = c.B.bang[(j + 1) * cap + c1];
+            }
+            if (i < n - 1)
+                c.B.bang[i * cap + c1] += angleToAdd;
+            else
+                c.B.bang[c1] += angleToAdd;
+            c.B.flags[c1] = flags_with_num_conn(f, n - 1);
+            return;
+        }
+    }
+}
+
+__global__ void k_delete(Ctx c) {
+    const int n = c.sc->staged;
+    const int cap = c.cellCap;
+    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
+        const int2 ent = c.staged[q];
+        const int x = ent.x;
+        if (!c.mark[x]) continue;
+        const DevWorld& w = c.worlds[ent.y];
+        const int nc = flags_num_conn(c.B.flags[x]);
+        int snap[kMaxBonds];
+        for (int b = 0; b < nc; ++b) snap[b] = c.B.btgt[b * cap + x];
+        const bool self = c.delFlag[x] != 0;
+        for (int b = 0; b < nc; ++b)
+            if (self || c.delFlag[w.cellOff + snap[b]]) delete_one_way(c, x, snap[b]);
+    }
+}
+
+// last kernel of the step: clears the flags of the staged cells and advances every world's
+// timestep (SimulationCudaFacade.cu:140-143)
+__global__ void k_clear_flags(Ctx c) {
+    const int n = c.sc->staged;
+    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
+        const int x = c.staged[q].x;
+        c.mark[x] = 0;
+        c.delFlag[x] = 0;
+    }
+    if (blockIdx.x == 0)
+        for (int wi = threadIdx.x; wi < c.numWorlds; wi += blockDim.x) c.ctr[wi].timestep += 1ull;
+}
+
+void launch_bonds(const Ctx& c, const LaunchShape& sh, hipStream_t s, Recorder* rec) {
+    const dim3 sg(sh.stagedGrid), cg(sh.candGrid), b(256);
+#define L(name, kernel, grid, ...)                              \
+    do {                                                        \
+        if (rec) rec->mark(name);                               \
+        hipLaunchKernelGGL(kernel, grid, b, 0, s, __VA_ARGS__); \
+    } while (0)
+    L("remap", k_remap, sg, c);
+    L("conn_own", k_conn_own, sg, c, 0);
+    L("conn_verlet_pos", k_conn_verlet_pos, sg, c);
+    L("conn_own", k_conn_own, sg, c, 1);
+    L("conn_verlet_vel", k_conn_verlet_vel, sg, c);
+    L("mark", k_mark, sg, c);
+    L("fuse_claim", k_fuse_claim, cg, c);
+    L("fuse_apply", k_fuse_apply, cg, c);
+    L("fuse_reset", k_fuse_reset, cg, c);
+    L("delete", k_delete, sg, c);
+    L("clear_flags", k_clear_flags, sg, c);
+#undef L
+}
+
+}  // namespace alien
+

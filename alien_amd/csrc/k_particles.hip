@@ -1,0 +1,166 @@
+// k_particles.hip — energy particles: movement, particle-particle fusion, absorption into cells and
+// reflection at barrier cells (ALIEN RadiationProcessor::movement/collision,
+// source/EngineGpuKernels/RadiationProcessor.cuh:61-170).
+//
+// The reference resolves concurrent fusions and absorptions with tryLock and skips on contention.
+// Here the outcome is fixed (R8): a particle targets the winner of its 1x1 slot in the particle map
+// (lowest index, pre-move position); it fuses only into a target that is not fusing itself, and a
+// sink accepts the lowest-index fusing particle; a cell accepts the lowest-index absorbing
+// particle.  Claims are atomicMin on indices, which is order independent.
+#include "kernels_common.hpp"
+#include "kernels.hpp"
+
+namespace alien {
+
+__global__ void k_part_move(Ctx c) {
+    for_live_particles(c, [&](int wi, const DevWorld& w, int k) {
+        const DevParams& P = c.params[wi];
+        const int src = c.psorted[k];
+        f2 p = to_f2(c.A.ppos[src]);
+        const f2 v = to_f2(c.A.pvel[src]);
+        p = torus_pos(add(p, mul(v, P.dt)), w.W, w.H);
+        c.B.ppos[k] = to_float2(p);
+        c.B.pvel[k] = to_float2(v);
+        c.B.pen[k] = c.A.pen[src];
+        c.B.plast[k] = c.A.plast[src];
+        c.B.pslot[k] = c.A.pslot[src];
+        c.B.pflags[k] = c.A.pflags[src];
+    });
+}
+
+__global__ void k_part_target(Ctx c) {
+    for_live_particles(c, [&](int, const DevWorld& w, int k) {
+        const f2 p = to_f2(c.B.ppos[k]);
+        int sx, sy;
+        slot_of(p, w.W, w.H, sx, sy);
+        // particle map winner of the slot (R2): first particle of the bin whose pre-move slot matches
+        int s, e;
+        part_bin_range(c, w, part_bin_local(w, sx, sy), s, e);
+        int t = -1;
+        for (int m = s; m < e; ++m) {
+            int qx, qy;
+            slot_of(to_f2(c.A.ppos[c.psorted[m]]), w.W, w.H, qx, qy);
+            if (qx == sx && qy == sy) {
+                t = m;
+                break;
+            }
+        }
+        int want = -1;
+        if (t >= 0 && t != k && lensq(sub(p, to_f2(c.B.ppos[t]))) < 0.5f) want = t;
+        c.want[k] = want;
+        // cell hit this step: lowest-index cell of the slot at map-build time (cellMap.getFirst)
+        int cell = -1;
+        if (want < 0) {
+            const f2 q = add(p, to_f2(c.B.pvel[k]));
+            int qx, qy;
+            slot_of(q, w.W, w.H, qx, qy);
+            int cs, ce;
+            cell_bin_range(c, w, cell_bin_local(w, qx, qy), cs, ce);
+            int bestA = 0x7fffffff;
+            for (int m = cs; m < ce; ++m) {
+                const int a = c.sorted[m];
+                int ax, ay;
+                slot_of(to_f2(c.A.pos[a]), w.W, w.H, ax, ay);
+                if (ax == qx && ay == qy && a < bestA) bestA = a;
+            }
+            if (bestA != 0x7fffffff) cell = cs + rank_in_bin(c.sorted, cs, ce, bestA);
+        }
+        c.absorbCell[k] = cell;
+    });
+}
+
+__global__ void k_part_claim(Ctx c) {
+    for_live_particles(c, [&](int, const DevWorld&, int k) {
+        const int t = c.want[k];
+        if (t >= 0 && c.want[t] < 0) atomicMin(&c.fclaim[t], k);
+    });
+}
+
+__global__ void k_part_apply(Ctx c) {
+    for_live_particles(c, [&](int wi, const DevWorld& w, int k) {
+        const DevParams& P = c.params[wi];
+        const int q = c.fclaim[k];
+        if (q != kClaimFree) {
+            // sink: merge the lowest-index particle fusing into it (RadiationProcessor.cuh:86-93)
+            c.fclaim[k] = kClaimFree;
+            const float eq = c.B.pen[q], ek = c.B.pen[k];
+            if (eq > kNearZero && ek > kNearZero) {
+                const float f1 = eq / (eq + ek);
+                const f2 vq = to_f2(c.B.pvel[q]), vk = to_f2(c.B.pvel[k]);
+                c.B.pvel[k] = to_float2(add(mul(vq, f1), mul(vk, 1.0f - f1)));
+                c.B.pen[k] = ek + eq;
+                c.B.plast[k] = kNone;
+                c.B.pen[q] = 0.0f;
+                c.B.pflags[q] = uint8_t(c.B.pflags[q] | kParticleDead);
+            }
+            c.absorbCell[k] = -1;
+            return;
+        }
+        if (c.want[k] >= 0) return;
+        const int cm = c.absorbCell[k];
+        if (cm < 0) return;
+        const uint32_t fc = c.B.flags[cm];
+        if (flags_barrier(fc)) {  // reflection (:99-106)
+            const f2 vr = sub(to_f2(c.B.pvel[k]), to_f2(c.B.vel[cm]));
+            const f2 rr = torus_dir(sub(to_f2(c.B.ppos[k]), to_f2(c.B.pos[cm])), float(w.W), float(w.H));
+            const float d = dot2(vr, rr);
+            if (d < 0.0f) {
+                const float trs = fmaxf(0.1f, lensq(rr));
+                c.B.pvel[k] = to_float2(add(sub(vr, dvd(mul(mul(rr, 2.0f), d), trs)), to_f2(c.B.vel[cm])));
+            }
+            c.absorbCell[k] = -1;
+            return;
+        }
+        if (c.B.plast[k] == c.B.slot[cm] || P.absorption[flags_color(fc) % kColors] < kNearZero) {
+            c.absorbCell[k] = -1;
+            return;
+        }
+        atomicMin(&c.cclaim[cm], k);
+    });
+}
+
+// absorption (:108-165); also bins the surviving particles for the next step
+__global__ void k_part_absorb(Ctx c) {
+    for_live_particles(c, [&](int wi, const DevWorld& w, int k) {
+        const int cm = c.absorbCell[k];
+        if (cm >= 0 && c.cclaim[cm] == k) {
+            c.cclaim[cm] = kClaimFree;
+            const DevParams& P = c.params[wi];
+            const uint32_t fc = c.B.flags[cm];
+            const float ra = P.absorption[flags_color(fc) % kColors];
+            float e = c.B.pen[k];
+            float transfer = e * ra;
+            if (e < 0.01f) transfer = e;
+            const int cslot = c.B.slot[cm];
+            float* ce = &c.energy[w.cellOff + cslot];
+            *ce = *ce + transfer;
+            e = e - transfer;
+            c.B.pen[k] = e;
+            if (e < kNearZero)
+                c.B.pflags[k] = uint8_t(c.B.pflags[k] | kParticleDead);
+            else
+                c.B.plast[k] = cslot;
+        }
+        if (c.B.pflags[k] & kParticleDead)
+            c.pbinOf[k] = -1;
+        else
+            pbin_for_next_step(c, w, k, to_f2(c.B.ppos[k]), c.countNext);
+    });
+}
+
+void launch_particles(const Ctx& c, const LaunchShape& sh, hipStream_t s, Recorder* rec) {
+    if (!sh.partBlocks) return;
+    const dim3 g(sh.partGrid), b(sh.partThreads);
+    if (rec) rec->mark("part_move");
+    hipLaunchKernelGGL(k_part_move, g, b, 0, s, c);
+    if (rec) rec->mark("part_target");
+    hipLaunchKernelGGL(k_part_target, g, b, 0, s, c);
+    if (rec) rec->mark("part_claim");
+    hipLaunchKernelGGL(k_part_claim, g, b, 0, s, c);
+    if (rec) rec->mark("part_apply");
+    hipLaunchKernelGGL(k_part_apply, g, b, 0, s, c);
+    if (rec) rec->mark("part_absorb");
+    hipLaunchKernelGGL(k_part_absorb, g, b, 0, s, c);
+}
+
+}  // namespace alien

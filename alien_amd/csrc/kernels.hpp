@@ -1,4 +1,4 @@
-// kernels.hpp — host-side launch interface of kernels.hip.
+// kernels.hpp — host-side launch interface of the kernels.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -7,18 +7,33 @@
 
 namespace alien {
 
-// Launch shape of one world set: number of blocks of the per-world block tables and their
-// threads per block.  Changing it never changes results (no launch-shape-dependent arithmetic).
+// Launch shape of one world set: sizes of the per-world block tables and grid sizes of the
+// list-driven kernels.  None of it changes results: no arithmetic depends on the launch shape.
 struct LaunchShape {
     int cellBlocks = 0, cellThreads = 256;
     int partBlocks = 0, partThreads = 256;
     int binBlocks = 0, pbinBlocks = 0, binThreads = 256;
+    int tileBlocks = 0;    // neighbour-pass workgroups (including XCD padding)
+    int partGrid = 64;     // grid-stride particle kernels
+    int stagedGrid = 16;   // grid-stride kernels over the staged (bonded/barrier) cells
+    int candGrid = 16;     // grid-stride kernels over fusion candidates
+    bool hasFluid = false, hasCollision = false;  // motion modes present in the set
+};
+
+// Optional per-launch hook: mark(name) is called right before each kernel launch and
+// mark(nullptr) after the last one, so a recorder can bracket every kernel with events.
+struct Recorder {
+    virtual ~Recorder() = default;
+    virtual void mark(const char* name) = 0;
 };
 
 void launch_fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t s);
-void launch_reset_density(const Ctx& c, const LaunchShape& sh, hipStream_t s);
-void launch_sort(const Ctx& c, const LaunchShape& sh, hipStream_t s);
-void launch_step(const Ctx& c, const LaunchShape& sh, hipStream_t s);
+// bins of the current state (A) for the next sort; the step keeps them up to date itself
+void launch_seed_bins(const Ctx& c, const LaunchShape& sh, hipStream_t s);
+void launch_step(const Ctx& c, const LaunchShape& sh, hipStream_t s, Recorder* rec = nullptr);
+// cell map of the current state in logical order (sorted/newidx), for introspection; consumes
+// the seeded bins and seeds them again afterwards
+void launch_debug_map(const Ctx& c, const LaunchShape& sh, hipStream_t s);
 void launch_debug_neighbors(const Ctx& c, const LaunchShape& sh, const uint64_t* offsets, int* out, int* counts,
                             hipStream_t s);
 

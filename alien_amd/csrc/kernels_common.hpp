@@ -1,0 +1,107 @@
+// kernels_common.hpp — device helpers shared by the kernel files.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "device_math.hpp"
+#include "engine_types.hpp"
+
+namespace alien {
+
+__device__ __forceinline__ void slot_of(f2 p, int W, int H, int& sx, int& sy) {
+    sx = wrap_int(floor_int(p.x), W);
+    sy = wrap_int(floor_int(p.y), H);
+}
+__device__ __forceinline__ int cell_bin_local(const DevWorld& w, int sx, int sy) {
+    int bx = min(sx / w.binSize, w.nbx - 1), by = min(sy / w.binSize, w.nby - 1);
+    return by * w.nbx + bx;
+}
+__device__ __forceinline__ int part_bin_local(const DevWorld& w, int sx, int sy) {
+    int bx = min(sx / kParticleBin, w.pnbx - 1), by = min(sy / kParticleBin, w.pnby - 1);
+    return by * w.pnbx + bx;
+}
+__device__ __forceinline__ int wrap_small(int x, int n) { return x < 0 ? x + n : (x >= n ? x - n : x); }
+
+// global new-cell index range of local cell bin b
+__device__ __forceinline__ void cell_bin_range(const Ctx& c, const DevWorld& w, int b, int& s, int& e) {
+    uint32_t base = c.start[w.binOff];
+    s = w.cellOff + int(c.start[w.binOff + b] - base);
+    e = w.cellOff + int(c.start[w.binOff + b + 1] - base);
+}
+__device__ __forceinline__ void part_bin_range(const Ctx& c, const DevWorld& w, int b, int& s, int& e) {
+    uint32_t base = c.start[w.pbinOff];
+    s = w.partOff + int(c.start[w.pbinOff + b] - base);
+    e = w.partOff + int(c.start[w.pbinOff + b + 1] - base);
+}
+
+// rank of entry value v among the entries of sorted[s, e): the cell's position in its bin under
+// the ascending-previous-index order (R1)
+__device__ __forceinline__ int rank_in_bin(const int* sorted, int s, int e, int v) {
+    int r = 0;
+    for (int t = s; t < e; ++t) r += sorted[t] < v ? 1 : 0;
+    return r;
+}
+
+// counting-sort bookkeeping for the next step: bin of the end-of-step position, counted into
+// `hist` with an atomic whose result is not needed (the scatter ranks entries later)
+__device__ __forceinline__ void bin_for_next_step(const Ctx& c, const DevWorld& w, int k, f2 p, uint32_t* hist) {
+    int sx, sy;
+    slot_of(p, w.W, w.H, sx, sy);
+    int gb = w.binOff + cell_bin_local(w, sx, sy);
+    c.binOf[k] = gb;
+    atomicAdd(&hist[gb], 1u);
+}
+__device__ __forceinline__ void pbin_for_next_step(const Ctx& c, const DevWorld& w, int k, f2 p, uint32_t* hist) {
+    int sx, sy;
+    slot_of(p, w.W, w.H, sx, sy);
+    int gb = w.pbinOff + part_bin_local(w, sx, sy);
+    c.pbinOf[k] = gb;
+    atomicAdd(&hist[gb], 1u);
+}
+
+__device__ __forceinline__ uint32_t wave_inclusive_scan(uint32_t v) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        uint32_t t = __shfl_up(v, d, 64);
+        if (lane >= d) v += t;
+    }
+    return v;
+}
+
+// exclusive scan across the block (blockDim multiple of 64, at most 1024); block total in `total`
+__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t& total) {
+    __shared__ uint32_t waveSums[17];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int numWaves = blockDim.x >> 6;
+    uint32_t inc = wave_inclusive_scan(v);
+    if (lane == 63) waveSums[wave] = inc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t s = 0;
+        for (int k = 0; k < numWaves; ++k) {
+            uint32_t t = waveSums[k];
+            waveSums[k] = s;
+            s += t;
+        }
+        waveSums[16] = s;
+    }
+    __syncthreads();
+    uint32_t res = inc - v + waveSums[wave];
+    total = waveSums[16];
+    __syncthreads();
+    return res;
+}
+
+// grid-stride iteration over the particle block table, live particles only
+template <class F> __device__ __forceinline__ void for_live_particles(const Ctx& c, F&& f) {
+    for (int e = blockIdx.x; e < c.numPartBlocks; e += gridDim.x) {
+        const int2 blk = c.partBlocks[e];
+        const int live = c.ctr[blk.x].numParticles;
+        const DevWorld& w = c.worlds[blk.x];
+        const int k = blk.y + int(threadIdx.x);
+        if (k < w.partOff + live) f(blk.x, w, k);
+    }
+}
+
+}  // namespace alien

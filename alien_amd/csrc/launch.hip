@@ -1,0 +1,91 @@
+// launch.hip — the kernel sequence of one timestep and the introspection passes.
+//
+// Step order follows _SimulationKernelsLauncher::calcTimestep
+// (source/EngineGpuKernels/SimulationKernelsLauncher.cu:25-91): maps, neighbour forces, particle
+// movement/collision, connection forces around the Verlet updates, friction, structural operations.
+#include "kernels_common.hpp"
+#include "kernels.hpp"
+
+namespace alien {
+
+void launch_scan(const Ctx& c, hipStream_t s, StepCounters* sc);
+void launch_cell_scatter(const Ctx& c, const LaunchShape& sh, hipStream_t s);
+void launch_part_sort(const Ctx& c, const LaunchShape& sh, hipStream_t s);
+void launch_part_scatter(const Ctx& c, const LaunchShape& sh, hipStream_t s);
+void launch_cell_binsort(const Ctx& c, const LaunchShape& sh, hipStream_t s);
+void launch_neighbor(const Ctx& c, const LaunchShape& sh, hipStream_t s, bool fluid, bool collision);
+void launch_particles(const Ctx& c, const LaunchShape& sh, hipStream_t s, Recorder* rec);
+void launch_bonds(const Ctx& c, const LaunchShape& sh, hipStream_t s, Recorder* rec);
+
+void launch_step(const Ctx& c, const LaunchShape& sh, hipStream_t s, Recorder* rec) {
+    if (rec) rec->mark("scan");
+    launch_scan(c, s, c.sc);
+    if (rec) rec->mark("cell_scatter");
+    launch_cell_scatter(c, sh, s);
+    if (rec) rec->mark("part_sort");
+    launch_part_sort(c, sh, s);
+    if (rec) rec->mark("neighbor");
+    launch_neighbor(c, sh, s, sh.hasFluid, sh.hasCollision);
+    launch_particles(c, sh, s, rec);
+    launch_bonds(c, sh, s, rec);
+    if (rec) rec->mark(nullptr);
+}
+
+// The scatters advance the cursor of every bin back to the bin size, so the histogram the next
+// step consumes is intact afterwards.
+void launch_debug_map(const Ctx& c, const LaunchShape& sh, hipStream_t s) {
+    launch_scan(c, s, nullptr);
+    launch_cell_scatter(c, sh, s);
+    launch_part_scatter(c, sh, s);
+    launch_cell_binsort(c, sh, s);
+}
+
+__global__ void k_debug_neighbors(Ctx c, const uint64_t* offsets, int* out, int* counts) {
+    int2 blk = c.cellBlocks[blockIdx.x];
+    const int wi = blk.x;
+    const DevWorld w = c.worlds[wi];
+    const int k = blk.y + int(threadIdx.x);
+    if (k >= w.cellOff + w.numCells) return;
+    const DevParams& P = c.params[wi];
+    const int i = c.sorted[k];
+    const f2 pi = to_f2(c.A.pos[i]);
+    const int deti = flags_detached(c.A.flags[i]);
+    int sx, sy;
+    slot_of(pi, w.W, w.H, sx, sy);
+    const int bx = min(sx / w.binSize, w.nbx - 1), by = min(sy / w.binSize, w.nby - 1);
+    int cnt = 0;
+    uint64_t o = offsets ? offsets[i] : 0;
+    for (int dy = -1; dy <= 1; ++dy) {
+        int y = by + dy;
+        y = y < 0 ? y + w.nby : (y >= w.nby ? y - w.nby : y);
+        for (int dx = -1; dx <= 1; ++dx) {
+            int x = bx + dx;
+            x = x < 0 ? x + w.nbx : (x >= w.nbx ? x - w.nbx : x);
+            int s, e;
+            cell_bin_range(c, w, y * w.nbx + x, s, e);
+            for (int m = s; m < e; ++m) {
+                const int j = c.sorted[m];
+                if (j == i) continue;
+                const f2 pj = to_f2(c.A.pos[j]);
+                if (deti + flags_detached(c.A.flags[j]) == 1) continue;
+                bool nb;
+                if (P.motion == 0)
+                    nb = len2(torus_dir(sub(pi, pj), float(w.W), float(w.H))) <= P.smoothing * 2.0f;
+                else
+                    nb = len2(sub(pj, pi)) <= P.collisionDistance;
+                if (!nb) continue;
+                if (out) out[o + cnt] = j;
+                ++cnt;
+            }
+        }
+    }
+    if (counts) counts[i] = cnt;
+}
+
+void launch_debug_neighbors(const Ctx& c, const LaunchShape& sh, const uint64_t* offsets, int* out, int* counts,
+                            hipStream_t s) {
+    if (sh.cellBlocks)
+        hipLaunchKernelGGL(k_debug_neighbors, dim3(sh.cellBlocks), dim3(sh.cellThreads), 0, s, c, offsets, out, counts);
+}
+
+}  // namespace alien

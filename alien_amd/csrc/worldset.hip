@@ -264,6 +264,44 @@ void WorldImage::toTO(std::vector<alien_cell_to>& cells, std::vector<alien_parti
 // ---------------------------------------------------------------------------------------------
 // WorldSet
 // ---------------------------------------------------------------------------------------------
+class WorldSet::EventRecorder : public Recorder {
+public:
+    explicit EventRecorder(hipStream_t s) : stream_(s) {}
+    ~EventRecorder() override {
+        for (auto e : pool_) (void)hipEventDestroy(e);
+    }
+    void mark(const char* name) override {
+        if (used_ == pool_.size()) {
+            hipEvent_t e;
+            check_hip(hipEventCreate(&e), "hipEventCreate");
+            pool_.push_back(e);
+        }
+        check_hip(hipEventRecord(pool_[used_], stream_), "hipEventRecord");
+        names_.push_back(name);
+        ++used_;
+    }
+    // elapsed time between consecutive marks, attributed to the kernel launched after the first
+    void collect(std::map<std::string, KernelTime>& acc) {
+        check_hip(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+        for (size_t i = 0; i + 1 < used_; ++i) {
+            if (!names_[i]) continue;
+            float ms = 0.0f;
+            check_hip(hipEventElapsedTime(&ms, pool_[i], pool_[i + 1]), "hipEventElapsedTime");
+            KernelTime& k = acc[names_[i]];
+            k.launches += 1;
+            k.totalMs += double(ms);
+        }
+        used_ = 0;
+        names_.clear();
+    }
+
+private:
+    hipStream_t stream_;
+    std::vector<hipEvent_t> pool_;
+    std::vector<const char*> names_;
+    size_t used_ = 0;
+};
+
 WorldSet::WorldSet() { check_hip(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate"); }
 
 WorldSet::~WorldSet() {
@@ -329,22 +367,34 @@ void WorldSet::build(const std::vector<Source>& sources) {
     std::vector<uint64_t> timesteps;
     std::vector<int> liveParticles;
     std::vector<uint8_t> pending;
+    std::vector<int> stagedHint;
     int cellOff = 0, partOff = 0, bins = 0;
     for (const auto& src : sources) {
         HostWorld h;
         int live = 0;
         uint64_t ts = 0;
         bool pend = false;
+        int tileX = 0, tileY = 0, hint = 0;
         if (src.image) {
+            if (src.from == this) throw Error(ALIEN_E_ARG, "a world set cannot be rebuilt from itself");
             h = src.image->host;
             live = src.image->numParticles;
             ts = src.image->timestep;
             pend = src.image->densityResetPending;
+            const WorldImage& im = *src.image;
+            for (int i = 0; i < h.numCells; ++i)
+                if (flags_num_conn(im.flags[i]) > 0 || flags_barrier(im.flags[i])) ++hint;
+            const int bs = bin_size_for(h.params);
+            choose_tile(im.pos, h.desc.worldSizeX, h.desc.worldSizeY, bs, tileX, tileY);
         } else {
+            if (src.from == this) throw Error(ALIEN_E_ARG, "a world set cannot be rebuilt from itself");
             h = src.from->hosts_.at(src.fromIndex);
             live = src.from->numParticles(src.fromIndex);
             ts = src.from->timestep(src.fromIndex);
             pend = src.from->densityPending_.at(src.fromIndex) != 0;
+            tileX = src.from->worlds_.at(src.fromIndex).tileX;
+            tileY = src.from->worlds_.at(src.fromIndex).tileY;
+            hint = src.from->stagedHint_.at(src.fromIndex);
         }
         validate_world(h.desc, h.params);
         h.particleCap = live;
@@ -361,6 +411,10 @@ void WorldSet::build(const std::vector<Source>& sources) {
         w.binOff = bins;
         w.partOff = partOff;
         w.partCap = live;
+        w.tileX = std::max(1, std::min(tileX, w.nbx));
+        w.tileY = std::max(1, std::min(tileY, w.nby));
+        w.ntx = (w.nbx + w.tileX - 1) / w.tileX;
+        w.nty = (w.nby + w.tileY - 1) / w.tileY;
         cellOff += h.numCells;
         partOff += live;
         bins += w.nbx * w.nby;
@@ -370,6 +424,7 @@ void WorldSet::build(const std::vector<Source>& sources) {
         timesteps.push_back(ts);
         liveParticles.push_back(live);
         pending.push_back(pend ? 1 : 0);
+        stagedHint.push_back(hint);
     }
     for (auto& w : worlds) {
         w.pbinOff = bins;
@@ -417,10 +472,9 @@ void WorldSet::build(const std::vector<Source>& sources) {
     energy_ = static_cast<float*>(A(sizeof(float) * NC));
     stiff_ = static_cast<float*>(A(sizeof(float) * NC));
     binOf_ = static_cast<int*>(A(sizeof(int) * NC));
-    rank_ = static_cast<int*>(A(sizeof(int) * NC));
     pbinOf_ = static_cast<int*>(A(sizeof(int) * NP));
-    prank_ = static_cast<int*>(A(sizeof(int) * NP));
-    count_ = static_cast<uint32_t*>(A(sizeof(uint32_t) * NB));
+    count_[0] = static_cast<uint32_t*>(A(sizeof(uint32_t) * NB));
+    count_[1] = static_cast<uint32_t*>(A(sizeof(uint32_t) * NB));
     start_ = static_cast<uint32_t*>(A(sizeof(uint32_t) * (size_t(NB) + 1)));
     blockSums_ = static_cast<uint32_t*>(A(sizeof(uint32_t) * (size_t(NB) / 2048 + 2)));
     sorted_ = static_cast<int*>(A(sizeof(int) * NC));
@@ -510,7 +564,13 @@ void WorldSet::build(const std::vector<Source>& sources) {
     worlds_ = std::move(worlds);
     params_ = std::move(params);
     densityPending_ = std::move(pending);
+    stagedHint_ = std::move(stagedHint);
     parity_ = 0;
+    int hintTotal = 0;
+    for (int h : stagedHint_) hintTotal += h;
+    shape_.stagedGrid = std::max(16, std::min(2048, (hintTotal + 255) / 256));
+    shape_.candGrid = 16;
+    shape_.partGrid = std::max(1, std::min(1024, (partOff + shape_.partThreads - 1) / shape_.partThreads));
 
     std::vector<DevCounters> ctr(worlds_.size());
     for (size_t i = 0; i < worlds_.size(); ++i) {
@@ -525,7 +585,8 @@ void WorldSet::build(const std::vector<Source>& sources) {
     }
     StepCounters sc{};
     check_hip(hipMemcpy(dSc_, &sc, sizeof(sc), hipMemcpyHostToDevice), "H2D");
-    launch_fill_u32(count_, 0u, size_t(NB), stream_);
+    launch_fill_u32(count_[0], 0u, size_t(NB), stream_);
+    launch_fill_u32(count_[1], 0u, size_t(NB), stream_);
     launch_fill_u32(reinterpret_cast<uint32_t*>(fclaim_), uint32_t(kClaimFree), size_t(NP), stream_);
     launch_fill_u32(reinterpret_cast<uint32_t*>(cclaim_), uint32_t(kClaimFree), size_t(NC), stream_);
     launch_fill_u32(reinterpret_cast<uint32_t*>(claim_), 0xffffffffu, size_t(NC) * 2, stream_);
@@ -534,7 +595,61 @@ void WorldSet::build(const std::vector<Source>& sources) {
     launch_fill_u32(reinterpret_cast<uint32_t*>(buf_[1].pflags), 0u, (size_t(NP) + 3) / 4, stream_);
     check_hip(hipGetLastError(), "fill kernels");
     buildTables();
+    launch_seed_bins(makeCtx(parity_), shape_, stream_);
+    check_hip(hipGetLastError(), "seed bins");
     synchronize();
+}
+
+// Tile shape of the neighbour pass for one world: the largest tile (about one interior cell per
+// thread) whose fullest halo, measured on the given positions, stays within 3/4 of the LDS
+// budget.  Only performance depends on it.
+void choose_tile(const std::vector<float2>& pos, int W, int H, int binSize, int& tileX, int& tileY) {
+    const int nbx = W / binSize, nby = H / binSize;
+    std::vector<int> cnt(size_t(nbx) * nby, 0);
+    for (const float2& p : pos) {
+        int sx = host_floor_int(p.x), sy = host_floor_int(p.y);
+        sx = ((sx % W) + W) % W;
+        sy = ((sy % H) + H) % H;
+        const int bx = std::min(sx / binSize, nbx - 1), by = std::min(sy / binSize, nby - 1);
+        ++cnt[size_t(by) * nbx + bx];
+    }
+    // 2D prefix sums for window counts (torus windows handled by clamping to the grid; the edge
+    // tiles see a little less than their true halo, the 25% headroom covers it)
+    std::vector<int64_t> pre(size_t(nbx + 1) * (nby + 1), 0);
+    for (int y = 0; y < nby; ++y)
+        for (int x = 0; x < nbx; ++x)
+            pre[size_t(y + 1) * (nbx + 1) + x + 1] = cnt[size_t(y) * nbx + x] + pre[size_t(y) * (nbx + 1) + x + 1] +
+                                                    pre[size_t(y + 1) * (nbx + 1) + x] - pre[size_t(y) * (nbx + 1) + x];
+    auto window = [&](int x0, int y0, int x1, int y1) {  // [x0,x1) x [y0,y1), clamped
+        x0 = std::max(x0, 0), y0 = std::max(y0, 0), x1 = std::min(x1, nbx), y1 = std::min(y1, nby);
+        if (x1 <= x0 || y1 <= y0) return int64_t(0);
+        return pre[size_t(y1) * (nbx + 1) + x1] - pre[size_t(y0) * (nbx + 1) + x1] - pre[size_t(y1) * (nbx + 1) + x0] +
+               pre[size_t(y0) * (nbx + 1) + x0];
+    };
+    const int xs[] = {64, 32, 16, 8, 4};
+    const int ys[] = {16, 8, 4, 2, 1};
+    int bestArea = 0;
+    tileX = 4;
+    tileY = 1;
+    for (int tx : xs)
+        for (int ty : ys) {
+            if ((tx + 2) * (ty + 2) > kMaxHaloBins) continue;
+            int64_t maxHalo = 0, sumInterior = 0, tilesCount = 0;
+            for (int y0 = 0; y0 < nby; y0 += ty)
+                for (int x0 = 0; x0 < nbx; x0 += tx) {
+                    maxHalo = std::max(maxHalo, window(x0 - 1, y0 - 1, x0 + tx + 1, y0 + ty + 1));
+                    sumInterior += window(x0, y0, x0 + tx, y0 + ty);
+                    ++tilesCount;
+                }
+            const int64_t meanInterior = tilesCount ? sumInterior / tilesCount : 0;
+            if (maxHalo * 4 > int64_t(kTileCap) * 3 || meanInterior > kTileThreads) continue;
+            const int area = tx * ty;
+            if (area > bestArea) {
+                bestArea = area;
+                tileX = tx;
+                tileY = ty;
+            }
+        }
 }
 
 void WorldSet::buildTables() {
@@ -557,14 +672,37 @@ void WorldSet::buildTables() {
         check_hip(hipMemcpy(p, v.data(), sizeof(int2) * v.size(), hipMemcpyHostToDevice), "H2D");
         return static_cast<int2*>(p);
     };
+    // neighbour-pass tiles, row-major per world; workgroup b runs on XCD b % 8, so each XCD gets
+    // one contiguous run of tiles and the halo rows its tiles share stay in that XCD's L2
+    std::vector<int2> tiles;
+    for (int wi = 0; wi < int(worlds_.size()); ++wi) {
+        const DevWorld& w = worlds_[wi];
+        if (w.numCells == 0) continue;
+        for (int t = 0; t < w.ntx * w.nty; ++t) tiles.push_back(make_int2(wi, t));
+    }
+    std::vector<int2> tileTable;
+    if (!tiles.empty()) {
+        const int T = int(tiles.size());
+        const int chunk = (T + kXcds - 1) / kXcds;
+        tileTable.assign(size_t(chunk) * kXcds, make_int2(-1, -1));
+        for (int b = 0; b < int(tileTable.size()); ++b) {
+            const int t = (b % kXcds) * chunk + b / kXcds;
+            if (t < T) tileTable[b] = tiles[t];
+        }
+    }
     cellBlocks_ = up(cells);
     binBlocks_ = up(bins);
     pbinBlocks_ = up(pbins);
     partBlocks_ = up(parts);
+    tileBlocks_ = up(tileTable);
     shape_.cellBlocks = int(cells.size());
     shape_.partBlocks = int(parts.size());
     shape_.binBlocks = int(bins.size());
     shape_.pbinBlocks = int(pbins.size());
+    shape_.tileBlocks = int(tileTable.size());
+    numPartBlocks_ = int(parts.size());
+    shape_.hasFluid = shape_.hasCollision = false;
+    for (const DevParams& p : params_) (p.motion == 0 ? shape_.hasFluid : shape_.hasCollision) = true;
     invalidateGraphs();
 }
 
@@ -582,6 +720,17 @@ void WorldSet::setUseGraph(bool on) {
     invalidateGraphs();
 }
 
+void WorldSet::setKernelTiming(bool on) {
+    timing_ = on;
+    if (on && !recorder_) recorder_ = std::make_unique<EventRecorder>(stream_);
+}
+
+std::map<std::string, WorldSet::KernelTime> WorldSet::takeKernelTimes() {
+    std::map<std::string, KernelTime> r;
+    r.swap(kernelTimes_);
+    return r;
+}
+
 Ctx WorldSet::makeCtx(int parity) const {
     Ctx c{};
     c.numWorlds = int(worlds_.size());
@@ -597,10 +746,9 @@ Ctx WorldSet::makeCtx(int parity) const {
     c.energy = energy_;
     c.stiff = stiff_;
     c.binOf = binOf_;
-    c.rank = rank_;
     c.pbinOf = pbinOf_;
-    c.prank = prank_;
-    c.count = count_;
+    c.count = count_[parity];
+    c.countNext = count_[parity ^ 1];
     c.start = start_;
     c.blockSums = blockSums_;
     c.sorted = sorted_;
@@ -626,6 +774,8 @@ Ctx WorldSet::makeCtx(int parity) const {
     c.binBlocks = binBlocks_;
     c.pbinBlocks = pbinBlocks_;
     c.partBlocks = partBlocks_;
+    c.tileBlocks = tileBlocks_;
+    c.numPartBlocks = numPartBlocks_;
     return c;
 }
 
@@ -669,6 +819,8 @@ void WorldSet::setParams(int w, const alien_params& p) {
     densityPending_.at(w) = 1;  // prepareForSimulationParametersChanges, SimulationCudaFacade.cu:667-669
 }
 
+// Reads the last step's list sizes: reports overflow and grows the grids of the list-driven
+// kernels when the lists outgrew them (the graphs are captured again on the next step).
 void WorldSet::checkErrors() {
     StepCounters sc{};
     check_hip(hipMemcpy(&sc, dSc_, sizeof(sc), hipMemcpyDeviceToHost), "D2H");
@@ -676,6 +828,13 @@ void WorldSet::checkErrors() {
         sc.errors = 0;
         check_hip(hipMemcpy(dSc_, &sc, sizeof(sc), hipMemcpyHostToDevice), "H2D");
         throw Error(ALIEN_E_CAPACITY, "fusion candidate list overflow");
+    }
+    const int wantStaged = std::min(2048, (sc.staged + 255) / 256);
+    const int wantCand = std::min(2048, (std::min(sc.candidates, candCap_) + 255) / 256);
+    if (wantStaged > 2 * shape_.stagedGrid || wantCand > 2 * shape_.candGrid) {
+        shape_.stagedGrid = std::max(shape_.stagedGrid, wantStaged);
+        shape_.candGrid = std::max(shape_.candGrid, wantCand);
+        invalidateGraphs();
     }
 }
 
@@ -690,7 +849,10 @@ void WorldSet::step(uint64_t n) {
         densityPending_[w] = 0;
     }
     for (uint64_t s = 0; s < n; ++s) {
-        if (useGraph_) {
+        if (timing_) {
+            launch_step(makeCtx(parity_), shape_, stream_, recorder_.get());
+            check_hip(hipGetLastError(), "kernel launch");
+        } else if (useGraph_) {
             hipGraphExec_t& g = graph_[parity_];
             if (!g) {
                 hipGraph_t graph = nullptr;
@@ -709,6 +871,7 @@ void WorldSet::step(uint64_t n) {
     }
     for (auto& h : hosts_) h.stepsSinceSet += n;
     synchronize();
+    if (timing_) recorder_->collect(kernelTimes_);
     checkErrors();
 }
 
@@ -800,9 +963,7 @@ alien_totals WorldSet::totals(int wi) {
 void WorldSet::cellMap(int wi, std::vector<int>& bins, std::vector<int>& slots) {
     synchronize();
     const Ctx c = makeCtx(parity_);
-    StepCounters sc{};
-    check_hip(hipMemcpy(dSc_, &sc, sizeof(sc), hipMemcpyHostToDevice), "H2D");
-    launch_sort(c, shape_, stream_);
+    launch_debug_map(c, shape_, stream_);
     check_hip(hipGetLastError(), "sort");
     synchronize();
     const DevWorld& w = worlds_.at(wi);
@@ -824,7 +985,7 @@ void WorldSet::cellMap(int wi, std::vector<int>& bins, std::vector<int>& slots) 
 void WorldSet::neighbors(int wi, std::vector<std::vector<int>>& bySlot) {
     synchronize();
     const Ctx c = makeCtx(parity_);
-    launch_sort(c, shape_, stream_);
+    launch_debug_map(c, shape_, stream_);
     int* counts = nullptr;
     int* out = nullptr;
     uint64_t* offs = nullptr;

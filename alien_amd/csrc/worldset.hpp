@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <map>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -32,6 +33,8 @@ int bin_size_for(const alien_params& p);
 void validate_params(const alien_params& p);
 void validate_world(const alien_world_desc& d, const alien_params& p);
 DevParams to_dev_params(const alien_params& p);
+void choose_tile(const std::vector<float2>& pos, int W, int H, int binSize, int& tileX, int& tileY);
+constexpr int kXcds = 8;  // accelerator complex dies of an MI355X
 
 // host-side identity and pass-through data of one world (fields the physics path never touches)
 struct HostWorld {
@@ -111,6 +114,13 @@ public:
     void setBlockSize(int threads);
     int blockSize() const { return shape_.cellThreads; }
     void setUseGraph(bool on);
+    // kernel timing mode: no graphs, every launch bracketed by events (see alien_set_kernel_timing)
+    void setKernelTiming(bool on);
+    struct KernelTime {
+        uint64_t launches = 0;
+        double totalMs = 0.0;
+    };
+    std::map<std::string, KernelTime> takeKernelTimes();
     size_t deviceBytes() const { return deviceBytes_; }
     static uint64_t estimateBytes(uint64_t cells, uint64_t particles, int W, int H, int binSize);
     void synchronize();
@@ -149,8 +159,9 @@ private:
     HotBuffers buf_[2]{};
     float* energy_ = nullptr;
     float* stiff_ = nullptr;
-    int *binOf_ = nullptr, *rank_ = nullptr, *pbinOf_ = nullptr, *prank_ = nullptr;
-    uint32_t *count_ = nullptr, *start_ = nullptr, *blockSums_ = nullptr;
+    int *binOf_ = nullptr, *pbinOf_ = nullptr;
+    uint32_t* count_[2] = {nullptr, nullptr};  // bin histograms, alternating with the state buffers
+    uint32_t *start_ = nullptr, *blockSums_ = nullptr;
     int *sorted_ = nullptr, *newidx_ = nullptr, *psorted_ = nullptr;
     int2* staged_ = nullptr;
     float2 *own_ = nullptr, *s1_ = nullptr, *s2_ = nullptr, *out1_ = nullptr, *out2_ = nullptr;
@@ -160,9 +171,18 @@ private:
     int4* cand_ = nullptr;
     int *want_ = nullptr, *fclaim_ = nullptr, *absorbCell_ = nullptr, *cclaim_ = nullptr;
     int2 *cellBlocks_ = nullptr, *binBlocks_ = nullptr, *pbinBlocks_ = nullptr, *partBlocks_ = nullptr;
+    int2* tileBlocks_ = nullptr;
+    int numPartBlocks_ = 0;
+    std::vector<int> stagedHint_;  // per world: bonded + barrier cells, sizes the staged-list grids
+    void adaptGrids();
     std::vector<Arr> tableAllocs_;
 
     hipGraphExec_t graph_[2] = {nullptr, nullptr};
+
+    class EventRecorder;
+    std::unique_ptr<EventRecorder> recorder_;
+    bool timing_ = false;
+    std::map<std::string, KernelTime> kernelTimes_;
 };
 
 }  // namespace alien

@@ -18,6 +18,18 @@ LIB_PATH = os.path.join(_HERE, "libalien_hip.so")
 _lib = None
 
 
+class KernelTime(ctypes.Structure):
+    _fields_ = [("name", ctypes.c_char * 40), ("launches", ctypes.c_uint64), ("totalMs", ctypes.c_double)]
+
+
+def _kernel_times(getter, handle):
+    cap = 64
+    buf = (KernelTime * cap)()
+    n = ctypes.c_int32()
+    check(getter(handle, buf, cap, ctypes.byref(n)))
+    return {buf[i].name.decode(): (buf[i].launches, buf[i].totalMs) for i in range(min(n.value, cap))}
+
+
 class AlienError(RuntimeError):
     def __init__(self, code, message):
         super().__init__(f"[{code}] {message}")
@@ -70,6 +82,10 @@ def lib():
         L.alien_batch_get_timestep.restype = u64
         L.alien_batch_set_block_size.argtypes = [vp, i32]
         L.alien_batch_set_use_graph.argtypes = [vp, i32]
+        L.alien_set_kernel_timing.argtypes = [vp, i32]
+        L.alien_get_kernel_timing.argtypes = [vp, ctypes.POINTER(KernelTime), i32, ctypes.POINTER(i32)]
+        L.alien_batch_set_kernel_timing.argtypes = [vp, i32]
+        L.alien_batch_get_kernel_timing.argtypes = [vp, ctypes.POINTER(KernelTime), i32, ctypes.POINTER(i32)]
         L.alien_server_create.argtypes = [u64, ctypes.c_uint32, ctypes.POINTER(vp)]
         L.alien_server_destroy.argtypes = [vp]
         L.alien_server_submit.argtypes = [vp, ctypes.POINTER(WorldDesc), ctypes.POINTER(Params), vp, u64, vp, u64, u64,
@@ -205,6 +221,13 @@ class Simulation:
     def clear(self):
         check(lib().alien_clear(self._h))
 
+    def set_kernel_timing(self, on):
+        check(lib().alien_set_kernel_timing(self._h, 1 if on else 0))
+
+    def kernel_times(self):
+        """{kernel name: (launches, total ms)} since the last call (kernel timing mode)."""
+        return _kernel_times(lib().alien_get_kernel_timing, self._h)
+
     def get_map(self):
         n, _ = self.counts()
         bins = np.zeros(max(n, 1), dtype=np.int32)
@@ -285,6 +308,12 @@ class Batch:
 
     def set_use_graph(self, on):
         check(lib().alien_batch_set_use_graph(self._h, 1 if on else 0))
+
+    def set_kernel_timing(self, on):
+        check(lib().alien_batch_set_kernel_timing(self._h, 1 if on else 0))
+
+    def kernel_times(self):
+        return _kernel_times(lib().alien_batch_get_kernel_timing, self._h)
 
 
 class Server:

@@ -179,6 +179,18 @@ int alien_debug_get_map(alien_world* w, int32_t* cellBin, uint64_t* cellId, uint
 int alien_debug_neighbors(alien_world* w, uint64_t* offsets, uint64_t offsetsCap,
                           uint64_t* neighborIds, uint64_t idsCap, uint64_t* numIds);
 
+/* Kernel timing (the analog of ALIEN's debug mode, Macros.cuh:76-80, which synchronizes after
+ * every launch): while enabled, steps run without graph capture and every kernel launch is
+ * bracketed by HIP events on the engine's stream.  alien_get_kernel_timing returns, per kernel
+ * name, the number of launches and their summed event time since the last call, and resets. */
+typedef struct alien_kernel_time {
+    char name[40];
+    uint64_t launches;
+    double totalMs;
+} alien_kernel_time;
+int alien_set_kernel_timing(alien_world* w, int32_t enable);
+int alien_get_kernel_timing(alien_world* w, alien_kernel_time* out, int32_t cap, int32_t* count);
+
 /* ---- batched worlds ------------------------------------------------------------------------ */
 
 typedef struct alien_batch alien_batch;
@@ -203,6 +215,8 @@ uint64_t alien_batch_get_timestep(const alien_batch* b, int32_t world);
 /* Launch shape: threads per workgroup for the cell kernels (64..1024, multiple of 64). */
 int alien_batch_set_block_size(alien_batch* b, int32_t threads);
 int alien_batch_set_use_graph(alien_batch* b, int32_t enable);
+int alien_batch_set_kernel_timing(alien_batch* b, int32_t enable);
+int alien_batch_get_kernel_timing(alien_batch* b, alien_kernel_time* out, int32_t cap, int32_t* count);
 
 /* ---- serving ------------------------------------------------------------------------------- */
 
