@@ -95,7 +95,9 @@ int alien_get_device_name(char* buf, size_t cap) {
         alien::check_hip(hipGetDevice(&dev), "hipGetDevice");
         hipDeviceProp_t prop{};
         alien::check_hip(hipGetDeviceProperties(&prop, dev), "hipGetDeviceProperties");
-        std::string name = std::string(prop.name) + " (" + prop.gcnArchName + ")";
+        std::string name = prop.name[0] ? std::string(prop.name) : std::string("AMD GPU");
+        name += " (" + std::string(prop.gcnArchName) + ", " + std::to_string(prop.multiProcessorCount) + " CUs, " +
+                std::to_string(prop.totalGlobalMem >> 30) + " GiB)";
         std::strncpy(buf, name.c_str(), cap - 1);
         buf[cap - 1] = 0;
     });

@@ -48,14 +48,19 @@ __device__ __forceinline__ int floor_int(float v) {
 }
 __device__ __forceinline__ int wrap_int(int x, int size) { return ((x % size) + size) % size; }
 
-// IEEE remainder(x, size) for |x| < 1.5 size (differences of wrapped positions)
-__device__ __forceinline__ float torus_rem(float x, float size) {
-    float half = size / 2.0f;
-    if (x > half) return x - size;
-    if (x < -half) return x + size;
-    return x;
+// a - b on the torus for wrapped coordinates (BaseMap::correctDirection, Map.cuh:35-39, applied to
+// a - b).  The period is subtracted from the operand beyond it before the difference is taken: that
+// subtraction is exact, so a difference across the seam is as accurate as one inside the world.
+__device__ __forceinline__ float torus_delta1(float a, float b, float size) {
+    const float d = a - b;
+    const float half = size / 2.0f;
+    if (d > half) return (a - size) - b;
+    if (d < -half) return a - (b - size);
+    return d;
 }
-__device__ __forceinline__ f2 torus_dir(f2 d, float W, float H) { return {torus_rem(d.x, W), torus_rem(d.y, H)}; }
+__device__ __forceinline__ f2 torus_delta(f2 a, f2 b, float W, float H) {
+    return {torus_delta1(a.x, b.x, W), torus_delta1(a.y, b.y, H)};
+}
 
 // BaseMap::correctPosition(float2&) (Map.cuh:20-26)
 __device__ __forceinline__ f2 torus_pos(f2 p, int W, int H) {

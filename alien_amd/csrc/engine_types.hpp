@@ -50,6 +50,10 @@ struct DevParams {
     float friction, maxVelocity, minDistance, fusionVelocity, maxBindingEnergy;
     float maxBindingDistance[kColors];
     float absorption[kColors];
+    // neighbour predicate without the square root: sqrt(s) <= R  <=>  s <= reachSq, where R is the
+    // interaction radius (2 h for fluid, cellMaxCollisionDistance for collision) and reachSq the
+    // largest float whose correctly rounded square root does not exceed R
+    float reachSq;
 };
 
 // per-world geometry and placement in the set
@@ -121,9 +125,9 @@ struct Ctx {
     int* newidx;   // [cellCap] A index -> new index
     int* psorted;  // [partCap]
     int2* staged;  // [cellCap] (new index, world)
-    float2* own;   // [cellCap]
-    float2* s1;
-    float2* s2;
+    float2* own;    // [cellCap]
+    float2* shift;  // [cellCap] Verlet displacement of a staged cell, applied after the second force pass
+    float2* s2;     // [cellCap] connection force of the first pass
     float2* out1;  // [kMaxBonds][cellCap]
     float2* out2;
     uint32_t* valid;  // [cellCap]

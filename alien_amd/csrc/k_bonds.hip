@@ -31,8 +31,17 @@ __global__ void k_remap(Ctx c) {
     }
 }
 
+// bond vector from x to o.  In the second pass the cells stand at pos + shift; the vector is taken
+// as the start-of-pass difference plus the difference of the shifts, which keeps the FP32 result
+// close to the exact one (rounding pos + shift first would cost up to ulp(world size)).
+__device__ __forceinline__ f2 bond_vector(const Ctx& c, int o, f2 px, f2 sx, int pass, float Wf, float Hf) {
+    const f2 d = torus_delta(to_f2(c.B.pos[o]), px, Wf, Hf);
+    if (pass == 0) return d;
+    return add(d, sub(to_f2(c.shift[o]), sx));
+}
+
 // connection forces of the staged cells: own force plus per-bond angle forces for the partners;
-// pass 1 also runs checkConnections (CellProcessor.cuh:493-523) on the new positions
+// pass 1 also runs checkConnections (CellProcessor.cuh:493-523) on the moved positions
 __global__ void k_conn_own(Ctx c, int pass) {
     const int n = c.sc->staged;
     const int cap = c.cellCap;
@@ -50,12 +59,13 @@ __global__ void k_conn_own(Ctx c, int pass) {
         const float Wf = float(w.W), Hf = float(w.H);
         const bool considerAngles = (c.ctr[wi].timestep % 3ull) == 0ull;
         const f2 px = to_f2(c.B.pos[x]);
+        const f2 shx = pass == 0 ? f2{0.0f, 0.0f} : to_f2(c.shift[x]);
         const float sx = c.stiff[w.cellOff + c.B.slot[x]];
         const float kx2 = sx * sx;
         int tg[kMaxBonds];
 #pragma unroll
         for (int b = 0; b < kMaxBonds; ++b) tg[b] = b < nc ? w.cellOff + c.B.btgt[b * cap + x] : -1;
-        f2 prevDisp = torus_dir(sub(to_f2(c.B.pos[tg[nc - 1]]), px), Wf, Hf);
+        f2 prevDisp = bond_vector(c, tg[nc - 1], px, shx, pass, Wf, Hf);
         f2 force = {0.0f, 0.0f};
         uint32_t v = 0u;
         bool tooLong = false;
@@ -64,7 +74,7 @@ __global__ void k_conn_own(Ctx c, int pass) {
             const int o = tg[b];
             const float so = c.stiff[w.cellOff + c.B.slot[o]];
             const float kc2 = so * so;
-            const f2 disp = torus_dir(sub(to_f2(c.B.pos[o]), px), Wf, Hf);
+            const f2 disp = bond_vector(c, o, px, shx, pass, Wf, Hf);
             const float ad = len2(disp);
             if (pass == 1 && ad > maxBind) tooLong = true;  // checkConnections on new positions
             const float dev = ad - c.B.brest[b * cap + x];
@@ -137,6 +147,8 @@ __device__ __forceinline__ f2 gather_conn(const Ctx& c, const DevWorld& w, int x
     return acc;
 }
 
+// Verlet position step (CellProcessor.cuh:525-544): the displacement is stored and applied after
+// the second force pass
 __global__ void k_conn_verlet_pos(Ctx c) {
     const int n = c.sc->staged;
     for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
@@ -146,22 +158,19 @@ __global__ void k_conn_verlet_pos(Ctx c) {
         const DevParams& P = c.params[wi];
         const uint32_t fx = c.B.flags[x];
         const float dt = P.dt;
-        const f2 p = to_f2(c.B.pos[x]);
         const f2 v = to_f2(c.B.vel[x]);
         if (flags_barrier(fx)) {
-            const f2 p2 = torus_pos(add(p, mul(v, dt)), w.W, w.H);
-            c.B.pos[x] = to_float2(p2);
-            bin_for_next_step(c, w, x, p2, c.countNext);
+            c.shift[x] = to_float2(mul(v, dt));
             continue;
         }
         const f2 s1 = gather_conn(c, w, x, flags_num_conn(fx));
-        const f2 p2 = torus_pos(add(p, add(mul(v, dt), dvd(mul(mul(s1, dt), dt), 2.0f))), w.W, w.H);
-        c.B.pos[x] = to_float2(p2);
+        c.shift[x] = to_float2(add(mul(v, dt), dvd(mul(mul(s1, dt), dt), 2.0f)));
         c.s2[x] = to_float2(s1);
-        bin_for_next_step(c, w, x, p2, c.countNext);
     }
 }
 
+// Verlet velocity step, friction and decay flag (CellProcessor.cuh:546-559, 707-723, 786-790); the
+// moved position is written here, and counted into the histogram of the next step
 __global__ void k_conn_verlet_vel(Ctx c) {
     const int n = c.sc->staged;
     for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
@@ -170,6 +179,9 @@ __global__ void k_conn_verlet_vel(Ctx c) {
         const DevWorld& w = c.worlds[wi];
         const DevParams& P = c.params[wi];
         const uint32_t fx = c.B.flags[x];
+        const f2 p2 = torus_pos(add(to_f2(c.B.pos[x]), to_f2(c.shift[x])), w.W, w.H);
+        c.B.pos[x] = to_float2(p2);
+        bin_for_next_step(c, w, x, p2, c.countNext);
         if (flags_barrier(fx)) continue;
         const f2 s1 = gather_conn(c, w, x, flags_num_conn(fx));
         const f2 acc = dvd(add(s1, to_f2(c.s2[x])), 2.0f);
@@ -177,7 +189,7 @@ __global__ void k_conn_verlet_vel(Ctx c) {
         v = add(v, mul(acc, P.dt));
         v = mul(v, 1.0f - P.friction);
         c.B.vel[x] = to_float2(v);
-        if (c.energy[w.cellOff + c.B.slot[x]] > P.maxBindingEnergy) c.delFlag[x] = 1;  // decay, :786-790
+        if (c.energy[w.cellOff + c.B.slot[x]] > P.maxBindingEnergy) c.delFlag[x] = 1;
     }
 }
 
@@ -269,7 +281,7 @@ __device__ bool add_one_way(const Ctx& c, const DevWorld& w, int c1, int c2, f2 
         return true;
     }
     if (n == 1) {
-        const f2 cd = torus_dir(sub(to_f2(c.B.pos[w.cellOff + c.B.btgt[c1]]), p1), Wf, Hf);
+        const f2 cd = torus_delta(to_f2(c.B.pos[w.cellOff + c.B.btgt[c1]]), p1, Wf, Hf);
         const float prevAngle = angle_of(cd);
         float angleDiff = newAngle - prevAngle;
         if (angleDiff < 0.0f) angleDiff += 360.0f;
@@ -288,8 +300,8 @@ __device__ bool add_one_way(const Ctx& c, const DevWorld& w, int c1, int c2, f2 
     bool found = false;
     for (; index < n; ++index) {
         const int prevIndex = (index + n - 1) % n;
-        prevAngle = angle_of(torus_dir(sub(to_f2(c.B.pos[w.cellOff + c.B.btgt[prevIndex * cap + c1]]), p1), Wf, Hf));
-        nextAngle = angle_of(torus_dir(sub(to_f2(c.B.pos[w.cellOff + c.B.btgt[index * cap + c1]]), p1), Wf, Hf));
+        prevAngle = angle_of(torus_delta(to_f2(c.B.pos[w.cellOff + c.B.btgt[prevIndex * cap + c1]]), p1, Wf, Hf));
+        nextAngle = angle_of(torus_delta(to_f2(c.B.pos[w.cellOff + c.B.btgt[index * cap + c1]]), p1, Wf, Hf));
         if (angle_between(prevAngle, nextAngle, newAngle) || prevIndex == index) {
             found = true;
             break;
@@ -337,7 +349,7 @@ __global__ void k_fuse_apply(Ctx c) {
         if (already) continue;
         if (!(na < flags_max_conn(fa) && nb < flags_max_conn(fb))) continue;
         // tryAddConnections (:211-240)
-        const f2 posDelta = torus_dir(sub(to_f2(c.B.pos[b]), to_f2(c.B.pos[a])), float(w.W), float(w.H));
+        const f2 posDelta = torus_delta(to_f2(c.B.pos[b]), to_f2(c.B.pos[a]), float(w.W), float(w.H));
         int st[kMaxBonds];
         float sr[kMaxBonds], sa[kMaxBonds];
         for (int k = 0; k < kMaxBonds; ++k) {
@@ -427,6 +439,12 @@ void launch_bonds(const Ctx& c, const LaunchShape& sh, hipStream_t s, Recorder* 
         if (rec) rec->mark(name);                               \
         hipLaunchKernelGGL(kernel, grid, b, 0, s, __VA_ARGS__); \
     } while (0)
+    if (!sh.hasBondWork) {
+        // no world can have a bonded or barrier cell this step (none has one and none can fuse):
+        // only the end-of-step bookkeeping runs
+        L("clear_flags", k_clear_flags, dim3(1), c);
+        return;
+    }
     L("remap", k_remap, sg, c);
     L("conn_own", k_conn_own, sg, c, 0);
     L("conn_verlet_pos", k_conn_verlet_pos, sg, c);

@@ -102,7 +102,7 @@ __global__ void k_part_apply(Ctx c) {
         const uint32_t fc = c.B.flags[cm];
         if (flags_barrier(fc)) {  // reflection (:99-106)
             const f2 vr = sub(to_f2(c.B.pvel[k]), to_f2(c.B.vel[cm]));
-            const f2 rr = torus_dir(sub(to_f2(c.B.ppos[k]), to_f2(c.B.pos[cm])), float(w.W), float(w.H));
+            const f2 rr = torus_delta(to_f2(c.B.ppos[k]), to_f2(c.B.pos[cm]), float(w.W), float(w.H));
             const float d = dot2(vr, rr);
             if (d < 0.0f) {
                 const float trs = fmaxf(0.1f, lensq(rr));

@@ -18,6 +18,7 @@ struct LaunchShape {
     int stagedGrid = 16;   // grid-stride kernels over the staged (bonded/barrier) cells
     int candGrid = 16;     // grid-stride kernels over fusion candidates
     bool hasFluid = false, hasCollision = false;  // motion modes present in the set
+    bool hasBondWork = true;  // some world has (or can get by fusion) a bonded or barrier cell
 };
 
 // Optional per-launch hook: mark(name) is called right before each kernel launch and

@@ -70,7 +70,7 @@ __global__ void k_debug_neighbors(Ctx c, const uint64_t* offsets, int* out, int*
                 if (deti + flags_detached(c.A.flags[j]) == 1) continue;
                 bool nb;
                 if (P.motion == 0)
-                    nb = len2(torus_dir(sub(pi, pj), float(w.W), float(w.H))) <= P.smoothing * 2.0f;
+                    nb = len2(torus_delta(pi, pj, float(w.W), float(w.H))) <= P.smoothing * 2.0f;
                 else
                     nb = len2(sub(pj, pi)) <= P.collisionDistance;
                 if (!nb) continue;

@@ -49,6 +49,15 @@ void validate_world(const alien_world_desc& d, const alien_params& p) {
         throw Error(ALIEN_E_ARG, "world area above 2^30 is not supported");
 }
 
+// largest float s with sqrtf(s) <= R (sqrtf is correctly rounded and monotone)
+static float sqrt_threshold(float R) {
+    float s = R * R;
+    while (s > 0.0f && std::sqrt(s) > R) s = std::nextafter(s, 0.0f);
+    while (std::sqrt(std::nextafter(s, std::numeric_limits<float>::infinity())) <= R)
+        s = std::nextafter(s, std::numeric_limits<float>::infinity());
+    return s;
+}
+
 DevParams to_dev_params(const alien_params& p) {
     DevParams d{};
     d.dt = p.timestepSize;
@@ -67,6 +76,7 @@ DevParams to_dev_params(const alien_params& p) {
         d.maxBindingDistance[c] = p.cellMaxBindingDistance[c];
         d.absorption[c] = p.radiationAbsorption[c];
     }
+    d.reachSq = sqrt_threshold(d.motion == 0 ? d.smoothing * 2.0f : d.collisionDistance);
     return d;
 }
 
@@ -143,6 +153,14 @@ WorldImage WorldImage::fromTO(const alien_world_desc& desc, const alien_params& 
         }
         img.energy[i] = c.energy;
         img.stiff[i] = c.stiffness;
+        for (int b = 0; b < c.numConnections; ++b) {  // bonds come in pairs a->b, b->a (TOs.cuh:30-35)
+            const int t = c.connections[b].cellIndex;
+            const alien_cell_to& o = cells[t];
+            int back = 0, same = 0;
+            for (int k = 0; k < o.numConnections && k < kMaxBonds; ++k) back += o.connections[k].cellIndex == i;
+            for (int k = 0; k < c.numConnections; ++k) same += c.connections[k].cellIndex == t;
+            if (t == i || back != 1 || same != 1) throw Error(ALIEN_E_ARG, "bonds must be symmetric and unique");
+        }
         h.cellId[i] = c.id;
         h.age[i] = c.age;
         h.creatureId[i] = c.creatureId;
@@ -368,6 +386,7 @@ void WorldSet::build(const std::vector<Source>& sources) {
     std::vector<int> liveParticles;
     std::vector<uint8_t> pending;
     std::vector<int> stagedHint;
+    std::vector<uint8_t> bondWork;
     int cellOff = 0, partOff = 0, bins = 0;
     for (const auto& src : sources) {
         HostWorld h;
@@ -375,6 +394,7 @@ void WorldSet::build(const std::vector<Source>& sources) {
         uint64_t ts = 0;
         bool pend = false;
         int tileX = 0, tileY = 0, hint = 0;
+        bool work = false;
         if (src.image) {
             if (src.from == this) throw Error(ALIEN_E_ARG, "a world set cannot be rebuilt from itself");
             h = src.image->host;
@@ -382,8 +402,12 @@ void WorldSet::build(const std::vector<Source>& sources) {
             ts = src.image->timestep;
             pend = src.image->densityResetPending;
             const WorldImage& im = *src.image;
-            for (int i = 0; i < h.numCells; ++i)
+            for (int i = 0; i < h.numCells; ++i) {
                 if (flags_num_conn(im.flags[i]) > 0 || flags_barrier(im.flags[i])) ++hint;
+                // bonds can appear only by fusion, which needs a free bond slot
+                if (flags_num_conn(im.flags[i]) > 0 || flags_barrier(im.flags[i]) || flags_max_conn(im.flags[i]) > 0)
+                    work = true;
+            }
             const int bs = bin_size_for(h.params);
             choose_tile(im.pos, h.desc.worldSizeX, h.desc.worldSizeY, bs, tileX, tileY);
         } else {
@@ -395,6 +419,7 @@ void WorldSet::build(const std::vector<Source>& sources) {
             tileX = src.from->worlds_.at(src.fromIndex).tileX;
             tileY = src.from->worlds_.at(src.fromIndex).tileY;
             hint = src.from->stagedHint_.at(src.fromIndex);
+            work = src.from->bondWork_.at(src.fromIndex) != 0;
         }
         validate_world(h.desc, h.params);
         h.particleCap = live;
@@ -425,6 +450,7 @@ void WorldSet::build(const std::vector<Source>& sources) {
         liveParticles.push_back(live);
         pending.push_back(pend ? 1 : 0);
         stagedHint.push_back(hint);
+        bondWork.push_back(work ? 1 : 0);
     }
     for (auto& w : worlds) {
         w.pbinOff = bins;
@@ -482,7 +508,7 @@ void WorldSet::build(const std::vector<Source>& sources) {
     psorted_ = static_cast<int*>(A(sizeof(int) * NP));
     staged_ = static_cast<int2*>(A(sizeof(int2) * NC));
     own_ = static_cast<float2*>(A(sizeof(float2) * NC));
-    s1_ = static_cast<float2*>(A(sizeof(float2) * NC));
+    shift_ = static_cast<float2*>(A(sizeof(float2) * NC));
     s2_ = static_cast<float2*>(A(sizeof(float2) * NC));
     out1_ = static_cast<float2*>(A(sizeof(float2) * kMaxBonds * size_t(NC)));
     out2_ = static_cast<float2*>(A(sizeof(float2) * kMaxBonds * size_t(NC)));
@@ -565,6 +591,9 @@ void WorldSet::build(const std::vector<Source>& sources) {
     params_ = std::move(params);
     densityPending_ = std::move(pending);
     stagedHint_ = std::move(stagedHint);
+    bondWork_ = std::move(bondWork);
+    shape_.hasBondWork = false;
+    for (uint8_t b : bondWork_) shape_.hasBondWork = shape_.hasBondWork || b != 0;
     parity_ = 0;
     int hintTotal = 0;
     for (int h : stagedHint_) hintTotal += h;
@@ -756,7 +785,7 @@ Ctx WorldSet::makeCtx(int parity) const {
     c.psorted = psorted_;
     c.staged = staged_;
     c.own = own_;
-    c.s1 = s1_;
+    c.shift = shift_;
     c.s2 = s2_;
     c.out1 = out1_;
     c.out2 = out2_;
@@ -817,6 +846,11 @@ void WorldSet::setParams(int w, const alien_params& p) {
     params_.at(w) = to_dev_params(p);
     check_hip(hipMemcpy(dParams_ + w, &params_[w], sizeof(DevParams), hipMemcpyHostToDevice), "H2D");
     densityPending_.at(w) = 1;  // prepareForSimulationParametersChanges, SimulationCudaFacade.cu:667-669
+    // the set of motion modes selects which neighbour-pass variants the step launches
+    const bool fluid = shape_.hasFluid, collision = shape_.hasCollision;
+    shape_.hasFluid = shape_.hasCollision = false;
+    for (const DevParams& q : params_) (q.motion == 0 ? shape_.hasFluid : shape_.hasCollision) = true;
+    if (fluid != shape_.hasFluid || collision != shape_.hasCollision) invalidateGraphs();
 }
 
 // Reads the last step's list sizes: reports overflow and grows the grids of the list-driven

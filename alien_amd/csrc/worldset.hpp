@@ -164,7 +164,7 @@ private:
     uint32_t *start_ = nullptr, *blockSums_ = nullptr;
     int *sorted_ = nullptr, *newidx_ = nullptr, *psorted_ = nullptr;
     int2* staged_ = nullptr;
-    float2 *own_ = nullptr, *s1_ = nullptr, *s2_ = nullptr, *out1_ = nullptr, *out2_ = nullptr;
+    float2 *own_ = nullptr, *shift_ = nullptr, *s2_ = nullptr, *out1_ = nullptr, *out2_ = nullptr;
     uint32_t* valid_ = nullptr;
     uint8_t *delFlag_ = nullptr, *mark_ = nullptr;
     unsigned long long* claim_ = nullptr;
@@ -173,7 +173,8 @@ private:
     int2 *cellBlocks_ = nullptr, *binBlocks_ = nullptr, *pbinBlocks_ = nullptr, *partBlocks_ = nullptr;
     int2* tileBlocks_ = nullptr;
     int numPartBlocks_ = 0;
-    std::vector<int> stagedHint_;  // per world: bonded + barrier cells, sizes the staged-list grids
+    std::vector<int> stagedHint_;    // per world: bonded + barrier cells, sizes the staged-list grids
+    std::vector<uint8_t> bondWork_;  // per world: a bonded or barrier cell exists or can appear by fusion
     void adaptGrids();
     std::vector<Arr> tableAllocs_;
 

@@ -41,8 +41,9 @@ PYBIND11_MODULE(_facade, m) {
                  alien::GeneralSettings g;
                  g.worldSizeX = sx;
                  g.worldSizeY = sy;
+                 const alien_params p = paramsFromBytes(params);  // reads a Python object: GIL held
                  py::gil_scoped_release rel;
-                 f.newSimulation(ts, g, paramsFromBytes(params));
+                 f.newSimulation(ts, g, p);
              })
         .def("setSimulationData",
              [](SimulationFacade& f, py::array cells, py::array particles) {

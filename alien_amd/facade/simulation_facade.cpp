@@ -51,7 +51,9 @@ void SimulationFacade::workerLoop() {
             if (_accessState == 1) {
                 _accessState = 2;  // parked: the caller has exclusive access
                 _stateCv.notify_all();
-                _stateCv.wait(lk, [&] { return _stop || _accessState == 0; });
+                // leave the park when the caller released it (0) or the next caller already asked
+                // again (1) before this thread woke up
+                _stateCv.wait(lk, [&] { return _stop || _accessState != 2; });
                 continue;
             }
         }

@@ -96,7 +96,8 @@ def _assign_bond_angles(cells, world):
     tgt_s = np.take_along_axis(safe, order, axis=1)
     dist_s = np.take_along_axis(dist, order, axis=1)
     prev = np.roll(ang_s, 1, axis=1)
-    gaps = ang_s - prev
+    with np.errstate(invalid="ignore"):  # unused slots hold inf - inf; masked below
+        gaps = ang_s - prev
     last = np.take_along_axis(ang_s, np.maximum(k - 1, 0)[:, None], axis=1)[:, 0]
     gaps[:, 0] = 360.0 - (last - ang_s[:, 0])
     gaps = np.where(np.arange(MAX_CELL_BONDS)[None, :] < k[:, None], gaps, 0.0)

@@ -212,7 +212,8 @@ int alien_batch_get_data(alien_batch* b, int32_t world, alien_cell_to* cells, ui
                          uint64_t* numParticles);
 int alien_batch_get_counts(alien_batch* b, int32_t world, uint64_t* numCells, uint64_t* numParticles);
 uint64_t alien_batch_get_timestep(const alien_batch* b, int32_t world);
-/* Launch shape: threads per workgroup for the cell kernels (64..1024, multiple of 64). */
+/* Launch shape: threads per workgroup for the cell kernels: 64, 128, 192 or 256 (results do not
+ * depend on it). */
 int alien_batch_set_block_size(alien_batch* b, int32_t threads);
 int alien_batch_set_use_graph(alien_batch* b, int32_t enable);
 int alien_batch_set_kernel_timing(alien_batch* b, int32_t enable);

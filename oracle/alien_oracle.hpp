@@ -30,8 +30,14 @@
 //   R10 dead particles are removed with a stable compaction.
 //   R12 SPH sums gathered per cell in visiting order.
 //   R13 collision neighbours use the uncorrected (seam-blind) distance like Map.cuh:189.
-// Parity status: every function here is pinned by tests/test_oracle_*.py against closed forms,
-// invariants, brute force or the reference's own test scenarios.
+// Two rules change only the rounding of the FP32 build (in exact arithmetic they are the
+// reference's formulas), so that FP32 stays within 1e-5 of FP64 also on large worlds:
+//   R14 torus differences subtract the period from the operand beyond it first (exact), instead
+//       of rounding a - b and then taking remainderf (Map.cuh:35-39).
+//   R15 the second connection-force pass and checkConnections evaluate bond vectors at the moved
+//       positions as (start difference) + (shift difference); positions are moved after it.
+// Parity status: pinned by tests/test_oracle.py against closed forms, invariants, brute force
+// and the reference's own test scenarios (see the docstring there for which test pins what).
 #pragma once
 
 #include <algorithm>
@@ -206,15 +212,18 @@ struct Torus {
         wrapInt(ix, iy);
         p = {R(ix) + fx, R(iy) + fy};
     }
-    template <class R> static R rem(R x, R size) {
-        R half = size / R(2);
-        if (x > half) return x - size;
-        if (x < -half) return x + size;
-        return x;
+    // a - b on the torus (Map.cuh:35-39 applies remainderf to the rounded difference).  Here the
+    // period is subtracted from the operand beyond it first: for wrapped coordinates that
+    // subtraction is exact (Sterbenz), so the result is the correctly rounded torus difference,
+    // also across the seam, where rounding a - b first loses up to ulp(size) / 2.
+    template <class R> static R delta1(R a, R b, R size) {
+        const R d = a - b;
+        const R half = size / R(2);
+        if (d > half) return (a - size) - b;
+        if (d < -half) return a - (b - size);
+        return d;
     }
-    template <class R> V2<R> correctDirection(V2<R> d) const {  // Map.cuh:35-39
-        return {rem(d.x, R(W)), rem(d.y, R(H))};
-    }
+    template <class R> V2<R> delta(V2<R> a, V2<R> b) const { return {delta1(a.x, b.x, R(W)), delta1(a.y, b.y, R(H))}; }
     template <class R> void slotOf(V2<R> p, int& sx, int& sy) const {  // Map.cuh:132-137
         sx = floorInt(p.x);
         sy = floorInt(p.y);
@@ -388,6 +397,14 @@ public:
                 d.bonds[b] = {t, R(s.connections[b].distance), R(s.connections[b].angleFromPrevious)};
             }
         }
+        for (uint64_t i = 0; i < n; ++i)
+            for (int b = 0; b < cells[i].numConn; ++b) {
+                const int t = cells[i].bonds[b].target;
+                int back = 0, same = 0;
+                for (int k = 0; k < cells[t].numConn; ++k) back += cells[t].bonds[k].target == int(i);
+                for (int k = 0; k < cells[i].numConn; ++k) same += cells[i].bonds[k].target == t;
+                if (t == int(i) || back != 1 || same != 1) throw std::invalid_argument("bonds must be symmetric");
+            }
         parts.assign(np, Particle<R>{});
         for (uint64_t i = 0; i < np; ++i) {
             auto& d = parts[i];
@@ -539,7 +556,7 @@ public:
         if (rawParams.motionType == ALIEN_MOTION_COLLISION) {
             return length(b.pos - a.pos) <= p.cellMaxCollisionDistance;
         }
-        V2<R> d = torus.correctDirection(a.pos - b.pos);
+        V2<R> d = torus.delta(a.pos, b.pos);
         return length(d) <= p.smoothingLength * R(2);
     }
 
@@ -560,7 +577,7 @@ public:
         forStencil(cellBinOfIndex[i], [&](int j) {
             if (!isNeighbor(i, j)) return;
             const auto& o = cells[j];
-            V2<R> posDelta = torus.correctDirection(c.pos - o.pos);
+            V2<R> posDelta = torus.delta(c.pos, o.pos);
             R distance = length(posDelta);
             if (!c.barrier && distance < cmd) pc = pc + posDelta * cmd / R(5);
             if (bonded(i, j)) return;
@@ -612,7 +629,7 @@ public:
         int numBarrier = 0;
         forStencil(cellBinOfIndex[i], [&](int j) {
             const auto& o = cells[j];
-            V2<R> posDelta = torus.correctDirection(c.pos - o.pos);
+            V2<R> posDelta = torus.delta(c.pos, o.pos);
             R distance = length(posDelta);
             if (!(distance <= h * R(2)) || c.detached + o.detached == 1) return;
             if (o.barrier) {
@@ -641,7 +658,7 @@ public:
             int cb = -1;
             R cbDist = R(0);
             for (int k = 0; k < numBarrier; ++k) {
-                R d = length(torus.correctDirection(c.pos - cells[barrierList[k]].pos));
+                R d = length(torus.delta(c.pos, cells[barrierList[k]].pos));
                 if (cb < 0 || d < cbDist) {
                     cb = barrierList[k];
                     cbDist = d;
@@ -650,15 +667,15 @@ public:
             const auto& b = cells[cb];
             V2<R> rr{R(0), R(0)};
             if (b.numConn <= 1) {
-                rr = torus.correctDirection(c.pos - b.pos);
+                rr = torus.delta(c.pos, b.pos);
             } else {
-                R angleToCell = angleOfVector(torus.correctDirection(c.pos - b.pos));
+                R angleToCell = angleOfVector(torus.delta(c.pos, b.pos));
                 int nc = b.numConn;
                 for (int k = 0; k < nc; ++k) {
                     const auto& o1 = cells[b.bonds[k].target];
                     const auto& o2 = cells[b.bonds[(k + 1) % nc].target];
-                    R a1 = angleOfVector(torus.correctDirection(o1.pos - b.pos));
-                    R a2 = angleOfVector(torus.correctDirection(o2.pos - b.pos));
+                    R a1 = angleOfVector(torus.delta(o1.pos, b.pos));
+                    R a2 = angleOfVector(torus.delta(o2.pos, b.pos));
                     if (isAngleInBetween(a1, a2, angleToCell)) {
                         rr = o2.pos - o1.pos;
                         rotateQuarterCounterClockwise(rr);
@@ -748,7 +765,7 @@ public:
             const auto& C = cells[c];
             if (C.barrier) {
                 V2<R> vr = P.vel - C.vel;
-                V2<R> rr = torus.correctDirection(P.pos - C.pos);
+                V2<R> rr = torus.delta(P.pos, C.pos);
                 R d = dot(vr, rr);
                 if (d < R(0)) {
                     R trs = std::max(R(0.1f), lengthSquared(rr));
@@ -787,7 +804,15 @@ public:
         std::vector<uint16_t> valid;    // bit b: out1[b], bit 6+b: out2[b]
     };
 
-    void connectionForces(bool considerAngles, std::vector<V2<R>>& s1) const {
+    // bond vector from cell x to cell o; with `shift`, both cells stand at pos + shift and the
+    // vector is the unshifted difference plus the difference of the shifts (R15)
+    V2<R> bondVector(int o, int x, const std::vector<V2<R>>* shift) const {
+        V2<R> d = torus.delta(cells[o].pos, cells[x].pos);
+        if (shift) d = d + ((*shift)[o] - (*shift)[x]);
+        return d;
+    }
+
+    void connectionForces(bool considerAngles, std::vector<V2<R>>& s1, const std::vector<V2<R>>* shift) const {
         const int n = int(cells.size());
         ConnOut co;
         co.own.assign(n, V2<R>{R(0), R(0)});
@@ -800,13 +825,13 @@ public:
             if (c.numConn == 0 || c.barrier) continue;
             V2<R> force{R(0), R(0)};
             const int nc = c.numConn;
-            V2<R> prevDisp = torus.correctDirection(cells[c.bonds[nc - 1].target].pos - c.pos);
+            V2<R> prevDisp = bondVector(c.bonds[nc - 1].target, x, shift);
             R kx2 = c.stiffness * c.stiffness;
             uint16_t v = 0;
             for (int b = 0; b < nc; ++b) {
                 const auto& o = cells[c.bonds[b].target];
                 R kc2 = o.stiffness * o.stiffness;
-                V2<R> disp = torus.correctDirection(o.pos - c.pos);
+                V2<R> disp = bondVector(c.bonds[b].target, x, shift);
                 R actualDistance = length(disp);
                 R deviation = actualDistance - c.bonds[b].rest;
                 force = force + normalized(disp) * deviation * (kx2 + kc2) / R(6);
@@ -907,7 +932,7 @@ public:
             return true;
         }
         if (c.numConn == 1) {
-            V2<R> cd = torus.correctDirection(cells[c.bonds[0].target].pos - c.pos);
+            V2<R> cd = torus.delta(cells[c.bonds[0].target].pos, c.pos);
             R prevAngle = angleOfVector(cd);
             R angleDiff = newAngle - prevAngle;
             if (angleDiff < R(0)) angleDiff += R(360);
@@ -927,8 +952,8 @@ public:
         bool found = false;
         for (; index < n; ++index) {
             int prevIndex = (index + n - 1) % n;
-            prevAngle = angleOfVector(torus.correctDirection(cells[c.bonds[prevIndex].target].pos - c.pos));
-            nextAngle = angleOfVector(torus.correctDirection(cells[c.bonds[index].target].pos - c.pos));
+            prevAngle = angleOfVector(torus.delta(cells[c.bonds[prevIndex].target].pos, c.pos));
+            nextAngle = angleOfVector(torus.delta(cells[c.bonds[index].target].pos, c.pos));
             if (isAngleInBetween(prevAngle, nextAngle, newAngle) || prevIndex == index) {
                 found = true;
                 break;
@@ -951,7 +976,7 @@ public:
     }
 
     bool tryAddConnections(int a, int b) {  // :211-240
-        V2<R> posDelta = torus.correctDirection(cells[b].pos - cells[a].pos);
+        V2<R> posDelta = torus.delta(cells[b].pos, cells[a].pos);
         Cell<R> saved = cells[a];
         if (!tryAddConnectionOneWay(a, b, posDelta)) return false;
         if (!tryAddConnectionOneWay(b, a, posDelta * R(-1))) {
@@ -1074,17 +1099,15 @@ public:
         particlePhase();
 
         // connection forces, Verlet (CellProcessor.cuh:408-559)
-        std::vector<V2<R>> s1, s2(n, V2<R>{R(0), R(0)});
-        connectionForces(considerAngles, s1);
+        std::vector<V2<R>> s1, s2(n, V2<R>{R(0), R(0)}), shift(n, V2<R>{R(0), R(0)});
+        connectionForces(considerAngles, s1, nullptr);
         const R dt = p.timestepSize;
-        for (int i = 0; i < n; ++i) {
+        for (int i = 0; i < n; ++i) {  // Verlet position step; the displacement is applied below (R15)
             auto& c = cells[i];
             if (c.barrier) {
-                c.pos = c.pos + c.vel * dt;
-                torus.correctPosition(c.pos);
+                shift[i] = c.vel * dt;
             } else {
-                c.pos = c.pos + (c.vel * dt + s1[i] * dt * dt / R(2));
-                torus.correctPosition(c.pos);
+                shift[i] = c.vel * dt + s1[i] * dt * dt / R(2);
                 s2[i] = s1[i];
             }
         }
@@ -1093,11 +1116,16 @@ public:
             const auto& c = cells[i];
             if (c.barrier) continue;
             for (int b = 0; b < c.numConn; ++b) {
-                V2<R> disp = torus.correctDirection(cells[c.bonds[b].target].pos - c.pos);
+                V2<R> disp = bondVector(c.bonds[b].target, i, &shift);
                 if (length(disp) > p.cellMaxBindingDistance[c.color % ALIEN_MAX_COLORS]) delFlag[i] = 1;
             }
         }
-        connectionForces(considerAngles, s1);
+        connectionForces(considerAngles, s1, &shift);
+        for (int i = 0; i < n; ++i) {
+            auto& c = cells[i];
+            c.pos = c.pos + shift[i];
+            torus.correctPosition(c.pos);
+        }
         for (int i = 0; i < n; ++i) {  // verletVelocityUpdate + applyFriction + decay
             auto& c = cells[i];
             if (c.barrier) continue;

@@ -162,4 +162,25 @@ int oracle_neighbors(void* hv, uint64_t* offsets, uint64_t offCap, uint64_t* ids
     });
 }
 
+// scalar helpers, exposed so the tests can check them against closed forms
+void oracle_sph_kernel(int precision, double q, double* w, double* dw) {
+    if (precision == 32) {
+        *w = oracle::calcKernel<float>(float(q));
+        *dw = oracle::calcKernel_d<float>(float(q));
+    } else {
+        *w = oracle::calcKernel<double>(q);
+        *dw = oracle::calcKernel_d<double>(q);
+    }
+}
+
+double oracle_angle_of_vector(int precision, double x, double y) {
+    if (precision == 32) return oracle::angleOfVector<float>(oracle::V2<float>{float(x), float(y)});
+    return oracle::angleOfVector<double>(oracle::V2<double>{x, y});
+}
+
+double oracle_subtract_angle(int precision, double a, double b) {
+    if (precision == 32) return oracle::subtractAngle<float>(float(a), float(b));
+    return oracle::subtractAngle<double>(a, b);
+}
+
 }  // extern "C"

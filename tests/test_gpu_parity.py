@@ -1,0 +1,476 @@
+"""Parity of the HIP path with the oracle, through the C ABI (needs an MI355X).
+
+Bar: cells, particles, bonds and topology byte-identical to the FP32 oracle after every compared
+step, on small worlds that exercise every branch and on the full-size workloads; the map order
+and the neighbour lists identical to the oracle's; within 1e-5 (norm-wise relative) of the FP64
+oracle after one step; energy and momentum within 1e-3 of the FP64 oracle after 1000 steps.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from alien_amd import abi, engine, worlds
+from alien_amd.abi import MOTION_COLLISION, MOTION_FLUID
+from oracle.pyoracle import OracleWorld, from_world
+
+from . import helpers as H
+
+pytestmark = pytest.mark.gpu
+
+CPU_THREADS = 16
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def diff_report(g, o, what):
+    if len(g) != len(o):
+        return f"{what}: count {len(g)} vs {len(o)}"
+    for f in g.dtype.names:
+        a, b = g[f], o[f]
+        if a.tobytes() != b.tobytes():
+            ne = np.nonzero((a != b).reshape(len(a), -1).any(axis=1))[0]
+            return f"{what}.{f}: {len(ne)} differ, first {ne[0]}: gpu {a[ne[0]]} oracle {b[ne[0]]}"
+    return f"{what}: identical"
+
+
+def assert_identical(sim_data, ora_data, ctx):
+    (gc, gp), (oc, op) = sim_data, ora_data
+    same_c = H.packed_bytes(gc) == H.packed_bytes(oc)
+    same_p = H.packed_bytes(gp) == H.packed_bytes(op)
+    assert same_c and same_p, f"{ctx}: {diff_report(gc, oc, 'cells')}; {diff_report(gp, op, 'particles')}"
+
+
+def run_both(w, checkpoints, precision=32):
+    sim = engine.Simulation.from_world(w)
+    ref = from_world(w, precision=precision, threads=CPU_THREADS)
+    done = 0
+    for s in checkpoints:
+        sim.step(s - done)
+        ref.step(s - done)
+        done = s
+        assert sim.timestep == ref.timestep == s
+        assert_identical(sim.get_data(), ref.get_data(), f"{w.name} step {s}")
+    return sim, ref
+
+
+# --------------------------------------------------------------------------------------------
+# bit-exact against the FP32 oracle
+
+
+@pytest.mark.parametrize("motion", [MOTION_FLUID, MOTION_COLLISION])
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_mix_bit_exact(motion, seed):
+    w = worlds.random_mix(100 * seed + motion, motion=motion)
+    say(f"mix seed={seed} motion={motion} cells={len(w.cells)}")
+    run_both(w, [1, 2, 3, 6, 20, 60])
+
+
+def test_mix_dense_fusion_bit_exact():
+    """Fast free cells with free bond slots: many fusion candidates and claim conflicts per step."""
+    w = worlds.random_mix(7, motion=MOTION_COLLISION, num_free=900, vel_sigma=0.6)
+    say(f"dense fusion mix cells={len(w.cells)}")
+    sim, ref = run_both(w, [1, 5, 30])
+    c, _ = sim.get_data()
+    assert H.bonds_symmetric(c)
+    np.testing.assert_allclose(H.angle_sums(c), 360.0, atol=2e-3)
+
+
+def test_plain_soft_body_bit_exact():
+    w = worlds.plain_soft_body()
+    say(f"plain soft body cells={len(w.cells)}")
+    run_both(w, [1, 10, 100])
+
+
+def test_fluid_256k_bit_exact():
+    w = worlds.large_fluid(num_cells=256_000, num_particles=32_000, size=(1024, 1024))
+    say("fluid 256k")
+    run_both(w, [1, 4])
+
+
+def test_large_fluid_full_size_bit_exact():
+    """The 4M-cell fluid world (4096^2, 500k particles) at full size, two steps."""
+    w = worlds.large_fluid()
+    say(f"large fluid cells={len(w.cells)} particles={len(w.particles)}")
+    run_both(w, [1, 2])
+
+
+def test_large_soft_body_full_size_bit_exact():
+    """The large soft-body world (4096^2, 40k bodies + free cells) at full size, two steps."""
+    w = worlds.large_soft_body()
+    say(f"large soft body cells={len(w.cells)}")
+    run_both(w, [1, 3])
+
+
+@pytest.mark.parametrize("motion", [MOTION_FLUID, MOTION_COLLISION])
+def test_map_order_and_neighbors(motion):
+    w = worlds.random_mix(300 + motion, motion=motion)
+    sim = engine.Simulation.from_world(w)
+    ref = from_world(w)
+    for step in (0, 7):
+        sim.step(step)
+        ref.step(step)
+        gb, gi, gg = sim.get_map()
+        ob, oi, og = ref.get_map()
+        assert gg == og
+        np.testing.assert_array_equal(gi, oi)
+        np.testing.assert_array_equal(gb, ob)
+        go, gn = sim.neighbors()
+        oo, on = ref.neighbors()
+        np.testing.assert_array_equal(go, oo)
+        np.testing.assert_array_equal(gn, on)
+    # the introspection calls do not disturb the run
+    sim.step(5)
+    ref.step(5)
+    assert_identical(sim.get_data(), ref.get_data(), "after introspection")
+
+
+def test_set_get_round_trip_is_exact():
+    """DataTransferTests.cpp:19-148."""
+    w = worlds.random_mix(12)
+    sim = engine.Simulation.from_world(w)
+    gc, gp = sim.get_data()
+    assert H.packed_bytes(gc) == H.packed_bytes(np.sort(w.cells, order="id"))
+    assert H.packed_bytes(gp) == H.packed_bytes(np.sort(w.particles, order="id"))
+
+
+# --------------------------------------------------------------------------------------------
+# tolerances against the FP64 oracle
+
+
+@pytest.mark.parametrize("name", ["plain_soft_body", "fluid_256k", "mix_collision"])
+def test_single_step_within_1e5_of_fp64(name):
+    w = {
+        "plain_soft_body": lambda: worlds.plain_soft_body(),
+        "fluid_256k": lambda: worlds.large_fluid(num_cells=256_000, num_particles=32_000, size=(1024, 1024)),
+        "mix_collision": lambda: worlds.random_mix(13, motion=MOTION_COLLISION),
+    }[name]()
+    sim = engine.Simulation.from_world(w)
+    ref = from_world(w, precision=64, threads=CPU_THREADS)
+    sim.step(1)
+    ref.step(1)
+    gc, gp = sim.get_data()
+    oc, op = ref.get_data()
+    assert gc["id"].tolist() == oc["id"].tolist()
+    assert np.abs(H.torus_diff(gc["pos"], oc["pos"], w.size)).max() <= 1e-5 * np.abs(oc["pos"]).max()
+    assert H.normwise_rel(gc["vel"], oc["vel"]) <= 1e-5
+    common = np.intersect1d(gp["id"], op["id"])
+    a, b = gp[np.isin(gp["id"], common)], op[np.isin(op["id"], common)]
+    keep = np.abs(a["energy"].astype(np.float64) - b["energy"]) <= 1e-5 * np.abs(b["energy"].astype(np.float64))
+    assert len(common) - keep.sum() <= max(1, len(op) // 1000)  # threshold decisions taken differently
+    if keep.any():
+        assert H.normwise_rel(a["vel"][keep], b["vel"][keep]) <= 1e-5
+
+
+def test_1000_step_drift_within_1e3_of_fp64():
+    w = worlds.random_mix(81, motion=MOTION_COLLISION, num_barrier=0)
+    p = w.params.copy()
+    p.cellMaxVelocity = 1e9
+    sim = engine.Simulation(w.size, p, w.cells, w.particles)
+    ref = OracleWorld(w.size, p, precision=64, threads=CPU_THREADS)
+    ref.set_data(w.cells, w.particles)
+    sim.step(1000)
+    ref.step(1000)
+    tg, to = sim.totals(), ref.totals()
+    assert abs(tg["energy"] - to["energy"]) <= 1e-3 * to["energy"]
+    mg, mo = np.array(tg["momentum"]), np.array(to["momentum"])
+    assert np.abs(mg - mo).max() <= 1e-3 * np.abs(mo).max()
+
+
+# --------------------------------------------------------------------------------------------
+# edge cases
+
+
+def tiny_world(cells, parts, size=(64, 64), motion=MOTION_COLLISION, **kw):
+    return worlds.World("tiny", size, H.params(motion, **kw), cells, parts, 10, 0)
+
+
+def test_empty_world():
+    w = tiny_world(H.cells(0), H.particles(0))
+    sim = engine.Simulation.from_world(w)
+    sim.step(5)
+    c, p = sim.get_data()
+    assert len(c) == 0 and len(p) == 0
+    assert sim.timestep == 5
+
+
+def test_particles_only_and_cells_only():
+    rng = np.random.default_rng(4)
+    pa = H.particles(300)
+    pa["pos"] = rng.uniform(0, 64, size=(300, 2)).astype(np.float32)
+    pa["vel"] = rng.normal(0, 0.4, size=(300, 2)).astype(np.float32)
+    pa["energy"] = rng.uniform(1, 5, size=300).astype(np.float32)
+    run_both(tiny_world(H.cells(0), pa), [1, 10])
+    c = H.cells(200)
+    c["pos"] = rng.uniform(0, 64, size=(200, 2)).astype(np.float32)
+    c["vel"] = rng.normal(0, 0.4, size=(200, 2)).astype(np.float32)
+    run_both(tiny_world(c, H.particles(0)), [1, 10])
+
+
+@pytest.mark.parametrize("motion", [MOTION_FLUID, MOTION_COLLISION])
+def test_seam_and_odd_sizes(motion):
+    """Cells on and across the torus seam, a world whose side is not a multiple of the bin edge
+    and the smallest world with three bins per axis."""
+    rng = np.random.default_rng(5 + motion)
+    for size in ((101, 67), (6, 6), (7, 9)):
+        n = min(60, size[0] * size[1] // 2)
+        c = H.cells(n)
+        xs = np.concatenate([np.full(n // 2, np.float32(size[0]) - np.float32(1e-3)), rng.uniform(0, size[0], n - n // 2)])
+        c["pos"][:, 0] = xs.astype(np.float32)
+        c["pos"][:, 1] = rng.uniform(0, size[1], n).astype(np.float32)
+        c["vel"] = rng.normal(0, 0.3, size=(n, 2)).astype(np.float32)
+        pa = H.particles(20)
+        pa["pos"] = rng.uniform(0, min(size), size=(20, 2)).astype(np.float32)
+        pa["energy"] = 2.0
+        w = tiny_world(c, pa, size=size, motion=motion, smoothingLength=0.65)
+        say(f"seam size={size} motion={motion}")
+        run_both(w, [1, 3, 12])
+
+
+def test_velocity_clamp_and_fast_cells():
+    rng = np.random.default_rng(6)
+    c = H.cells(400)
+    c["pos"] = rng.uniform(0, 64, size=(400, 2)).astype(np.float32)
+    c["vel"] = rng.normal(0, 1.5, size=(400, 2)).astype(np.float32)
+    run_both(tiny_world(c, H.particles(0)), [1, 5, 20])
+
+
+def test_barrier_reflection_fluid():
+    w = worlds.random_mix(15, motion=MOTION_FLUID, num_barrier=120, num_free=400)
+    run_both(w, [1, 5, 25])
+
+
+def test_bonded_pair_at_rest_does_nothing():
+    """MuscleTests.cpp:31-65."""
+    c = H.cells(2)
+    c["pos"] = [[10.0, 10.0], [11.0, 10.0]]
+    H.bond(c, 0, 1, 1.0)
+    w = tiny_world(c, H.particles(0), friction=0.0)
+    sim = engine.Simulation.from_world(w)
+    sim.step(1)
+    out, _ = sim.get_data()
+    np.testing.assert_array_equal(out["vel"], 0.0)
+    assert abs(float(out["pos"][1, 0] - out["pos"][0, 0]) - 1.0) < 1e-6
+
+
+# --------------------------------------------------------------------------------------------
+# facade behaviour
+
+
+def test_set_params_mid_run():
+    w = worlds.random_mix(16, motion=MOTION_FLUID)
+    sim = engine.Simulation.from_world(w)
+    ref = from_world(w)
+    sim.step(4)
+    ref.step(4)
+    p = w.params.copy()
+    p.friction = 0.01
+    p.pressureStrength = 0.2
+    sim.set_params(p)
+    ref.set_params(p)
+    sim.step(6)
+    ref.step(6)
+    assert_identical(sim.get_data(), ref.get_data(), "after set_params")
+    q = sim.get_params()
+    assert q.friction == np.float32(0.01)
+
+
+def test_motion_switch_rebuilds_geometry():
+    w = worlds.random_mix(17, motion=MOTION_FLUID)
+    sim = engine.Simulation.from_world(w)
+    ref = from_world(w)
+    sim.step(3)
+    ref.step(3)
+    p = w.params.copy()
+    p.motionType = MOTION_COLLISION
+    sim.set_params(p)
+    ref.set_params(p)
+    sim.step(5)
+    ref.step(5)
+    assert_identical(sim.get_data(), ref.get_data(), "after motion switch")
+
+
+def test_set_timestep_changes_angle_phase():
+    w = worlds.random_mix(18)
+    sim = engine.Simulation.from_world(w)
+    ref = from_world(w)
+    sim.set_timestep(1)
+    ref.timestep = 1
+    sim.step(4)
+    ref.step(4)
+    assert sim.timestep == 5
+    assert_identical(sim.get_data(), ref.get_data(), "timestep 1..5")
+
+
+def test_set_data_replaces_and_grows():
+    small = worlds.random_mix(19)
+    big = worlds.random_mix(20, num_free=1500, patches=12, num_particles=1500)  # same 64x64 torus
+    sim = engine.Simulation.from_world(small)
+    sim.step(2)
+    sim.set_data(big.cells, big.particles)
+    ref = from_world(big)
+    ref.timestep = 2
+    sim.step(5)
+    ref.step(5)
+    assert_identical(sim.get_data(), ref.get_data(), "after growing set_data")
+
+
+def test_clear():
+    w = worlds.random_mix(21)
+    sim = engine.Simulation.from_world(w)
+    sim.step(2)
+    sim.clear()
+    assert sim.counts() == (0, 0)
+    sim.step(1)
+
+
+def test_invalid_parameters_are_rejected():
+    w = worlds.random_mix(22)
+    sim = engine.Simulation.from_world(w)
+    p = w.params.copy()
+    p.innerFriction = 0.3
+    with pytest.raises(engine.AlienError) as e:
+        sim.set_params(p)
+    assert e.value.code == abi.E_ARG
+    sim.step(1)  # still usable
+
+
+def test_invalid_bonds_are_rejected():
+    c = H.cells(3)
+    c["pos"] = [[1.0, 1.0], [2.0, 1.0], [3.0, 1.0]]
+    c["numConnections"][0] = 1
+    c["connections"][0, 0]["cellIndex"] = 7  # out of range
+    with pytest.raises(engine.AlienError) as e:
+        engine.Simulation((16, 16), H.params(), c, H.particles(0))
+    assert e.value.code == abi.E_ARG
+
+
+def test_kernel_timing_mode_same_results():
+    w = worlds.random_mix(23, motion=MOTION_FLUID)
+    sim = engine.Simulation.from_world(w)
+    ref = from_world(w)
+    sim.set_kernel_timing(True)
+    sim.step(6)
+    ref.step(6)
+    t = sim.kernel_times()
+    assert "neighbor" in t and t["neighbor"][0] == 6
+    assert_identical(sim.get_data(), ref.get_data(), "timing mode")
+    sim.set_kernel_timing(False)
+    sim.step(3)
+    ref.step(3)
+    assert_identical(sim.get_data(), ref.get_data(), "graph mode after timing mode")
+
+
+def test_cpp_facade_end_to_end():
+    """The C++ SimulationFacade (EngineInterface/SimulationFacade.h:13-110 subset) on one worker
+    thread, through the same C ABI."""
+    from alien_amd import _facade
+
+    w = worlds.random_mix(24)
+    f = _facade.SimulationFacade()
+    f.newSimulation(0, w.size[0], w.size[1], bytes(w.params))
+    f.setSimulationData(w.cells, w.particles)
+    f.calcTimesteps(9)
+    assert f.getCurrentTimestep() == 9
+    cb, pb = f.getSimulationData()
+    ref = from_world(w)
+    ref.step(9)
+    oc, op = ref.get_data()
+    fc = np.frombuffer(cb, dtype=abi.CELL_DTYPE)
+    fp = np.frombuffer(pb, dtype=abi.PARTICLE_DTYPE)
+    assert H.packed_bytes(fc) == H.packed_bytes(oc) and H.packed_bytes(fp) == H.packed_bytes(op)
+    assert f.getWorldSize() == tuple(w.size)
+    assert f.getGpuName()
+    # failure detection: a failing worker step surfaces as an error, then the facade refuses work
+    f.testOnly_failNextWorkerStep("injected failure")
+    with pytest.raises(RuntimeError, match="injected failure"):
+        f.calcTimesteps(1)
+    with pytest.raises(RuntimeError, match="invalid state"):
+        f.calcTimesteps(1)
+    f.closeSimulation()
+
+
+# --------------------------------------------------------------------------------------------
+# batch and server: each world's result is the same alone and in the mix
+
+
+def mixed_worlds(n=8):
+    out = []
+    for k in range(n):
+        motion = MOTION_FLUID if k % 3 == 0 else MOTION_COLLISION
+        size = [(64, 64), (96, 80), (128, 128), (48, 100)][k % 4]
+        out.append(worlds.random_mix(400 + k, motion=motion, size=size, num_free=150 + 70 * k,
+                                     num_particles=50 + 30 * k))
+    return out
+
+
+@pytest.mark.parametrize("block", [64, 128, 256])
+@pytest.mark.parametrize("graph", [True, False])
+def test_batch_equals_alone(block, graph):
+    ws = mixed_worlds()
+    b = engine.Batch()
+    for w in ws:
+        b.add(w)
+    b.set_block_size(block)
+    b.set_use_graph(graph)
+    b.step(3)
+    b.step(9)
+    for i, w in enumerate(ws):
+        ref = from_world(w)
+        ref.step(12)
+        assert b.timestep(i) == 12
+        assert_identical(b.get_data(i), ref.get_data(), f"batch world {i} block={block} graph={graph}")
+
+
+def test_batch_grows_between_calls():
+    ws = mixed_worlds(6)
+    b = engine.Batch()
+    for w in ws[:3]:
+        b.add(w)
+    b.step(4)
+    for w in ws[3:]:
+        b.add(w)
+    b.step(4)
+    for i, w in enumerate(ws):
+        ref = from_world(w)
+        ref.step(8 if i < 3 else 4)
+        assert_identical(b.get_data(i), ref.get_data(), f"grown batch world {i}")
+
+
+def test_server_results_equal_alone():
+    ws = mixed_worlds(10)
+    steps = [30, 5, 80, 12, 50, 7, 64, 20, 3, 41]
+    srv = engine.Server(budget_bytes=2 << 30, chunk_steps=16)
+    jobs = [srv.submit(w.size, w.params, w.cells, w.particles, s) for w, s in zip(ws, steps)]
+    for j in jobs:
+        srv.wait(j, timeout=120)
+    for j, w, s in zip(jobs, ws, steps):
+        info = srv.poll(j)
+        assert info.state == abi.JOB_DONE and info.stepsDone == s
+        ref = from_world(w)
+        ref.step(s)
+        assert_identical(srv.snapshot(j), ref.get_data(), f"job {j} ({s} steps)")
+        srv.release(j)
+    srv.close()
+
+
+def test_server_budget_admission():
+    """A budget that fits one job at a time still completes all of them, in submission order."""
+    ws = mixed_worlds(4)
+    need = max(engine.Server.estimate_bytes(len(w.cells), len(w.particles), w.size) for w in ws)
+    srv = engine.Server(budget_bytes=need, chunk_steps=8)
+    jobs = [srv.submit(w.size, w.params, w.cells, w.particles, 10) for w in ws]
+    for j in jobs:
+        srv.wait(j, timeout=120)
+    infos = [srv.poll(j) for j in jobs]
+    assert all(i.state == abi.JOB_DONE for i in infos)
+    starts = [i.startTime for i in infos]
+    assert starts == sorted(starts)
+    for j, w in zip(jobs, ws):
+        ref = from_world(w)
+        ref.step(10)
+        assert_identical(srv.snapshot(j), ref.get_data(), f"budgeted job {j}")
+    srv.close()
