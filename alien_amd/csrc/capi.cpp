@@ -59,6 +59,8 @@ void copyOut(const std::vector<alien_cell_to>& c, const std::vector<alien_partic
     if (!p.empty()) std::memcpy(parts, p.data(), p.size() * sizeof(alien_particle_to));
 }
 
+constexpr int kStripCapMax = alien::kStripMax;
+
 double nowSeconds() {
     using namespace std::chrono;
     return duration<double>(steady_clock::now().time_since_epoch()).count();
@@ -451,6 +453,47 @@ int alien_batch_get_kernel_timing(alien_batch* b, alien_kernel_time* out, int32_
             return;
         }
         copyTimes(*b->set, out, cap, count);
+    });
+}
+
+int alien_batch_autotune(alien_batch* b, uint32_t trialSteps, int32_t* blockSize, int32_t* stripCap,
+                         double* msPerStep) {
+    return guarded([&] {
+        require(b != nullptr, "null batch");
+        require(trialSteps > 0, "trialSteps must be positive");
+        b->flush();
+        if (!b->set) return;
+        // time every launch shape on a copy of the batch; results never depend on the shape
+        std::vector<WorldSet::Source> src(b->set->numWorlds());
+        for (int i = 0; i < int(src.size()); ++i) {
+            src[i].from = b->set.get();
+            src[i].fromIndex = i;
+        }
+        WorldSet trial;
+        trial.setUseGraph(b->useGraph);
+        trial.build(src);
+        double best = 1e300;
+        int bestBlock = b->blockSize, bestStrip = b->set->stripCap();
+        for (int blk : {64, 128, 256})
+            for (int sc : {kStripCapMax, 32, 16}) {
+                trial.setBlockSize(blk);
+                trial.setStripCap(sc);
+                trial.step(1);  // captures the graphs of this shape
+                const double t0 = nowSeconds();
+                trial.step(trialSteps);
+                const double dt = (nowSeconds() - t0) / double(trialSteps);
+                if (dt < best) {
+                    best = dt;
+                    bestBlock = blk;
+                    bestStrip = sc;
+                }
+            }
+        b->blockSize = bestBlock;
+        b->set->setBlockSize(bestBlock);
+        b->set->setStripCap(bestStrip);
+        if (blockSize) *blockSize = bestBlock;
+        if (stripCap) *stripCap = bestStrip;
+        if (msPerStep) *msPerStep = best * 1000.0;
     });
 }
 

@@ -46,7 +46,13 @@ __device__ __forceinline__ int floor_int(float v) {
     if (static_cast<float>(r) > v) --r;
     return r;
 }
-__device__ __forceinline__ int wrap_int(int x, int size) { return ((x % size) + size) % size; }
+// x mod size in [0, size); one period off either way (the common case) without a division
+__device__ __forceinline__ int wrap_int(int x, int size) {
+    if (x >= 0 && x < size) return x;
+    if (x < 0 && x >= -size) return x + size;
+    if (x >= size && x - size < size) return x - size;
+    return ((x % size) + size) % size;
+}
 
 // a - b on the torus for wrapped coordinates (BaseMap::correctDirection, Map.cuh:35-39, applied to
 // a - b).  The period is subtracted from the operand beyond it before the difference is taken: that

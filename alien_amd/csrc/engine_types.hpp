@@ -16,10 +16,11 @@ constexpr int kMaxBonds = 6;
 constexpr int kColors = 7;
 constexpr int kParticleBin = 8;  // edge of the energy-particle bins, in world units
 constexpr int kNone = -1;
-// neighbour-pass tiles: threads per workgroup, cells staged in LDS, halo bins per tile
-constexpr int kTileThreads = 128;
-constexpr int kTileCap = 512;
-constexpr int kMaxHaloBins = 512;
+// neighbour pass: one wave per strip of bin columns; entries of one staged bin row (strip plus two
+// halo columns) in LDS; widest strip
+constexpr int kRowThreads = 64;
+constexpr int kRowCap = 128;
+constexpr int kStripMax = 64;
 constexpr int kClaimFree = 0x7fffffff;
 
 // cell flags word
@@ -65,8 +66,8 @@ struct DevWorld {
     int binOff;             // first cell bin in the combined bin arrays
     int pbinOff;            // first particle bin in the combined bin arrays
     int partOff, partCap;   // particle arrays
-    int tileX, tileY;       // bins per neighbour-pass tile (launch shape only; never changes results)
-    int ntx, nty;           // tiles per axis
+    int tileX, tileY;       // neighbour pass: bins per strip, bin rows per chunk (launch shape only)
+    int ntx, nty;           // strips, chunks
 };
 
 // per-world device counters
@@ -121,7 +122,14 @@ struct Ctx {
     uint32_t* countNext;  // [numBins] bin sizes of the end-of-step state (next step's `count`)
     uint32_t* start;      // [numBins + 1] exclusive scan
     uint32_t* blockSums;
-    int* sorted;   // [cellCap] new index -> A index
+    int* sorted;   // [cellCap] map order (bin, then arrival) -> A index
+    // [cellCap] the start-of-step state of the cells in map order, written next to `sorted`, so
+    // that the neighbour pass reads each row of bins as one contiguous range
+    float2* mpos;
+    float2* mvel;
+    float* mdens;
+    uint32_t* mflags;
+    int* mslot;
     int* newidx;   // [cellCap] A index -> new index
     int* psorted;  // [partCap]
     int2* staged;  // [cellCap] (new index, world)

@@ -30,6 +30,7 @@ struct Nb {
     uint32_t flags;
     int aidx;  // index in the start-of-step arrays (A)
     int lidx;  // index in the end-of-step arrays (B), the logical index of this step
+    int slot;  // cold slot (stable identity)
 };
 
 __device__ __forceinline__ void push_candidate(const Ctx& c, int a, int b, int w) {
@@ -268,7 +269,7 @@ __device__ __forceinline__ void cell_finish(const Ctx& c, const DevWorld& w, con
     // the cell in the end-of-step order; bond targets stay in start-of-step indices until the
     // staged cells are renumbered (k_remap)
     c.B.flags[k] = fi;
-    c.B.slot[k] = c.A.slot[i];
+    c.B.slot[k] = s.slot;
     c.B.dens[k] = kFluid ? a.rho : s.dens;
     for (int b = 0; b < nci; ++b) {
         c.B.btgt[b * cap + k] = bt[b];
@@ -319,40 +320,62 @@ __device__ __forceinline__ void cell_direct(const Ctx& c, const DevWorld& w, con
     cell_finish<kFluid>(c, w, P, wi, s, bt, a);
 }
 
-struct TileShared {
-    float2 pos[kTileCap];
-    float2 vel[kTileCap];
-    float dens[kTileCap];
-    uint32_t flags[kTileCap];
-    int aidx[kTileCap];
-    int lidx[kTileCap];
-    uint16_t perm[kTileCap];        // slot (bin order, then rank) -> entry
-    int hbStart[kMaxHaloBins + 1];  // first slot of every halo bin (exclusive scan)
-    union {
-        struct {
-            int hbGlob[kMaxHaloBins];  // position of the bin's first entry in `sorted`
-            uint16_t ebin[kTileCap];   // halo bin of every entry
-        } st;
-        struct {
-            unsigned long long mask[kTileThreads];   // neighbours among the stencil candidates
-            unsigned long long bmask[kTileThreads];  // ... that are bond partners
-            int pbase[kTileThreads + 1];             // first pair of every thread
-            int owner[kTileThreads];                 // entry of the thread's cell
-            uint32_t rows[3][kTileThreads];          // stencil rows: first slot | count << 16
-            float term[kPairTerms][kTileThreads];
-            uint32_t pflag[kTileThreads];
-        } cp;
-    } u;
+
+// ---------------------------------------------------------------------------------------------
+// Row streaming.  A workgroup (one wave) owns a strip of up to kStripMax bin columns and a chunk of
+// bin rows, and walks the rows top to bottom with rows y-1, y, y+1 staged in LDS (a ring of three
+// rows).  While row y is computed, the entries of row y+2 and the bin counters of row y+3 are on
+// their way into registers; row y+2 is committed into the ring slot row y-1 leaves.  A staged
+// row covers the strip plus one halo column on each side, its entries stored bin by bin in rank
+// order (R1), so the three stencil bins of a cell in one row are one contiguous range.
+// ---------------------------------------------------------------------------------------------
+constexpr int kRowBins = kStripMax + 2;
+constexpr int kPairCap = 256;  // pair records per evaluation window
+
+struct RowShared {
+    int rs[3][kRowBins + 1];  // per staged row: first slot of every halo-column bin, then the total
+    int rm[3][kRowBins];      // map position of every bin's first entry
+    float2 pos[3][kRowCap];   // entries by slot
+    float2 vel[3][kRowCap];
+    float dens[3][kRowCap];
+    uint32_t flags[3][kRowCap];
+    int aidx[3][kRowCap];
+    int slot[3][kRowCap];
+    uint8_t sbin[3][kRowCap];  // slot -> halo column
+    // the row in flight
+    int nrs[kRowBins + 1];
+    int nrm[kRowBins];
+    uint8_t nebin[kRowCap];  // arrival index -> halo column
+    int naidx[kRowCap];      // arrival index -> previous index
+    // pair records of the current window: the cell's slot in the middle row, and the neighbour as
+    // bonded << 15 | stencil row << 8 | slot
+    uint8_t pself[kPairCap];
+    uint16_t pnb[kPairCap];
+    float term[kPairTerms][kRowThreads];
+    uint32_t pflag[kRowThreads];
 };
 
-__device__ __forceinline__ Nb lds_nb(const TileShared& S, int e) {
+// registers of a row in flight: bin counters of the halo columns lane and lane + 64, then the
+// entries lane and lane + 64 in arrival order
+struct RowFlight {
+    uint32_t s0[2], s1[2];
+    int aidx[2], slot[2];
+    float2 pos[2], vel[2];
+    float dens[2];
+    uint32_t flags[2];
+    int total;
+};
+
+__device__ __forceinline__ Nb row_nb(const RowShared& S, int r, int sl) {
+    const int hx = S.sbin[r][sl];
     Nb o;
-    o.pos = to_f2(S.pos[e]);
-    o.vel = to_f2(S.vel[e]);
-    o.dens = S.dens[e];
-    o.flags = S.flags[e];
-    o.aidx = S.aidx[e];
-    o.lidx = S.lidx[e];
+    o.pos = to_f2(S.pos[r][sl]);
+    o.vel = to_f2(S.vel[r][sl]);
+    o.dens = S.dens[r][sl];
+    o.flags = S.flags[r][sl];
+    o.aidx = S.aidx[r][sl];
+    o.slot = S.slot[r][sl];
+    o.lidx = S.rm[r][hx] + (sl - S.rs[r][hx]);
     return o;
 }
 
@@ -364,111 +387,288 @@ __device__ __forceinline__ Nb global_nb(const Ctx& c, int a, int l) {
     o.flags = c.A.flags[a];
     o.aidx = a;
     o.lidx = l;
+    o.slot = c.A.slot[a];
     return o;
 }
 
-// position of the r-th set bit of m (m has more than r set bits)
-__device__ __forceinline__ int nth_bit(unsigned long long m, int r) {
-    for (int q = 0; q < r; ++q) m &= m - 1ull;
-    return __builtin_ctzll(m);
+struct StripGeom {
+    int x0, IX, HX;  // first interior bin column, interior columns, halo columns (IX + 2)
+    int nbx, nby, binOff, cellOff;
+    uint32_t base;   // start[] value of the world's first bin
+};
+
+__device__ __forceinline__ int strip_bin(const StripGeom& g, int yy, int hx) {
+    return g.binOff + wrap_small(yy, g.nby) * g.nbx + wrap_small(g.x0 - 1 + hx, g.nbx);
 }
 
-// interior slot q of the tile (slots of halo rows 1..IY, columns 1..IX, row by row) and its bin
-__device__ __forceinline__ void interior_slot(const TileShared& S, int q, int HX, int IX, int IY, int& slot, int& hb) {
-    int acc = 0, row = 1;
-    slot = -1;
-    for (int rr = 1; rr <= IY; ++rr) {
-        const int n = S.hbStart[rr * HX + 1 + IX] - S.hbStart[rr * HX + 1];
-        if (q < acc + n) {
-            slot = S.hbStart[rr * HX + 1] + (q - acc);
-            row = rr;
-            break;
+// A: bin counters of row yy
+__device__ __forceinline__ void row_issue_counts(const Ctx& c, const StripGeom& g, int yy, RowFlight& F) {
+    const int lane = threadIdx.x;
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+        const int hx = lane + q * kRowThreads;
+        F.s0[q] = F.s1[q] = 0u;
+        if (hx < g.HX) {
+            const int gb = strip_bin(g, yy, hx);
+            F.s0[q] = c.start[gb];
+            F.s1[q] = c.start[gb + 1];
         }
-        acc += n;
     }
-    int lo = row * HX + 1, hi = row * HX + 1 + IX;  // hbStart[lo] <= slot < hbStart[hi]
-    while (hi - lo > 1) {
-        const int mid = (lo + hi) >> 1;
-        if (S.hbStart[mid] <= slot)
-            lo = mid;
-        else
-            hi = mid;
-    }
-    hb = lo;
 }
 
-template <bool kFluid>
-__global__ void __launch_bounds__(kTileThreads) k_neighbor_tile(Ctx c) {
-    __shared__ TileShared S;
-    const int2 tb = c.tileBlocks[blockIdx.x];
-    if (tb.x < 0) return;
-    const int wi = tb.x;
-    const DevWorld& w = c.worlds[wi];
-    const DevParams& P = c.params[wi];
-    if ((P.motion == 0) != kFluid) return;
-    const int tx = tb.y % w.ntx, ty = tb.y / w.ntx;
-    const int x0 = tx * w.tileX, y0 = ty * w.tileY;
-    const int IX = min(w.tileX, w.nbx - x0), IY = min(w.tileY, w.nby - y0);
-    const int HX = IX + 2, HY = IY + 2, nh = HX * HY;
-    const uint32_t base = c.start[w.binOff];
-    const int tid = threadIdx.x;
-    const float Wf = float(w.W), Hf = float(w.H);
-    constexpr int kPer = kMaxHaloBins / kTileThreads;
-    constexpr int kEnt = kTileCap / kTileThreads;
-
-    // 1a. halo bins: entry counts and slot offsets
-    int cnt[kPer];
-    uint32_t local = 0;
-#pragma unroll
-    for (int q = 0; q < kPer; ++q) {
-        const int hb = tid * kPer + q;
-        int n = 0;
-        if (hb < nh) {
-            const int rr = hb / HX, cc = hb - rr * HX;
-            const int gx = wrap_small(x0 - 1 + cc, w.nbx), gy = wrap_small(y0 - 1 + rr, w.nby);
-            const int gb = w.binOff + gy * w.nbx + gx;
-            const uint32_t s = c.start[gb];
-            n = int(c.start[gb + 1] - s);
-            S.u.st.hbGlob[hb] = w.cellOff + int(s - base);
-        }
-        cnt[q] = n;
-        local += uint32_t(n);
+// B: slots of the row's bins, then the entry loads (map-order copy, contiguous per bin)
+__device__ __forceinline__ void row_issue_entries(const Ctx& c, const StripGeom& g, RowShared& S, RowFlight& F) {
+    const int lane = threadIdx.x;
+    const int n0 = lane < g.HX ? int(F.s1[0] - F.s0[0]) : 0;
+    const int n1 = lane + kRowThreads < g.HX ? int(F.s1[1] - F.s0[1]) : 0;
+    const int inc0 = int(wave_inclusive_scan(uint32_t(n0)));
+    const int tot0 = __shfl(inc0, kRowThreads - 1);
+    const int inc1 = int(wave_inclusive_scan(uint32_t(n1)));
+    F.total = tot0 + __shfl(inc1, kRowThreads - 1);
+    const int off0 = inc0 - n0, off1 = tot0 + inc1 - n1;
+    if (lane < g.HX) {
+        S.nrs[lane] = off0;
+        S.nrm[lane] = g.cellOff + int(F.s0[0] - g.base);
+        for (int t = 0; t < n0; ++t) S.nebin[off0 + t] = uint8_t(lane);
     }
-    uint32_t total;
-    uint32_t off = block_exclusive_scan(local, total);
-    const bool staged = total <= uint32_t(kTileCap);
-#pragma unroll
-    for (int q = 0; q < kPer; ++q) {
-        const int hb = tid * kPer + q;
-        if (hb < nh) {
-            S.hbStart[hb] = int(off);
-            if (staged)
-                for (int t = 0; t < cnt[q]; ++t) S.u.st.ebin[off + t] = uint16_t(hb);
-        }
-        off += uint32_t(cnt[q]);
+    if (lane + kRowThreads < g.HX) {
+        S.nrs[lane + kRowThreads] = off1;
+        S.nrm[lane + kRowThreads] = g.cellOff + int(F.s0[1] - g.base);
+        for (int t = 0; t < n1; ++t) S.nebin[off1 + t] = uint8_t(lane + kRowThreads);
     }
-    if (tid == 0) S.hbStart[nh] = int(total);
+    if (lane == 0) S.nrs[g.HX] = F.total;
     __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+        const int e = lane + q * kRowThreads;
+        if (e < F.total) {
+            const int hx = S.nebin[e];
+            const int m = S.nrm[hx] + (e - S.nrs[hx]);
+            F.aidx[q] = c.sorted[m];
+            F.slot[q] = c.mslot[m];
+            F.pos[q] = c.mpos[m];
+            F.vel[q] = c.mvel[m];
+            F.dens[q] = c.mdens[m];
+            F.flags[q] = c.mflags[m];
+        }
+    }
+}
 
-    int interior = 0;
-    for (int rr = 1; rr <= IY; ++rr) interior += S.hbStart[rr * HX + 1 + IX] - S.hbStart[rr * HX + 1];
+// C: the row into ring slot r, every entry at its rank by previous index inside its bin (R1)
+__device__ __forceinline__ void row_commit(const StripGeom& g, RowShared& S, const RowFlight& F, int r) {
+    const int lane = threadIdx.x;
+    for (int i = lane; i <= g.HX; i += kRowThreads) S.rs[r][i] = S.nrs[i];
+    for (int i = lane; i < g.HX; i += kRowThreads) S.rm[r][i] = S.nrm[i];
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+        const int e = lane + q * kRowThreads;
+        if (e < F.total) S.naidx[e] = F.aidx[q];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+        const int e = lane + q * kRowThreads;
+        if (e < F.total) {
+            const int hx = S.nebin[e];
+            const int s0 = S.nrs[hx], s1 = S.nrs[hx + 1];
+            const int a = F.aidx[q];
+            int rank = 0;
+            for (int u = s0; u < s1; ++u) rank += S.naidx[u] < a ? 1 : 0;
+            const int sl = s0 + rank;
+            S.pos[r][sl] = F.pos[q];
+            S.vel[r][sl] = F.vel[q];
+            S.dens[r][sl] = F.dens[q];
+            S.flags[r][sl] = F.flags[q];
+            S.aidx[r][sl] = a;
+            S.slot[r][sl] = F.slot[q];
+            S.sbin[r][sl] = uint8_t(hx);
+        }
+    }
+    __syncthreads();
+}
 
-    if (!staged) {
-        // dense tile: every thread walks the stencil of its cells straight from global memory;
-        // entries of a bin are visited by increasing previous index (selection over the bin)
-        for (int q = tid; q < interior; q += kTileThreads) {
-            int lq, hb;
-            interior_slot(S, q, HX, IX, IY, lq, hb);
-            const int m0 = S.u.st.hbGlob[hb];
-            const int n0 = S.hbStart[hb + 1] - S.hbStart[hb];
-            const int i = c.sorted[m0 + (lq - S.hbStart[hb])];
-            const Nb self = global_nb(c, i, m0 + rank_in_bin(c.sorted, m0, m0 + n0, i));
+// neighbour predicate on staged data (is_neighbor, with the cell itself recognised by its slot)
+template <bool kFluid>
+__device__ __forceinline__ bool staged_neighbor(const Nb& s, f2 opos, uint32_t oflags, bool isSelf, float Wf, float Hf,
+                                                float reachSq) {
+    if (flags_detached(s.flags) + flags_detached(oflags) == 1) return false;
+    if constexpr (kFluid)
+        return lensq(torus_delta(s.pos, opos, Wf, Hf)) <= reachSq;
+    else
+        return !isSelf && lensq(sub(opos, s.pos)) <= reachSq;
+}
+
+// The cells of row y (ring slot rc; rows y-1 and y+1 in rm_ and rp).
+template <bool kFluid>
+__device__ __forceinline__ void row_compute(const Ctx& c, const DevWorld& w, const DevParams& P, int wi,
+                                            const StripGeom& g, RowShared& S, int rm_, int rc, int rp) {
+    const int lane = threadIdx.x;
+    const float Wf = float(w.W), Hf = float(w.H);
+    const int cs = S.rs[rc][1];
+    const int ncell = S.rs[rc][g.IX + 1] - cs;
+    for (int q0 = 0; q0 < ncell; q0 += kRowThreads) {
+        // candidates of this lane's cell: three contiguous slot ranges, one per stencil row
+        const int q = q0 + lane;
+        const bool active = q < ncell;
+        Nb self;
+        int bt[kMaxBonds];
+        int a0s[3] = {0, 0, 0}, ns[3] = {0, 0, 0};
+        unsigned long long mask = 0ull, bmask = 0ull;
+        bool direct = false;
+        const int sl = cs + q;
+        if (active) {
+            self = row_nb(S, rc, sl);
+            load_bonds(c, self, bt);
+            const int hx = S.sbin[rc][sl];
+            a0s[0] = S.rs[rm_][hx - 1];
+            ns[0] = S.rs[rm_][hx + 2] - a0s[0];
+            a0s[1] = S.rs[rc][hx - 1];
+            ns[1] = S.rs[rc][hx + 2] - a0s[1];
+            a0s[2] = S.rs[rp][hx - 1];
+            ns[2] = S.rs[rp][hx + 2] - a0s[2];
+            if (ns[0] + ns[1] + ns[2] > 64) {
+                direct = true;
+            } else {
+                const bool hasBonds = flags_num_conn(self.flags) > 0;
+                int cidx = 0;
+#pragma unroll
+                for (int r = 0; r < 3; ++r) {
+                    const int rr = r == 0 ? rm_ : (r == 1 ? rc : rp);
+                    const int a0 = a0s[r], n = ns[r];
+                    for (int u = 0; u < n; u += 2) {
+                        // two candidates per iteration: their loads overlap
+                        const f2 p0 = to_f2(S.pos[rr][a0 + u]);
+                        const uint32_t f0 = S.flags[rr][a0 + u];
+                        const bool two = u + 1 < n;
+                        const f2 p1 = two ? to_f2(S.pos[rr][a0 + u + 1]) : p0;
+                        const uint32_t f1 = two ? S.flags[rr][a0 + u + 1] : f0;
+                        if (staged_neighbor<kFluid>(self, p0, f0, r == 1 && a0 + u == sl, Wf, Hf, P.reachSq)) {
+                            mask |= 1ull << (cidx + u);
+                            if (hasBonds && is_bonded(bt, S.aidx[rr][a0 + u] - w.cellOff)) bmask |= 1ull << (cidx + u);
+                        }
+                        if (two &&
+                            staged_neighbor<kFluid>(self, p1, f1, r == 1 && a0 + u + 1 == sl, Wf, Hf, P.reachSq)) {
+                            mask |= 1ull << (cidx + u + 1);
+                            if (hasBonds && is_bonded(bt, S.aidx[rr][a0 + u + 1] - w.cellOff))
+                                bmask |= 1ull << (cidx + u + 1);
+                        }
+                    }
+                    cidx += n;
+                }
+            }
+        }
+        const int count = __builtin_popcountll(mask);
+        const int inc = int(wave_inclusive_scan(uint32_t(count)));
+        const int numPairs = __shfl(inc, kRowThreads - 1);
+        const int pbase = inc - count;
+
+        Acc a;
+        if (active) acc_init(a, self);
+        for (int W0 = 0; W0 < numPairs; W0 += kPairCap) {
+            const int wend = min(numPairs, W0 + kPairCap);
+            // owners write the records of their pairs inside this window
+            if (count > 0 && pbase < wend && pbase + count > W0) {
+                unsigned long long m = mask;
+                for (int p = pbase; m; ++p) {
+                    const int ci = __builtin_ctzll(m);
+                    m &= m - 1ull;
+                    if (p < W0 || p >= wend) continue;
+                    int r = 0, rest = ci;
+                    if (rest >= ns[0]) {
+                        rest -= ns[0];
+                        r = 1;
+                        if (rest >= ns[1]) {
+                            rest -= ns[1];
+                            r = 2;
+                        }
+                    }
+                    const int nslot = a0s[r == 0 ? 0 : (r == 1 ? 1 : 2)] + rest;
+                    const uint32_t bonded = uint32_t((bmask >> ci) & 1ull);
+                    S.pself[p - W0] = uint8_t(sl);
+                    S.pnb[p - W0] = uint16_t((bonded << 15) | (uint32_t(r) << 8) | uint32_t(nslot));
+                }
+            }
+            __syncthreads();
+            for (int off = W0; off < wend; off += kRowThreads) {
+                // pair terms, one pair per lane
+                const int p = off + lane;
+                if (p < wend) {
+                    const uint32_t rec = S.pnb[p - W0];
+                    const int r = int((rec >> 8) & 3u);
+                    const int rr = r == 0 ? rm_ : (r == 1 ? rc : rp);
+                    const Nb s = row_nb(S, rc, S.pself[p - W0]);
+                    const Nb o = row_nb(S, rr, int(rec & 0xFFu));
+                    float t[kPairTerms];
+                    const uint32_t f = pair_term<kFluid>(c, w, P, wi, s, o, (rec >> 15) != 0u, t);
+#pragma unroll
+                    for (int v = 0; v < kPairTerms; ++v) S.term[v][lane] = t[v];
+                    S.pflag[lane] = f;
+                }
+                __syncthreads();
+                // the owner adds its pairs of this round in visiting order
+                if (count > 0) {
+                    const int lo = max(pbase, off), hi = min(pbase + count, min(off + kRowThreads, wend));
+                    for (int p2 = lo; p2 < hi; ++p2) {
+                        const int k = p2 - off;
+                        float t[kPairTerms];
+#pragma unroll
+                        for (int v = 0; v < kPairTerms; ++v) t[v] = S.term[v][k];
+                        acc_add<kFluid>(a, S.pflag[k], t);
+                    }
+                }
+                __syncthreads();
+            }
+        }
+        if (active) {
+            if (direct) {
+                cell_direct<kFluid>(c, w, P, wi, self, [&](auto&& visit) {
+#pragma unroll
+                    for (int r = 0; r < 3; ++r) {
+                        const int rr = r == 0 ? rm_ : (r == 1 ? rc : rp);
+                        for (int u = a0s[r]; u < a0s[r] + ns[r]; ++u) visit(row_nb(S, rr, u));
+                    }
+                });
+            } else {
+                cell_finish<kFluid>(c, w, P, wi, self, bt, a);
+            }
+        }
+    }
+}
+
+// A chunk with a row too dense for LDS: every cell walks its stencil in global memory, entries of
+// a bin by increasing previous index (selection over the bin), same order and results.
+template <bool kFluid>
+__device__ void rows_direct(const Ctx& c, const DevWorld& w, const DevParams& P, int wi, const StripGeom& g, int y0,
+                            int IY) {
+    const uint32_t base = g.base;
+    auto binRange = [&](int gb, int& s0, int& n) {
+        s0 = g.cellOff + int(c.start[gb] - base);
+        n = int(c.start[gb + 1] - c.start[gb]);
+    };
+    for (int y = y0; y < y0 + IY; ++y) {
+        const int rowFirst = g.binOff + y * g.nbx + g.x0;
+        const int m0 = g.cellOff + int(c.start[rowFirst] - base);
+        const int m1 = g.cellOff + int(c.start[rowFirst + g.IX] - base);
+        for (int m = m0 + int(threadIdx.x); m < m1; m += kRowThreads) {
+            // the bin of map position m: the last bin of the row segment starting at or before m
+            int lo = 0, hi = g.IX;
+            while (hi - lo > 1) {
+                const int mid = (lo + hi) >> 1;
+                if (g.cellOff + int(c.start[rowFirst + mid] - base) <= m)
+                    lo = mid;
+                else
+                    hi = mid;
+            }
+            const int hx = lo + 1;
+            int bs, bn;
+            binRange(strip_bin(g, y, hx), bs, bn);
+            const int i = c.sorted[m];
+            const Nb self = global_nb(c, i, bs + rank_in_bin(c.sorted, bs, bs + bn, i));
             cell_direct<kFluid>(c, w, P, wi, self, [&](auto&& visit) {
                 for (int dy = -1; dy <= 1; ++dy)
                     for (int dx = -1; dx <= 1; ++dx) {
-                        const int nb = hb + dy * HX + dx;
-                        const int s0 = S.u.st.hbGlob[nb];
-                        const int n = S.hbStart[nb + 1] - S.hbStart[nb];
+                        int s0, n;
+                        binRange(strip_bin(g, y + dy, hx + dx), s0, n);
                         int last = -1;
                         for (int t = 0; t < n; ++t) {
                             int best = 0x7fffffff;
@@ -482,176 +682,79 @@ __global__ void __launch_bounds__(kTileThreads) k_neighbor_tile(Ctx c) {
                     }
             });
         }
+    }
+}
+
+template <bool kFluid>
+__global__ void __launch_bounds__(kRowThreads) k_neighbor_rows(Ctx c) {
+    __shared__ RowShared S;
+    const int2 tb = c.tileBlocks[blockIdx.x];
+    if (tb.x < 0) return;
+    const int wi = tb.x;
+    const DevWorld& w = c.worlds[wi];
+    const DevParams& P = c.params[wi];
+    if ((P.motion == 0) != kFluid) return;
+    const int strip = tb.y % w.ntx, chunk = tb.y / w.ntx;
+    StripGeom g;
+    g.x0 = strip * w.tileX;
+    g.IX = min(w.tileX, w.nbx - g.x0);
+    g.HX = g.IX + 2;
+    g.nbx = w.nbx;
+    g.nby = w.nby;
+    g.binOff = w.binOff;
+    g.cellOff = w.cellOff;
+    g.base = c.start[w.binOff];
+    const int y0 = chunk * w.tileY;
+    const int IY = min(w.tileY, w.nby - y0);
+    const int lane = threadIdx.x;
+
+    // rows y0-1 .. y0+IY must each fit the LDS row capacity
+    bool tooDense = false;
+    if (lane < IY + 2) {
+        const int yy = y0 - 1 + lane;
+        const int rowFirst = g.binOff + wrap_small(yy, g.nby) * g.nbx + g.x0;
+        int total = int(c.start[rowFirst + g.IX] - c.start[rowFirst]);
+        const int gl = strip_bin(g, yy, 0), gr = strip_bin(g, yy, g.IX + 1);
+        total += int(c.start[gl + 1] - c.start[gl]) + int(c.start[gr + 1] - c.start[gr]);
+        tooDense = total > kRowCap;
+    }
+    if (__any(tooDense)) {
+        rows_direct<kFluid>(c, w, P, wi, g, y0, IY);
         return;
     }
 
-    // 1b. entries: one thread per entry, all loads of a thread in flight together
-    {
-        int ii[kEnt];
-#pragma unroll
-        for (int q = 0; q < kEnt; ++q) {
-            const int e = tid + q * kTileThreads;
-            ii[q] = -1;
-            if (e < int(total)) {
-                const int hb = S.u.st.ebin[e];
-                ii[q] = c.sorted[S.u.st.hbGlob[hb] + (e - S.hbStart[hb])];
-            }
-        }
-#pragma unroll
-        for (int q = 0; q < kEnt; ++q) {
-            const int e = tid + q * kTileThreads;
-            const int i = ii[q];
-            if (i >= 0) {
-                S.pos[e] = c.A.pos[i];
-                S.vel[e] = c.A.vel[i];
-                S.dens[e] = c.A.dens[i];
-                S.flags[e] = c.A.flags[i];
-                S.aidx[e] = i;
-            }
-        }
+    RowFlight F, G;
+    // rows y0-1, y0, y0+1 into ring slots 0, 1, 2
+    for (int r = 0; r < 3; ++r) {
+        row_issue_counts(c, g, y0 - 1 + r, F);
+        row_issue_entries(c, g, S, F);
+        row_commit(g, S, F, r);
     }
-    __syncthreads();
-    // 1c. rank inside the bin by previous index (R1)
-    for (int e = tid; e < int(total); e += kTileThreads) {
-        const int hb = S.u.st.ebin[e];
-        const int s0 = S.hbStart[hb], s1 = S.hbStart[hb + 1];
-        const int a = S.aidx[e];
-        int r = 0;
-        for (int u = s0; u < s1; ++u) r += S.aidx[u] < a ? 1 : 0;
-        S.perm[s0 + r] = uint16_t(e);
-        S.lidx[e] = S.u.st.hbGlob[hb] + r;
-    }
-    __syncthreads();
-
-    for (int q0 = 0; q0 < interior; q0 += kTileThreads) {
-        // 2. stencil candidates of this thread's cell
-        const int q = q0 + tid;
-        const bool active = q < interior;
-        Nb self;
-        int bt[kMaxBonds];
-        int own = -1;
-        bool direct = false;
-        uint32_t rows[3] = {0u, 0u, 0u};
-        unsigned long long mask = 0ull, bmask = 0ull;
-        if (active) {
-            int lq, hb;
-            interior_slot(S, q, HX, IX, IY, lq, hb);
-            own = S.perm[lq];
-            self = lds_nb(S, own);
-            load_bonds(c, self, bt);
-            int ncand = 0;
+    if (IY >= 2) row_issue_counts(c, g, y0 + 2, F);
+    for (int y = y0; y < y0 + IY; ++y) {
+        const int rc = (y - y0 + 1) % 3;
+        const int rm_ = (rc + 2) % 3, rp = (rc + 1) % 3;
+        const bool next = y + 2 <= y0 + IY;  // row y+2 is the lower halo of row y+1
+        if (next) {
+            row_issue_entries(c, g, S, F);
+            if (y + 3 <= y0 + IY) row_issue_counts(c, g, y + 3, G);
+        }
+        row_compute<kFluid>(c, w, P, wi, g, S, rm_, rc, rp);
+        if (next) {
+            row_commit(g, S, F, rm_);  // row y+2 takes the slot of row y-1
 #pragma unroll
-            for (int r = 0; r < 3; ++r) {
-                const int rb = hb + (r - 1) * HX;
-                const int a0 = S.hbStart[rb - 1], a1 = S.hbStart[rb + 2];
-                rows[r] = uint32_t(a0) | (uint32_t(a1 - a0) << 16);
-                ncand += a1 - a0;
-            }
-            if (ncand > 64) {
-                direct = true;
-            } else {
-                const bool hasBonds = flags_num_conn(self.flags) > 0;
-                int cidx = 0;
-#pragma unroll
-                for (int r = 0; r < 3; ++r) {
-                    const int a0 = int(rows[r] & 0xFFFFu), n = int(rows[r] >> 16);
-                    for (int sl = a0; sl < a0 + n; ++sl, ++cidx) {
-                        const int e = S.perm[sl];
-                        Nb o;
-                        o.pos = to_f2(S.pos[e]);
-                        o.flags = S.flags[e];
-                        o.aidx = S.aidx[e];
-                        if (is_neighbor<kFluid>(self, o, Wf, Hf, P.reachSq)) {
-                            mask |= 1ull << cidx;
-                            if (hasBonds && is_bonded(bt, o.aidx - w.cellOff)) bmask |= 1ull << cidx;
-                        }
-                    }
-                }
+            for (int q = 0; q < 2; ++q) {
+                F.s0[q] = G.s0[q];
+                F.s1[q] = G.s1[q];
             }
         }
-        const int count = __builtin_popcountll(mask);
-        uint32_t numPairs;
-        const int pbase = int(block_exclusive_scan(uint32_t(count), numPairs));
-        S.u.cp.mask[tid] = mask;
-        S.u.cp.bmask[tid] = bmask;
-        S.u.cp.pbase[tid] = pbase;
-        S.u.cp.owner[tid] = own;
-#pragma unroll
-        for (int r = 0; r < 3; ++r) S.u.cp.rows[r][tid] = rows[r];
-        if (tid == 0) S.u.cp.pbase[kTileThreads] = int(numPairs);
-        __syncthreads();
-
-        Acc a;
-        if (active) acc_init(a, self);
-        for (int off2 = 0; off2 < int(numPairs); off2 += kTileThreads) {
-            // 3. pair terms, one pair per thread
-            const int p = off2 + tid;
-            if (p < int(numPairs)) {
-                int lo = 0, hi = kTileThreads;  // owner: last thread with pbase <= p
-                while (hi - lo > 1) {
-                    const int mid = (lo + hi) >> 1;
-                    if (S.u.cp.pbase[mid] <= p)
-                        lo = mid;
-                    else
-                        hi = mid;
-                }
-                const int ci = nth_bit(S.u.cp.mask[lo], p - S.u.cp.pbase[lo]);
-                int sl = -1, rest = ci;
-#pragma unroll
-                for (int r = 0; r < 3; ++r) {
-                    const uint32_t rw = S.u.cp.rows[r][lo];
-                    const int n = int(rw >> 16);
-                    if (sl < 0) {
-                        if (rest < n)
-                            sl = int(rw & 0xFFFFu) + rest;
-                        else
-                            rest -= n;
-                    }
-                }
-                const Nb s = lds_nb(S, S.u.cp.owner[lo]);
-                const Nb o = lds_nb(S, S.perm[sl]);
-                float t[kPairTerms];
-                const bool bonded = ((S.u.cp.bmask[lo] >> ci) & 1ull) != 0ull;
-                const uint32_t f = pair_term<kFluid>(c, w, P, wi, s, o, bonded, t);
-#pragma unroll
-                for (int v = 0; v < kPairTerms; ++v) S.u.cp.term[v][tid] = t[v];
-                S.u.cp.pflag[tid] = f;
-            }
-            __syncthreads();
-            // 4. the owner adds its pairs of this round in order
-            if (count > 0) {
-                const int lo = max(pbase, off2), hi = min(pbase + count, off2 + kTileThreads);
-                for (int p2 = lo; p2 < hi; ++p2) {
-                    const int slotp = p2 - off2;
-                    float t[kPairTerms];
-#pragma unroll
-                    for (int v = 0; v < kPairTerms; ++v) t[v] = S.u.cp.term[v][slotp];
-                    acc_add<kFluid>(a, S.u.cp.pflag[slotp], t);
-                }
-            }
-            __syncthreads();
-        }
-        if (active) {
-            if (direct) {
-                cell_direct<kFluid>(c, w, P, wi, self, [&](auto&& visit) {
-#pragma unroll
-                    for (int r = 0; r < 3; ++r) {
-                        const int a0 = int(rows[r] & 0xFFFFu), n = int(rows[r] >> 16);
-                        for (int sl = a0; sl < a0 + n; ++sl) visit(lds_nb(S, S.perm[sl]));
-                    }
-                });
-            } else {
-                cell_finish<kFluid>(c, w, P, wi, self, bt, a);
-            }
-        }
-        __syncthreads();  // the pair arrays are reused by the next group of cells
     }
 }
 
 void launch_neighbor(const Ctx& c, const LaunchShape& sh, hipStream_t s, bool fluid, bool collision) {
     if (!sh.tileBlocks) return;
-    if (fluid) hipLaunchKernelGGL(k_neighbor_tile<true>, dim3(sh.tileBlocks), dim3(kTileThreads), 0, s, c);
-    if (collision) hipLaunchKernelGGL(k_neighbor_tile<false>, dim3(sh.tileBlocks), dim3(kTileThreads), 0, s, c);
+    if (fluid) hipLaunchKernelGGL(k_neighbor_rows<true>, dim3(sh.tileBlocks), dim3(kRowThreads), 0, s, c);
+    if (collision) hipLaunchKernelGGL(k_neighbor_rows<false>, dim3(sh.tileBlocks), dim3(kRowThreads), 0, s, c);
 }
 
 }  // namespace alien

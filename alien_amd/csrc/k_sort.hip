@@ -88,10 +88,10 @@ __global__ void __launch_bounds__(1024) k_scan_top(uint32_t* blockSums, int numT
     }
 }
 
-// writes start[] and clears both histograms: `count` becomes the scatter cursor, `other` the
-// histogram of the end-of-step state
-__global__ void __launch_bounds__(kScanThreads) k_scan_down(uint32_t* count, uint32_t* other,
-                                                            const uint32_t* blockSums, uint32_t* start, int n) {
+// writes start[]; `count` stays: the scatter counts it back down to zero, which leaves it cleared
+// for the step after next (the two histograms alternate)
+__global__ void __launch_bounds__(kScanThreads) k_scan_down(const uint32_t* count, const uint32_t* blockSums,
+                                                            uint32_t* start, int n) {
     const size_t base = size_t(blockIdx.x) * kScanTile + size_t(threadIdx.x) * kScanItems;
     uint32_t v[8];
     load8(count, base, n, v);
@@ -107,37 +107,40 @@ __global__ void __launch_bounds__(kScanThreads) k_scan_down(uint32_t* count, uin
     if (base + 8 <= size_t(n)) {
         *reinterpret_cast<uint4*>(start + base) = make_uint4(o[0], o[1], o[2], o[3]);
         *reinterpret_cast<uint4*>(start + base + 4) = make_uint4(o[4], o[5], o[6], o[7]);
-        *reinterpret_cast<uint4*>(count + base) = make_uint4(0u, 0u, 0u, 0u);
-        *reinterpret_cast<uint4*>(count + base + 4) = make_uint4(0u, 0u, 0u, 0u);
-        *reinterpret_cast<uint4*>(other + base) = make_uint4(0u, 0u, 0u, 0u);
-        *reinterpret_cast<uint4*>(other + base + 4) = make_uint4(0u, 0u, 0u, 0u);
     } else {
 #pragma unroll
         for (int k = 0; k < 8; ++k)
-            if (base + k < size_t(n)) {
-                start[base + k] = o[k];
-                count[base + k] = 0u;
-                other[base + k] = 0u;
-            }
+            if (base + k < size_t(n)) start[base + k] = o[k];
     }
 }
 
-// entries land in their bin in arrival order; the in-bin order is fixed later by ranking
+// entries land in their bin in arrival order (the in-bin order is fixed later by ranking), with
+// a copy of the state the neighbour pass reads; counting the bin down to zero hands every entry
+// its own position and clears the histogram
 __global__ void k_cell_scatter(Ctx c) {
     int2 blk = c.cellBlocks[blockIdx.x];
     const DevWorld w = c.worlds[blk.x];
     int i = blk.y + int(threadIdx.x);
     if (i >= w.cellOff + w.numCells) return;
+    const float2 p = c.A.pos[i], v = c.A.vel[i];
+    const float d = c.A.dens[i];
+    const uint32_t f = c.A.flags[i];
+    const int sl = c.A.slot[i];
     int gb = c.binOf[i];
-    int dest = w.cellOff + int(c.start[gb] - c.start[w.binOff]) + int(atomicAdd(&c.count[gb], 1u));
+    int dest = w.cellOff + int(c.start[gb] - c.start[w.binOff]) + int(atomicSub(&c.count[gb], 1u)) - 1;
     c.sorted[dest] = i;
+    c.mpos[dest] = p;
+    c.mvel[dest] = v;
+    c.mdens[dest] = d;
+    c.mflags[dest] = f;
+    c.mslot[dest] = sl;
 }
 
 __global__ void k_part_scatter(Ctx c) {
     for_live_particles(c, [&](int, const DevWorld& w, int i) {
         int gb = c.pbinOf[i];
         if (gb < 0) return;
-        int dest = w.partOff + int(c.start[gb] - c.start[w.pbinOff]) + int(atomicAdd(&c.count[gb], 1u));
+        int dest = w.partOff + int(c.start[gb] - c.start[w.pbinOff]) + int(atomicSub(&c.count[gb], 1u)) - 1;
         c.psorted[dest] = i;
     });
 }
@@ -231,8 +234,7 @@ void launch_scan(const Ctx& c, hipStream_t s, StepCounters* sc) {
     const int tiles = (c.numBins + kScanTile - 1) / kScanTile;
     hipLaunchKernelGGL(k_scan_reduce, dim3(tiles), dim3(kScanThreads), 0, s, c.count, c.blockSums, c.numBins);
     hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, s, c.blockSums, tiles, c.start, c.numBins, sc);
-    hipLaunchKernelGGL(k_scan_down, dim3(tiles), dim3(kScanThreads), 0, s, c.count, c.countNext, c.blockSums, c.start,
-                       c.numBins);
+    hipLaunchKernelGGL(k_scan_down, dim3(tiles), dim3(kScanThreads), 0, s, c.count, c.blockSums, c.start, c.numBins);
 }
 
 void launch_cell_scatter(const Ctx& c, const LaunchShape& sh, hipStream_t s) {

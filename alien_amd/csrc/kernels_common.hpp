@@ -12,8 +12,16 @@ __device__ __forceinline__ void slot_of(f2 p, int W, int H, int& sx, int& sy) {
     sx = wrap_int(floor_int(p.x), W);
     sy = wrap_int(floor_int(p.y), H);
 }
+// floor(a / b) for 0 <= a < 2^24, 0 < b < 2^24: the product with the approximate reciprocal is
+// within a few units of the quotient, the corrections make it exact
+__device__ __forceinline__ int div_small(int a, int b) {
+    int q = int(float(a) * __builtin_amdgcn_rcpf(float(b)));
+    while ((q + 1) * b <= a) ++q;
+    while (q > 0 && q * b > a) --q;
+    return q;
+}
 __device__ __forceinline__ int cell_bin_local(const DevWorld& w, int sx, int sy) {
-    int bx = min(sx / w.binSize, w.nbx - 1), by = min(sy / w.binSize, w.nby - 1);
+    int bx = min(div_small(sx, w.binSize), w.nbx - 1), by = min(div_small(sy, w.binSize), w.nby - 1);
     return by * w.nbx + bx;
 }
 __device__ __forceinline__ int part_bin_local(const DevWorld& w, int sx, int sy) {

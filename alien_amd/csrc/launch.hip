@@ -31,13 +31,14 @@ void launch_step(const Ctx& c, const LaunchShape& sh, hipStream_t s, Recorder* r
     if (rec) rec->mark(nullptr);
 }
 
-// The scatters advance the cursor of every bin back to the bin size, so the histogram the next
-// step consumes is intact afterwards.
+// The scatters count the histogram down to zero; seeding the bins again restores it for the next
+// step.
 void launch_debug_map(const Ctx& c, const LaunchShape& sh, hipStream_t s) {
     launch_scan(c, s, nullptr);
     launch_cell_scatter(c, sh, s);
     launch_part_scatter(c, sh, s);
     launch_cell_binsort(c, sh, s);
+    launch_seed_bins(c, sh, s);
 }
 
 __global__ void k_debug_neighbors(Ctx c, const uint64_t* offsets, int* out, int* counts) {

@@ -45,6 +45,8 @@ void validate_world(const alien_world_desc& d, const alien_params& p) {
     int s = bin_size_for(p);
     if (d.worldSizeX < 3 * s || d.worldSizeY < 3 * s)
         throw Error(ALIEN_E_ARG, "world must span at least three interaction bins per axis");
+    if (d.worldSizeX > (1 << 24) || d.worldSizeY > (1 << 24))
+        throw Error(ALIEN_E_ARG, "world side above 2^24 is not supported");
     if (int64_t(d.worldSizeX) * int64_t(d.worldSizeY) > (int64_t(1) << 30))
         throw Error(ALIEN_E_ARG, "world area above 2^30 is not supported");
 }
@@ -358,7 +360,7 @@ void WorldSet::synchronize() { check_hip(hipStreamSynchronize(stream_), "hipStre
 
 uint64_t WorldSet::estimateBytes(uint64_t cells, uint64_t particles, int W, int H, int binSize) {
     const uint64_t perCell = 2 * (8 + 8 + 4 + 4 + 4 + kMaxBonds * 12) + 8 + 4 * 4 + 8 + 3 * 8 + 2 * kMaxBonds * 8 + 4 +
-                             2 + 8 + 4 * 16 + 4;
+                             2 + 8 + 4 * 16 + 4 + (8 + 8 + 4 + 4 + 4);
     const uint64_t perParticle = 2 * (8 + 8 + 4 + 4 + 4 + 1) + 6 * 4;
     const uint64_t bins = uint64_t(std::max(1, W / binSize)) * uint64_t(std::max(1, H / binSize)) +
                           uint64_t(std::max(1, W / kParticleBin)) * uint64_t(std::max(1, H / kParticleBin));
@@ -387,13 +389,14 @@ void WorldSet::build(const std::vector<Source>& sources) {
     std::vector<uint8_t> pending;
     std::vector<int> stagedHint;
     std::vector<uint8_t> bondWork;
+    std::vector<int> stripChoice;
     int cellOff = 0, partOff = 0, bins = 0;
     for (const auto& src : sources) {
         HostWorld h;
         int live = 0;
         uint64_t ts = 0;
         bool pend = false;
-        int tileX = 0, tileY = 0, hint = 0;
+        int strip = kStripMax, hint = 0;
         bool work = false;
         if (src.image) {
             if (src.from == this) throw Error(ALIEN_E_ARG, "a world set cannot be rebuilt from itself");
@@ -409,15 +412,14 @@ void WorldSet::build(const std::vector<Source>& sources) {
                     work = true;
             }
             const int bs = bin_size_for(h.params);
-            choose_tile(im.pos, h.desc.worldSizeX, h.desc.worldSizeY, bs, tileX, tileY);
+            strip = choose_strip(im.pos, h.desc.worldSizeX, h.desc.worldSizeY, bs);
         } else {
             if (src.from == this) throw Error(ALIEN_E_ARG, "a world set cannot be rebuilt from itself");
             h = src.from->hosts_.at(src.fromIndex);
             live = src.from->numParticles(src.fromIndex);
             ts = src.from->timestep(src.fromIndex);
             pend = src.from->densityPending_.at(src.fromIndex) != 0;
-            tileX = src.from->worlds_.at(src.fromIndex).tileX;
-            tileY = src.from->worlds_.at(src.fromIndex).tileY;
+            strip = src.from->stripChoice_.at(src.fromIndex);
             hint = src.from->stagedHint_.at(src.fromIndex);
             work = src.from->bondWork_.at(src.fromIndex) != 0;
         }
@@ -436,10 +438,8 @@ void WorldSet::build(const std::vector<Source>& sources) {
         w.binOff = bins;
         w.partOff = partOff;
         w.partCap = live;
-        w.tileX = std::max(1, std::min(tileX, w.nbx));
-        w.tileY = std::max(1, std::min(tileY, w.nby));
-        w.ntx = (w.nbx + w.tileX - 1) / w.tileX;
-        w.nty = (w.nby + w.tileY - 1) / w.tileY;
+        strip_shape(w, std::min(strip, stripCap_));
+        stripChoice.push_back(strip);
         cellOff += h.numCells;
         partOff += live;
         bins += w.nbx * w.nby;
@@ -504,6 +504,11 @@ void WorldSet::build(const std::vector<Source>& sources) {
     start_ = static_cast<uint32_t*>(A(sizeof(uint32_t) * (size_t(NB) + 1)));
     blockSums_ = static_cast<uint32_t*>(A(sizeof(uint32_t) * (size_t(NB) / 2048 + 2)));
     sorted_ = static_cast<int*>(A(sizeof(int) * NC));
+    mpos_ = static_cast<float2*>(A(sizeof(float2) * NC));
+    mvel_ = static_cast<float2*>(A(sizeof(float2) * NC));
+    mdens_ = static_cast<float*>(A(sizeof(float) * NC));
+    mflags_ = static_cast<uint32_t*>(A(sizeof(uint32_t) * NC));
+    mslot_ = static_cast<int*>(A(sizeof(int) * NC));
     newidx_ = static_cast<int*>(A(sizeof(int) * NC));
     psorted_ = static_cast<int*>(A(sizeof(int) * NP));
     staged_ = static_cast<int2*>(A(sizeof(int2) * NC));
@@ -592,6 +597,7 @@ void WorldSet::build(const std::vector<Source>& sources) {
     densityPending_ = std::move(pending);
     stagedHint_ = std::move(stagedHint);
     bondWork_ = std::move(bondWork);
+    stripChoice_ = std::move(stripChoice);
     shape_.hasBondWork = false;
     for (uint8_t b : bondWork_) shape_.hasBondWork = shape_.hasBondWork || b != 0;
     parity_ = 0;
@@ -629,10 +635,9 @@ void WorldSet::build(const std::vector<Source>& sources) {
     synchronize();
 }
 
-// Tile shape of the neighbour pass for one world: the largest tile (about one interior cell per
-// thread) whose fullest halo, measured on the given positions, stays within 3/4 of the LDS
-// budget.  Only performance depends on it.
-void choose_tile(const std::vector<float2>& pos, int W, int H, int binSize, int& tileX, int& tileY) {
+// Strip shape of the neighbour pass for one world, measured on the given positions.  Only
+// performance depends on it.
+int choose_strip(const std::vector<float2>& pos, int W, int H, int binSize) {
     const int nbx = W / binSize, nby = H / binSize;
     std::vector<int> cnt(size_t(nbx) * nby, 0);
     for (const float2& p : pos) {
@@ -655,30 +660,25 @@ void choose_tile(const std::vector<float2>& pos, int W, int H, int binSize, int&
         return pre[size_t(y1) * (nbx + 1) + x1] - pre[size_t(y0) * (nbx + 1) + x1] - pre[size_t(y1) * (nbx + 1) + x0] +
                pre[size_t(y0) * (nbx + 1) + x0];
     };
-    const int xs[] = {64, 32, 16, 8, 4};
-    const int ys[] = {16, 8, 4, 2, 1};
-    int bestArea = 0;
-    tileX = 4;
-    tileY = 1;
-    for (int tx : xs)
-        for (int ty : ys) {
-            if ((tx + 2) * (ty + 2) > kMaxHaloBins) continue;
-            int64_t maxHalo = 0, sumInterior = 0, tilesCount = 0;
-            for (int y0 = 0; y0 < nby; y0 += ty)
-                for (int x0 = 0; x0 < nbx; x0 += tx) {
-                    maxHalo = std::max(maxHalo, window(x0 - 1, y0 - 1, x0 + tx + 1, y0 + ty + 1));
-                    sumInterior += window(x0, y0, x0 + tx, y0 + ty);
-                    ++tilesCount;
-                }
-            const int64_t meanInterior = tilesCount ? sumInterior / tilesCount : 0;
-            if (maxHalo * 4 > int64_t(kTileCap) * 3 || meanInterior > kTileThreads) continue;
-            const int area = tx * ty;
-            if (area > bestArea) {
-                bestArea = area;
-                tileX = tx;
-                tileY = ty;
-            }
-        }
+    // widest strip whose staged rows (strip plus halo columns, one bin row) stay within 3/4 of the
+    // LDS row capacity everywhere; denser rows take the slower global path
+    for (int sw : {kStripMax, 32, 16, 8}) {
+        int64_t maxRow = 0;
+        for (int y = 0; y < nby; ++y)
+            for (int x0 = 0; x0 < nbx; x0 += sw) maxRow = std::max(maxRow, window(x0 - 1, y, x0 + sw + 1, y + 1));
+        if (maxRow * 4 <= int64_t(kRowCap) * 3) return sw;
+    }
+    return 4;
+}
+
+// Strip width (capped by the world's columns), rows per chunk (long enough to reuse the staged
+// rows, short enough to give the whole chip work) and the strip/chunk counts.
+void strip_shape(DevWorld& w, int strip) {
+    w.tileX = std::max(1, std::min(strip, w.nbx));
+    w.ntx = (w.nbx + w.tileX - 1) / w.tileX;
+    w.tileY = std::max(2, std::min(16, int((int64_t(w.ntx) * w.nby) / 4096)));
+    w.tileY = std::min(w.tileY, w.nby);
+    w.nty = (w.nby + w.tileY - 1) / w.tileY;
 }
 
 void WorldSet::buildTables() {
@@ -744,6 +744,16 @@ void WorldSet::setBlockSize(int threads) {
     if (!worlds_.empty()) buildTables();
 }
 
+void WorldSet::setStripCap(int cap) {
+    if (cap != 8 && cap != 16 && cap != 32 && cap != kStripMax)
+        throw Error(ALIEN_E_ARG, "strip cap must be 8, 16, 32 or 64");
+    stripCap_ = cap;
+    if (worlds_.empty()) return;
+    for (size_t i = 0; i < worlds_.size(); ++i) strip_shape(worlds_[i], std::min(stripChoice_[i], cap));
+    check_hip(hipMemcpy(dWorlds_, worlds_.data(), sizeof(DevWorld) * worlds_.size(), hipMemcpyHostToDevice), "H2D");
+    buildTables();
+}
+
 void WorldSet::setUseGraph(bool on) {
     useGraph_ = on;
     invalidateGraphs();
@@ -781,6 +791,11 @@ Ctx WorldSet::makeCtx(int parity) const {
     c.start = start_;
     c.blockSums = blockSums_;
     c.sorted = sorted_;
+    c.mpos = mpos_;
+    c.mvel = mvel_;
+    c.mdens = mdens_;
+    c.mflags = mflags_;
+    c.mslot = mslot_;
     c.newidx = newidx_;
     c.psorted = psorted_;
     c.staged = staged_;

@@ -33,7 +33,8 @@ int bin_size_for(const alien_params& p);
 void validate_params(const alien_params& p);
 void validate_world(const alien_world_desc& d, const alien_params& p);
 DevParams to_dev_params(const alien_params& p);
-void choose_tile(const std::vector<float2>& pos, int W, int H, int binSize, int& tileX, int& tileY);
+int choose_strip(const std::vector<float2>& pos, int W, int H, int binSize);
+void strip_shape(DevWorld& w, int strip);
 constexpr int kXcds = 8;  // accelerator complex dies of an MI355X
 
 // host-side identity and pass-through data of one world (fields the physics path never touches)
@@ -113,6 +114,9 @@ public:
 
     void setBlockSize(int threads);
     int blockSize() const { return shape_.cellThreads; }
+    // neighbour-pass strips no wider than cap bins (8, 16, 32 or 64); launch shape only
+    void setStripCap(int cap);
+    int stripCap() const { return stripCap_; }
     void setUseGraph(bool on);
     // kernel timing mode: no graphs, every launch bracketed by events (see alien_set_kernel_timing)
     void setKernelTiming(bool on);
@@ -163,6 +167,10 @@ private:
     uint32_t* count_[2] = {nullptr, nullptr};  // bin histograms, alternating with the state buffers
     uint32_t *start_ = nullptr, *blockSums_ = nullptr;
     int *sorted_ = nullptr, *newidx_ = nullptr, *psorted_ = nullptr;
+    float2 *mpos_ = nullptr, *mvel_ = nullptr;
+    float* mdens_ = nullptr;
+    uint32_t* mflags_ = nullptr;
+    int* mslot_ = nullptr;
     int2* staged_ = nullptr;
     float2 *own_ = nullptr, *shift_ = nullptr, *s2_ = nullptr, *out1_ = nullptr, *out2_ = nullptr;
     uint32_t* valid_ = nullptr;
@@ -175,6 +183,8 @@ private:
     int numPartBlocks_ = 0;
     std::vector<int> stagedHint_;    // per world: bonded + barrier cells, sizes the staged-list grids
     std::vector<uint8_t> bondWork_;  // per world: a bonded or barrier cell exists or can appear by fusion
+    std::vector<int> stripChoice_;   // per world: strip width chosen from its cell density
+    int stripCap_ = kStripMax;
     void adaptGrids();
     std::vector<Arr> tableAllocs_;
 

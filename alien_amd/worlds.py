@@ -296,31 +296,70 @@ def batch_world(rng, side, density, steps, seed=0):
     return World(f"batch_{side}", size, params, cells, parts, steps, seed)
 
 
-def batched_worlds(seed=SEED_BASE + 3, count=256, side_range=(256, 2048), density_range=(0.02, 0.3), steps=500):
+def batched_world_specs(seed=SEED_BASE + 3, count=256, side_range=(256, 2048), density_range=(0.02, 0.3)):
+    """(side, density, child seed) of every world of the batched workload: sides log-uniform,
+    densities uniform, each world filled from its own child seed (so worlds can be built in any
+    order or in parallel)."""
     rng = np.random.default_rng(seed)
     lo, hi = math.log(side_range[0]), math.log(side_range[1])
     sides = np.exp(rng.uniform(lo, hi, size=count)).astype(np.int64)
     dens = rng.uniform(density_range[0], density_range[1], size=count)
-    return [batch_world(rng, s, d, steps, seed) for s, d in zip(sides, dens)]
+    children = np.random.SeedSequence(seed).spawn(count)
+    return [(int(s), float(d), c) for s, d, c in zip(sides, dens, children)]
 
 
-def serving_jobs(seed=SEED_BASE + 4, rate=20.0, duration=600.0, side_range=(256, 2048),
-                 density_range=(0.02, 0.3), step_range=(100, 10_000), max_jobs=None):
-    """Poisson arrivals: list of (arrival_time, World)."""
+def batched_world(spec, steps=500):
+    side, dens, child = spec
+    return batch_world(np.random.default_rng(child), side, dens, steps, seed=int(child.entropy))
+
+
+def batched_worlds(seed=SEED_BASE + 3, count=256, side_range=(256, 2048), density_range=(0.02, 0.3), steps=500,
+                   processes=1):
+    specs = batched_world_specs(seed, count, side_range, density_range)
+    if processes <= 1:
+        return [batched_world(sp, steps) for sp in specs]
+    import functools
+    import multiprocessing
+
+    with multiprocessing.get_context("fork").Pool(processes) as pool:
+        return pool.map(functools.partial(batched_world, steps=steps), specs, chunksize=1)
+
+
+def serving_job_specs(seed=SEED_BASE + 4, rate=20.0, duration=600.0, side_range=(256, 2048),
+                      density_range=(0.02, 0.3), step_range=(100, 10_000), max_jobs=None):
+    """Poisson arrivals: list of (arrival_time, side, density, steps, child seed)."""
     rng = np.random.default_rng(seed)
     t = 0.0
-    jobs = []
+    specs = []
     lo, hi = math.log(side_range[0]), math.log(side_range[1])
     slo, shi = math.log(step_range[0]), math.log(step_range[1])
     while True:
         t += rng.exponential(1.0 / rate)
-        if t > duration or (max_jobs is not None and len(jobs) >= max_jobs):
+        if t > duration or (max_jobs is not None and len(specs) >= max_jobs):
             break
         side = int(math.exp(rng.uniform(lo, hi)))
         dens = rng.uniform(*density_range)
         steps = int(math.exp(rng.uniform(slo, shi)))
-        jobs.append((t, batch_world(rng, side, dens, steps)))
-    return jobs
+        specs.append((t, side, dens, steps))
+    children = np.random.SeedSequence(seed).spawn(len(specs))
+    return [sp + (ch,) for sp, ch in zip(specs, children)]
+
+
+def serving_job(spec):
+    t, side, dens, steps, child = spec
+    return t, batch_world(np.random.default_rng(child), side, dens, steps, seed=int(child.entropy))
+
+
+def serving_jobs(seed=SEED_BASE + 4, rate=20.0, duration=600.0, side_range=(256, 2048),
+                 density_range=(0.02, 0.3), step_range=(100, 10_000), max_jobs=None, processes=1):
+    """Poisson arrivals: list of (arrival_time, World)."""
+    specs = serving_job_specs(seed, rate, duration, side_range, density_range, step_range, max_jobs)
+    if processes <= 1:
+        return [serving_job(sp) for sp in specs]
+    import multiprocessing
+
+    with multiprocessing.get_context("fork").Pool(processes) as pool:
+        return pool.map(serving_job, specs, chunksize=1)
 
 
 def random_mix(seed, size=(64, 64), num_free=300, patches=6, num_particles=200, num_barrier=20,

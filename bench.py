@@ -40,7 +40,30 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="bounded CPU baseline sample length")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-kernel-timing", action="store_true")
+    ap.add_argument("--workload", choices=["large_fluid", "batch", "serving"], default="large_fluid",
+                    help="large_fluid: the headline line; batch: 256 uneven worlds in one graph vs per-world "
+                         "sequential runs; serving: Poisson job arrivals through the job server")
+    ap.add_argument("--batch-worlds", type=int, default=256)
+    ap.add_argument("--serving-seconds", type=float, default=60.0, help="arrival window of the serving run")
+    ap.add_argument("--serving-rate", type=float, default=20.0)
+    ap.add_argument("--budget-gb", type=float, default=200.0)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run only a plain device copy of known size (calibrates the HBM byte counters)")
     return ap.parse_args()
+
+
+def copy_calibration():
+    """A plain copy kernel moving a known number of bytes, for scaling rocprofv3 FETCH_SIZE /
+    WRITE_SIZE readings into HBM bytes (read 256 MiB, write 256 MiB per launch)."""
+    import torch
+
+    n = 64 * 1024 * 1024  # float32 elements = 256 MiB
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(5):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration": {"bytes_read": 4 * n, "bytes_written": 4 * n, "launches": 5}}), flush=True)
 
 
 def cpu_baseline(world, seconds):
@@ -73,12 +96,115 @@ def cpu_baseline(world, seconds):
     }
 
 
+def gen_processes():
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(n, 16))
+
+
+def bench_batch(args):
+    """256 worlds of uneven size (side log-uniform in [256, 2048], density U(0.02, 0.3), soft-body
+    patches, collision mode) advanced together, one captured graph replay per step, against the
+    same worlds run one after another as plain single-world simulations of the same library."""
+    from alien_amd import engine, worlds
+
+    t0 = time.perf_counter()
+    ws = worlds.batched_worlds(count=args.batch_worlds, steps=args.steps, processes=gen_processes())
+    gen_s = time.perf_counter() - t0
+    cells = sum(len(w.cells) for w in ws)
+    b = engine.Batch()
+    for w in ws:
+        b.add(w)
+    b.step(args.warmup)
+    tune = b.autotune(trial_steps=max(2, min(10, args.steps)))
+    t0 = time.perf_counter()
+    b.step(args.steps)
+    batched_s = time.perf_counter() - t0
+    b.close()
+    # per-world sequential: each world alone, same steps, same library
+    seq_s = 0.0
+    for w in ws:
+        sim = engine.Simulation.from_world(w)
+        sim.step(args.warmup)
+        t0 = time.perf_counter()
+        sim.step(args.steps)
+        seq_s += time.perf_counter() - t0
+        sim.close()
+    value = cells * args.steps / batched_s
+    seq_value = cells * args.steps / seq_s
+    print(json.dumps({
+        "metric": "cell-timesteps/s (256 uneven worlds, batched, 1 GPU)", "value": value,
+        "unit": "cell-timesteps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": 1000.0 * batched_s / args.steps, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "batched_worlds", "worlds": len(ws), "cells": cells,
+                   "sides": [int(min(w.size[0] for w in ws)), int(max(w.size[0] for w in ws))],
+                   "seed": worlds.SEED_BASE + 3, "device": engine.gpu_name()},
+        "per_world_sequential": {"value": seq_value, "seconds": seq_s},
+        "batched_over_sequential": value / seq_value,
+        "autotune": {"block": tune[0], "strip_cap": tune[1], "trial_ms_per_step": tune[2]},
+        "generation_s": gen_s,
+    }), flush=True)
+
+
+def bench_serving(args):
+    """Poisson job arrivals (rate, window) of batched-workload worlds with log-uniform step
+    counts, admitted by an HBM budget into the continuous-batching server; latency = submit to
+    snapshot ready."""
+    from alien_amd import engine, worlds
+
+    t0 = time.perf_counter()
+    jobs = worlds.serving_jobs(rate=args.serving_rate, duration=args.serving_seconds, processes=gen_processes())
+    gen_s = time.perf_counter() - t0
+    srv = engine.Server(budget_bytes=int(args.budget_gb * 2**30), chunk_steps=50)
+    ids = []
+    start = time.perf_counter()
+    for t_arr, w in jobs:
+        delay = t_arr - (time.perf_counter() - start)
+        if delay > 0:
+            time.sleep(delay)
+        ids.append(srv.submit(w.size, w.params, w.cells, w.particles, w.steps))
+    for j in ids:
+        srv.wait(j, timeout=3600)
+    infos = [srv.poll(j) for j in ids]
+    lat = sorted(i.finishTime - i.submitTime for i in infos)
+    work = sum(i.numCells * i.stepsDone for i in infos)
+    span = max(i.finishTime for i in infos) - min(i.submitTime for i in infos)
+    srv.close()
+
+    def pct(q):
+        return lat[min(len(lat) - 1, int(q * (len(lat) - 1) + 0.5))]
+
+    print(json.dumps({
+        "metric": "serving job latency p50/p99 and aggregate cell-timesteps/s (1 GPU)", "value": work / span,
+        "unit": "cell-timesteps/s", "n_gpus": 1, "steps": None, "warmup": 0, "ms_per_step": None,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "serving", "rate_per_s": args.serving_rate, "window_s": args.serving_seconds,
+                   "jobs": len(jobs), "budget_gb": args.budget_gb, "seed": worlds.SEED_BASE + 4,
+                   "device": engine.gpu_name()},
+        "latency_s": {"p50": pct(0.5), "p99": pct(0.99), "max": lat[-1]},
+        "generation_s": gen_s,
+    }), flush=True)
+
+
 def main():
     args = parse()
+    if args.copy_calibration:
+        copy_calibration()
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (--gpus {args.gpus} requested); "
               "independent worlds would shard one world per GPU with no exchange.", file=sys.stderr)
         sys.exit(2)
+
+    if args.workload == "batch":
+        bench_batch(args)
+        return
+    if args.workload == "serving":
+        bench_serving(args)
+        return
 
     from alien_amd import engine, worlds
 

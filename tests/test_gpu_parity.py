@@ -474,3 +474,21 @@ def test_server_budget_admission():
         ref.step(10)
         assert_identical(srv.snapshot(j), ref.get_data(), f"budgeted job {j}")
     srv.close()
+
+
+def test_batch_autotune_keeps_results():
+    """The autotuner times launch shapes on a copy: the batch's worlds do not advance and the
+    results after tuning are the oracle's."""
+    ws = mixed_worlds(6)
+    b = engine.Batch()
+    for w in ws:
+        b.add(w)
+    b.step(2)
+    blk, strip, ms = b.autotune(trial_steps=3)
+    assert blk in (64, 128, 256) and strip in (16, 32, 64) and ms > 0
+    assert all(b.timestep(i) == 2 for i in range(len(ws)))
+    b.step(6)
+    for i, w in enumerate(ws):
+        ref = from_world(w)
+        ref.step(8)
+        assert_identical(b.get_data(i), ref.get_data(), f"tuned batch world {i}")
