@@ -33,10 +33,14 @@ __device__ __forceinline__ float dot2(f2 a, f2 b) { return a.x * b.x + a.y * b.y
 __device__ __forceinline__ float lensq(f2 v) { return v.x * v.x + v.y * v.y; }
 __device__ __forceinline__ float len2(f2 v) { return __builtin_sqrtf(v.x * v.x + v.y * v.y); }
 
-// unit vector with the (1,0) fallback below 1e-5 (Math.cuh:246-261)
+// unit vector with the (1,0) fallback below 1e-5 (Math.cuh:246-261), as the product with the
+// reciprocal length
 __device__ __forceinline__ f2 unit(f2 v) {
     float l = __builtin_sqrtf(v.x * v.x + v.y * v.y);
-    if (l > kNearZero) return {v.x / l, v.y / l};
+    if (l > kNearZero) {
+        const float inv = 1.0f / l;
+        return {v.x * inv, v.y * inv};
+    }
     return {1.0f, 0.0f};
 }
 

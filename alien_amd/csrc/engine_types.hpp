@@ -21,6 +21,7 @@ constexpr int kNone = -1;
 constexpr int kRowThreads = 64;
 constexpr int kRowCap = 128;
 constexpr int kStripMax = 64;
+constexpr int kChunkMax = 16;  // bin rows per neighbour-pass chunk
 constexpr int kClaimFree = 0x7fffffff;
 
 // cell flags word
@@ -55,6 +56,10 @@ struct DevParams {
     // interaction radius (2 h for fluid, cellMaxCollisionDistance for collision) and reachSq the
     // largest float whose correctly rounded square root does not exceed R
     float reachSq;
+    // the same for the other length comparisons of the pass: |v| > maxVelocity <=> |v|^2 > maxVelSq,
+    // |v| > 0.5 <=> |v|^2 > fastSq, |dv| < fusionVelocity <=> |dv|^2 <= fusionBelowSq
+    float maxVelSq, fastSq, fusionBelowSq;
+    float invH, invH2, invH3;  // 1/h, 1/h^2, 1/h^3 (fluid)
 };
 
 // per-world geometry and placement in the set
@@ -118,8 +123,16 @@ struct Ctx {
     float* stiff;   // cold [cellCap]
     int* binOf;     // [cellCap] global bin of A-cell i
     int* pbinOf;    // [partCap] global bin of A-particle i, -1 if dead
-    uint32_t* count;      // [numBins] bin sizes of the start-of-step state; scatter cursor
-    uint32_t* countNext;  // [numBins] bin sizes of the end-of-step state (next step's `count`)
+    // Bin sizes.  A cell bin's size is priv[b] + count[b]: priv holds the cells counted by the
+    // neighbour-pass wave that owns the bin (plain stores, every cell bin written every step),
+    // count the cells counted with global atomics (those that left their wave's region, bonded
+    // and barrier cells, seeding, and all particle bins).  The scatter counts `count` back down.
+    uint32_t* count;      // [numBins] start-of-step state
+    uint32_t* countNext;  // [numBins] end-of-step state (next step's `count`)
+    uint32_t* priv;       // [numCellBins] start-of-step state
+    uint32_t* privNext;   // [numCellBins] end-of-step state
+    int* arrival;         // [cellCap] per end-of-step cell: its place among its bin's arrivals
+    int numCellBins;
     uint32_t* start;      // [numBins + 1] exclusive scan
     uint32_t* blockSums;
     int* sorted;   // [cellCap] map order (bin, then arrival) -> A index

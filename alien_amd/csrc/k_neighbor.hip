@@ -5,20 +5,19 @@
 // cells without bonds also the Verlet updates and friction of the same step (:525-559, :707-723),
 // which reduce to local operations when the connection forces are zero.
 //
-// One workgroup owns a tile of TX x TY bins.
-//  1. Staging: the bin counters of the tile and its one-bin halo give every halo bin its range of
-//     LDS entries; one thread per entry loads the cell, then every entry is ranked inside its bin by
-//     previous index (the stable in-bin order of the cell map, R1) into the permutation `perm`.
-//  2. Candidates: one thread per interior cell walks the 3x3 bin stencil (three contiguous slot
-//     ranges) with the neighbour predicate evaluated on squared lengths, and records the
-//     neighbours as a bit mask in visiting order.
-//  3. Pair terms: the neighbour pairs of all threads are laid out back to back (block scan of the
-//     mask popcounts) and evaluated 128 at a time with every lane busy; the terms go to LDS.
-//  4. Each thread adds its own pairs' terms in visiting order (the float sums are the same as a
-//     sequential walk of the stencil), then finishes the cell.
-// A cell with more than 64 stencil candidates, or a tile whose halo does not fit LDS, walks its
-// stencil sequentially instead (same order, same results).  The kernel is instantiated per motion
-// mode; a workgroup of a world in the other mode exits at once.
+// One wave owns a strip of bin columns and walks it row by row (see "Row streaming" below).  Per
+// row it (1) tests the 3x3 bin stencil of every cell with the neighbour predicate on squared
+// lengths and records the neighbours as a bit mask in visiting order, (2) evaluates all neighbour
+// pairs of the row one pair per lane, and (3) lets every cell add its pairs' terms in visiting
+// order, which gives the float sums of a sequential stencil walk bit for bit.  A cell with more
+// than 64 stencil candidates, or a strip whose rows do not fit LDS, walks its stencil
+// sequentially instead (same order, same results).  The kernel is instantiated per motion mode; a
+// workgroup of a world in the other mode exits at once.
+//
+// Divisions by per-world constants (1/h, 1/h^2, 1/h^3), by a cell's density, by |d| and by the
+// collision and viscosity denominators are products with the correctly rounded reciprocal (R16
+// of the oracle), and the length comparisons |v| > c are tests of |v|^2 against the exact
+// threshold of the correctly rounded square root.
 #include "kernels_common.hpp"
 #include "kernels.hpp"
 
@@ -26,7 +25,7 @@ namespace alien {
 
 struct Nb {
     f2 pos, vel;
-    float dens;
+    float dens;  // fluid: 1 / density; collision: density
     uint32_t flags;
     int aidx;  // index in the start-of-step arrays (A)
     int lidx;  // index in the end-of-step arrays (B), the logical index of this step
@@ -49,10 +48,11 @@ constexpr uint32_t kPairForce = 8u;    // force terms
 constexpr int kPairTerms = 7;
 
 // neighbour predicate (fluid: CellProcessor.cuh:137-143; collision: Map.cuh:189 with
-// CellProcessor.cuh:290-296): |d| <= R tested as |d|^2 <= reachSq, which is exactly equivalent
+// CellProcessor.cuh:290-296): |d| <= R tested as |d|^2 <= reachSq, which is exactly equivalent.
+// Cells are never detached here (detached cells come from cell functions), so the detached rule
+// never applies.
 template <bool kFluid>
 __device__ __forceinline__ bool is_neighbor(const Nb& s, const Nb& o, float Wf, float Hf, float reachSq) {
-    if (flags_detached(s.flags) + flags_detached(o.flags) == 1) return false;
     if constexpr (kFluid)
         return lensq(torus_delta(s.pos, o.pos, Wf, Hf)) <= reachSq;
     else
@@ -76,8 +76,7 @@ __device__ __forceinline__ uint32_t pair_term(const Ctx& c, const DevWorld& w, c
             t[1] = d;
             return kPairBarrier;
         }
-        const float h = P.smoothing;
-        t[0] = sph_w(d / h) / (h * h);
+        t[0] = sph_w(d * P.invH) * P.invH2;
         if (o.aidx == s.aidx) return kPairSelf;
         if (!bari && d < cmd) {
             const f2 D = dvd(mul(pd, cmd), 5.0f);
@@ -86,27 +85,27 @@ __device__ __forceinline__ uint32_t pair_term(const Ctx& c, const DevWorld& w, c
             f |= kPairOverlap;
         }
         if (bonded) return f;
-        const float di = s.dens, dj = o.dens;
         const f2 vd = sub(s.vel, o.vel);
-        const float factor = di / (di * di) + dj / (dj * dj);
+        const float factor = s.dens + o.dens;  // 1/density_i + 1/density_j
         if (fabsf(d) > kNearZero) {
-            const float kd = sph_dw(d / h) / (h * h * h);
-            const f2 Fp = mul(mul(dvd(pd, -d), factor), kd);
-            const f2 Fv = dvd(mul(mul(dvd(vd, dj), d), kd), d * d + 0.25f);
+            const float kd = sph_dw(d * P.invH) * P.invH3;
+            const f2 Fp = mul(mul(mul(pd, 1.0f / (-d)), factor), kd);
+            const f2 Fv = mul(mul(mul(mul(vd, o.dens), d), kd), 1.0f / (d * d + 0.25f));
             t[3] = Fp.x;
             t[4] = Fp.y;
             t[5] = Fv.x;
             t[6] = Fv.y;
             f |= kPairForce;
         }
-        if (len2(vd) >= P.fusionVelocity && s.lidx < o.lidx && flags_num_conn(s.flags) < flags_max_conn(s.flags) &&
-            flags_num_conn(o.flags) < flags_max_conn(o.flags) && !bari) {
+        if (flags_num_conn(s.flags) < flags_max_conn(s.flags) && flags_num_conn(o.flags) < flags_max_conn(o.flags) &&
+            !bari && s.lidx < o.lidx && lensq(vd) > P.fusionBelowSq) {
             const float ei = c.energy[w.cellOff + c.A.slot[s.aidx]];
             const float ej = c.energy[w.cellOff + c.A.slot[o.aidx]];
             if (ei <= P.maxBindingEnergy && ej <= P.maxBindingEnergy) push_candidate(c, s.lidx, o.lidx, wi);
         }
     } else {
-        // CellProcessor.cuh:297-361
+        // CellProcessor.cuh:297-361.  f(j,i) mirrors f(i,j): unit(-d) = -unit(d) and
+        // (-dv).(-d) = dv.d exactly, so only the barrier factor and the branch differ.
         if (!bari && d < cmd) {
             const f2 D = dvd(mul(pd, cmd), 5.0f);
             t[0] = D.x;
@@ -114,39 +113,27 @@ __device__ __forceinline__ uint32_t pair_term(const Ctx& c, const DevWorld& w, c
             f |= kPairOverlap;
         }
         if (bonded) return f;
-        const float r = P.collisionDistance;
         const f2 vd = sub(s.vel, o.vel);
+        const float dvp = dot2(vd, pd);
         const bool appr = dot2(pd, vd) < 0.0f;
         const bool barj = flags_barrier(o.flags);
-        f2 fij, fji;
-        {
-            const float bf = bari ? 2.0f : 1.0f;
-            if (len2(s.vel) > 0.5f && appr) {
-                const float dsq = d * d + 0.25f;
-                fij = mul(dvd(mul(pd, dot2(vd, pd)), -2.0f * dsq), bf);
-            } else {
-                fij = mul(mul(mul(unit(pd), r - len2(pd)), P.repulsion), bf);
-            }
-        }
-        {
-            const f2 pdn = neg(pd), vdn = neg(vd);
-            const float bf = barj ? 2.0f : 1.0f;
-            if (len2(o.vel) > 0.5f && appr) {
-                const float dsq = d * d + 0.25f;
-                fji = mul(dvd(mul(pdn, dot2(vdn, pdn)), -2.0f * dsq), bf);
-            } else {
-                fji = mul(mul(mul(unit(pdn), r - len2(pdn)), P.repulsion), bf);
-            }
-        }
-        const f2 nfji = neg(fji);
+        const bool fastI = appr && lensq(s.vel) > P.fastSq;
+        const bool fastJ = appr && lensq(o.vel) > P.fastSq;
+        f2 fastT = {0.0f, 0.0f}, repT = {0.0f, 0.0f};
+        if (fastI || fastJ) fastT = mul(mul(pd, dvp), 1.0f / (-2.0f * (d * d + 0.25f)));
+        if (!(fastI && fastJ)) repT = mul(mul(unit(pd), P.collisionDistance - d), P.repulsion);
+        const f2 fij = mul(fastI ? fastT : repT, bari ? 2.0f : 1.0f);
+        // -f(j,i); unit() of a (near) zero vector is (1,0) in both directions, so there the
+        // repulsion is not mirrored
+        const f2 fjRep = mul(repT, barj ? 2.0f : 1.0f);
+        const f2 nfji = fastJ ? mul(fastT, barj ? 2.0f : 1.0f) : (d > kNearZero ? fjRep : neg(fjRep));
         t[2] = fij.x;
         t[3] = fij.y;
         t[4] = nfji.x;
         t[5] = nfji.y;
         f |= kPairForce;
-        if (s.lidx < o.lidx && flags_num_conn(s.flags) < flags_max_conn(s.flags) &&
-            flags_num_conn(o.flags) < flags_max_conn(o.flags) && len2(vd) >= P.fusionVelocity && appr && !bari &&
-            !barj) {
+        if (flags_num_conn(s.flags) < flags_max_conn(s.flags) && flags_num_conn(o.flags) < flags_max_conn(o.flags) &&
+            appr && !bari && !barj && s.lidx < o.lidx && lensq(vd) > P.fusionBelowSq) {
             const float ei = c.energy[w.cellOff + c.A.slot[s.aidx]];
             const float ej = c.energy[w.cellOff + c.A.slot[o.aidx]];
             if (ei <= P.maxBindingEnergy && ej <= P.maxBindingEnergy) push_candidate(c, s.lidx, o.lidx, wi);
@@ -218,10 +205,32 @@ __device__ __forceinline__ bool is_bonded(const int bt[kMaxBonds], int jLocal) {
     return bonded;
 }
 
+// The bins a neighbour-pass wave owns (x0..x0+IX-1, y0..y0+IY-1) and its LDS histogram of the
+// next step's cells in them; hist == nullptr counts every cell in the shared histogram.
+struct Region {
+    uint32_t* hist;
+    int x0, y0, IX, IY;
+};
+
+// Next-step bin of an unbonded cell: counted in the owning wave's LDS histogram when the bin is
+// in the wave's region, else in the shared histogram.
+__device__ __forceinline__ void bin_next(const Ctx& c, const DevWorld& w, int k, f2 p, const Region& R) {
+    int sx, sy;
+    slot_of(p, w.W, w.H, sx, sy);
+    const int bx = min(div_small(sx, w.binSize), w.nbx - 1), by = min(div_small(sy, w.binSize), w.nby - 1);
+    const int gb = w.binOff + by * w.nbx + bx;
+    c.binOf[k] = gb;
+    const int lx = bx - R.x0, ly = by - R.y0;
+    if (R.hist && lx >= 0 && lx < R.IX && ly >= 0 && ly < R.IY)
+        c.arrival[k] = int(atomicAdd(&R.hist[ly * R.IX + lx], 1u));
+    else
+        c.arrival[k] = kArrivalShared | int(atomicAdd(&c.countNext[gb], 1u));
+}
+
 // Barrier reflection, applyForces, and the end-of-step writes of one cell.
 template <bool kFluid>
 __device__ __forceinline__ void cell_finish(const Ctx& c, const DevWorld& w, const DevParams& P, int wi, const Nb& s,
-                                            const int bt[kMaxBonds], Acc& a) {
+                                            const int bt[kMaxBonds], Acc& a, const Region& R) {
     const int cap = c.cellCap;
     const float Wf = float(w.W), Hf = float(w.H);
     const int i = s.aidx, k = s.lidx;
@@ -280,20 +289,20 @@ __device__ __forceinline__ void cell_finish(const Ctx& c, const DevWorld& w, con
     f2 v = vi;
     if (!bari) {  // applyForces
         v = add(v, F);
-        if (len2(v) > P.maxVelocity) v = mul(unit(v), P.maxVelocity);
+        if (lensq(v) > P.maxVelSq) v = mul(unit(v), P.maxVelocity);
     }
     if (!bari && nci == 0) {
         // no bonds: both connection-force passes yield zero, so the Verlet position and velocity
         // updates and friction of this step are applied here with the same operations
         const f2 zero = {0.0f, 0.0f};
         const float dt = P.dt;
-        const f2 p2 = torus_pos(add(pc, add(mul(v, dt), dvd(mul(mul(zero, dt), dt), 2.0f))), w.W, w.H);
-        const f2 acc = dvd(add(zero, zero), 2.0f);
+        const f2 p2 = torus_pos(add(pc, add(mul(v, dt), mul(mul(mul(zero, dt), dt), 0.5f))), w.W, w.H);
+        const f2 acc = mul(add(zero, zero), 0.5f);
         v = add(v, mul(acc, dt));
         v = mul(v, 1.0f - P.friction);
         c.B.pos[k] = to_float2(p2);
         c.B.vel[k] = to_float2(v);
-        bin_for_next_step(c, w, k, p2, c.countNext);
+        bin_next(c, w, k, p2, R);
     } else {
         c.B.pos[k] = to_float2(pc);
         c.B.vel[k] = to_float2(v);
@@ -305,7 +314,7 @@ __device__ __forceinline__ void cell_finish(const Ctx& c, const DevWorld& w, con
 // One cell, neighbours visited one after another (forEachNb calls visit(o) in stencil order).
 template <bool kFluid, class ForEach>
 __device__ __forceinline__ void cell_direct(const Ctx& c, const DevWorld& w, const DevParams& P, int wi, const Nb& s,
-                                            ForEach&& forEachNb) {
+                                            const Region& R, ForEach&& forEachNb) {
     const float Wf = float(w.W), Hf = float(w.H);
     int bt[kMaxBonds];
     load_bonds(c, s, bt);
@@ -317,7 +326,7 @@ __device__ __forceinline__ void cell_direct(const Ctx& c, const DevWorld& w, con
         const uint32_t f = pair_term<kFluid>(c, w, P, wi, s, o, is_bonded(bt, o.aidx - w.cellOff), t);
         acc_add<kFluid>(a, f, t);
     });
-    cell_finish<kFluid>(c, w, P, wi, s, bt, a);
+    cell_finish<kFluid>(c, w, P, wi, s, bt, a, R);
 }
 
 
@@ -334,13 +343,13 @@ constexpr int kPairCap = 256;  // pair records per evaluation window
 
 struct RowShared {
     int rs[3][kRowBins + 1];  // per staged row: first slot of every halo-column bin, then the total
-    int rm[3][kRowBins];      // map position of every bin's first entry
     float2 pos[3][kRowCap];   // entries by slot
     float2 vel[3][kRowCap];
-    float dens[3][kRowCap];
+    float dens[3][kRowCap];   // fluid: 1 / density; collision: density
     uint32_t flags[3][kRowCap];
     int aidx[3][kRowCap];
     int slot[3][kRowCap];
+    int lidx[3][kRowCap];      // logical index of this step (position in the cell map)
     uint8_t sbin[3][kRowCap];  // slot -> halo column
     // the row in flight
     int nrs[kRowBins + 1];
@@ -353,6 +362,7 @@ struct RowShared {
     uint16_t pnb[kPairCap];
     float term[kPairTerms][kRowThreads];
     uint32_t pflag[kRowThreads];
+    uint32_t hist[kStripMax * kChunkMax];  // next-step cells of the wave's own bins
 };
 
 // registers of a row in flight: bin counters of the halo columns lane and lane + 64, then the
@@ -367,7 +377,6 @@ struct RowFlight {
 };
 
 __device__ __forceinline__ Nb row_nb(const RowShared& S, int r, int sl) {
-    const int hx = S.sbin[r][sl];
     Nb o;
     o.pos = to_f2(S.pos[r][sl]);
     o.vel = to_f2(S.vel[r][sl]);
@@ -375,15 +384,16 @@ __device__ __forceinline__ Nb row_nb(const RowShared& S, int r, int sl) {
     o.flags = S.flags[r][sl];
     o.aidx = S.aidx[r][sl];
     o.slot = S.slot[r][sl];
-    o.lidx = S.rm[r][hx] + (sl - S.rs[r][hx]);
+    o.lidx = S.lidx[r][sl];
     return o;
 }
 
+template <bool kFluid>
 __device__ __forceinline__ Nb global_nb(const Ctx& c, int a, int l) {
     Nb o;
     o.pos = to_f2(c.A.pos[a]);
     o.vel = to_f2(c.A.vel[a]);
-    o.dens = c.A.dens[a];
+    o.dens = kFluid ? 1.0f / c.A.dens[a] : c.A.dens[a];
     o.flags = c.A.flags[a];
     o.aidx = a;
     o.lidx = l;
@@ -455,10 +465,10 @@ __device__ __forceinline__ void row_issue_entries(const Ctx& c, const StripGeom&
 }
 
 // C: the row into ring slot r, every entry at its rank by previous index inside its bin (R1)
+template <bool kFluid>
 __device__ __forceinline__ void row_commit(const StripGeom& g, RowShared& S, const RowFlight& F, int r) {
     const int lane = threadIdx.x;
     for (int i = lane; i <= g.HX; i += kRowThreads) S.rs[r][i] = S.nrs[i];
-    for (int i = lane; i < g.HX; i += kRowThreads) S.rm[r][i] = S.nrm[i];
 #pragma unroll
     for (int q = 0; q < 2; ++q) {
         const int e = lane + q * kRowThreads;
@@ -477,11 +487,12 @@ __device__ __forceinline__ void row_commit(const StripGeom& g, RowShared& S, con
             const int sl = s0 + rank;
             S.pos[r][sl] = F.pos[q];
             S.vel[r][sl] = F.vel[q];
-            S.dens[r][sl] = F.dens[q];
+            S.dens[r][sl] = kFluid ? 1.0f / F.dens[q] : F.dens[q];
             S.flags[r][sl] = F.flags[q];
             S.aidx[r][sl] = a;
             S.slot[r][sl] = F.slot[q];
             S.sbin[r][sl] = uint8_t(hx);
+            S.lidx[r][sl] = S.nrm[hx] + rank;
         }
     }
     __syncthreads();
@@ -489,9 +500,7 @@ __device__ __forceinline__ void row_commit(const StripGeom& g, RowShared& S, con
 
 // neighbour predicate on staged data (is_neighbor, with the cell itself recognised by its slot)
 template <bool kFluid>
-__device__ __forceinline__ bool staged_neighbor(const Nb& s, f2 opos, uint32_t oflags, bool isSelf, float Wf, float Hf,
-                                                float reachSq) {
-    if (flags_detached(s.flags) + flags_detached(oflags) == 1) return false;
+__device__ __forceinline__ bool staged_neighbor(const Nb& s, f2 opos, bool isSelf, float Wf, float Hf, float reachSq) {
     if constexpr (kFluid)
         return lensq(torus_delta(s.pos, opos, Wf, Hf)) <= reachSq;
     else
@@ -501,11 +510,13 @@ __device__ __forceinline__ bool staged_neighbor(const Nb& s, f2 opos, uint32_t o
 // The cells of row y (ring slot rc; rows y-1 and y+1 in rm_ and rp).
 template <bool kFluid>
 __device__ __forceinline__ void row_compute(const Ctx& c, const DevWorld& w, const DevParams& P, int wi,
-                                            const StripGeom& g, RowShared& S, int rm_, int rc, int rp) {
+                                            const StripGeom& g, RowShared& S, const Region& R, int rm_, int rc,
+                                            int rp) {
     const int lane = threadIdx.x;
     const float Wf = float(w.W), Hf = float(w.H);
     const int cs = S.rs[rc][1];
     const int ncell = S.rs[rc][g.IX + 1] - cs;
+    const float w0 = kFluid ? sph_w(0.0f) * P.invH2 : 0.0f;  // a fluid cell's own density term
     for (int q0 = 0; q0 < ncell; q0 += kRowThreads) {
         // candidates of this lane's cell: three contiguous slot ranges, one per stencil row
         const int q = q0 + lane;
@@ -515,6 +526,7 @@ __device__ __forceinline__ void row_compute(const Ctx& c, const DevWorld& w, con
         int a0s[3] = {0, 0, 0}, ns[3] = {0, 0, 0};
         unsigned long long mask = 0ull, bmask = 0ull;
         bool direct = false;
+        int selfIdx = -1;  // fluid: number of the cell's pairs that come before its own density term
         const int sl = cs + q;
         if (active) {
             self = row_nb(S, rc, sl);
@@ -530,6 +542,9 @@ __device__ __forceinline__ void row_compute(const Ctx& c, const DevWorld& w, con
                 direct = true;
             } else {
                 const bool hasBonds = flags_num_conn(self.flags) > 0;
+                // fluid: the cell's own density term needs no pair evaluation; only its place in
+                // the visiting order is kept (a barrier cell meets itself as a barrier instead)
+                const bool elideSelf = kFluid && !flags_barrier(self.flags);
                 int cidx = 0;
 #pragma unroll
                 for (int r = 0; r < 3; ++r) {
@@ -538,19 +553,21 @@ __device__ __forceinline__ void row_compute(const Ctx& c, const DevWorld& w, con
                     for (int u = 0; u < n; u += 2) {
                         // two candidates per iteration: their loads overlap
                         const f2 p0 = to_f2(S.pos[rr][a0 + u]);
-                        const uint32_t f0 = S.flags[rr][a0 + u];
                         const bool two = u + 1 < n;
                         const f2 p1 = two ? to_f2(S.pos[rr][a0 + u + 1]) : p0;
-                        const uint32_t f1 = two ? S.flags[rr][a0 + u + 1] : f0;
-                        if (staged_neighbor<kFluid>(self, p0, f0, r == 1 && a0 + u == sl, Wf, Hf, P.reachSq)) {
-                            mask |= 1ull << (cidx + u);
-                            if (hasBonds && is_bonded(bt, S.aidx[rr][a0 + u] - w.cellOff)) bmask |= 1ull << (cidx + u);
-                        }
-                        if (two &&
-                            staged_neighbor<kFluid>(self, p1, f1, r == 1 && a0 + u + 1 == sl, Wf, Hf, P.reachSq)) {
-                            mask |= 1ull << (cidx + u + 1);
-                            if (hasBonds && is_bonded(bt, S.aidx[rr][a0 + u + 1] - w.cellOff))
-                                bmask |= 1ull << (cidx + u + 1);
+#pragma unroll
+                        for (int h = 0; h < 2; ++h) {
+                            if (h == 1 && !two) break;
+                            const int cu = a0 + u + h;
+                            const bool isSelf = r == 1 && cu == sl;
+                            if (elideSelf && isSelf) {
+                                selfIdx = __builtin_popcountll(mask);
+                                continue;
+                            }
+                            if (staged_neighbor<kFluid>(self, h == 0 ? p0 : p1, isSelf, Wf, Hf, P.reachSq)) {
+                                mask |= 1ull << (cidx + u + h);
+                                if (hasBonds && is_bonded(bt, S.aidx[rr][cu] - w.cellOff)) bmask |= 1ull << (cidx + u + h);
+                            }
                         }
                     }
                     cidx += n;
@@ -610,6 +627,7 @@ __device__ __forceinline__ void row_compute(const Ctx& c, const DevWorld& w, con
                     const int lo = max(pbase, off), hi = min(pbase + count, min(off + kRowThreads, wend));
                     for (int p2 = lo; p2 < hi; ++p2) {
                         const int k = p2 - off;
+                        if (p2 - pbase == selfIdx) a.rho = a.rho + w0;
                         float t[kPairTerms];
 #pragma unroll
                         for (int v = 0; v < kPairTerms; ++v) t[v] = S.term[v][k];
@@ -619,9 +637,10 @@ __device__ __forceinline__ void row_compute(const Ctx& c, const DevWorld& w, con
                 __syncthreads();
             }
         }
+        if (active && selfIdx == count) a.rho = a.rho + w0;  // the own term after all pairs
         if (active) {
             if (direct) {
-                cell_direct<kFluid>(c, w, P, wi, self, [&](auto&& visit) {
+                cell_direct<kFluid>(c, w, P, wi, self, R, [&](auto&& visit) {
 #pragma unroll
                     for (int r = 0; r < 3; ++r) {
                         const int rr = r == 0 ? rm_ : (r == 1 ? rc : rp);
@@ -629,7 +648,7 @@ __device__ __forceinline__ void row_compute(const Ctx& c, const DevWorld& w, con
                     }
                 });
             } else {
-                cell_finish<kFluid>(c, w, P, wi, self, bt, a);
+                cell_finish<kFluid>(c, w, P, wi, self, bt, a, R);
             }
         }
     }
@@ -641,6 +660,7 @@ template <bool kFluid>
 __device__ void rows_direct(const Ctx& c, const DevWorld& w, const DevParams& P, int wi, const StripGeom& g, int y0,
                             int IY) {
     const uint32_t base = g.base;
+    const Region R{nullptr, 0, 0, 0, 0};  // every cell in the shared histogram
     auto binRange = [&](int gb, int& s0, int& n) {
         s0 = g.cellOff + int(c.start[gb] - base);
         n = int(c.start[gb + 1] - c.start[gb]);
@@ -663,8 +683,8 @@ __device__ void rows_direct(const Ctx& c, const DevWorld& w, const DevParams& P,
             int bs, bn;
             binRange(strip_bin(g, y, hx), bs, bn);
             const int i = c.sorted[m];
-            const Nb self = global_nb(c, i, bs + rank_in_bin(c.sorted, bs, bs + bn, i));
-            cell_direct<kFluid>(c, w, P, wi, self, [&](auto&& visit) {
+            const Nb self = global_nb<kFluid>(c, i, bs + rank_in_bin(c.sorted, bs, bs + bn, i));
+            cell_direct<kFluid>(c, w, P, wi, self, R, [&](auto&& visit) {
                 for (int dy = -1; dy <= 1; ++dy)
                     for (int dx = -1; dx <= 1; ++dx) {
                         int s0, n;
@@ -677,7 +697,7 @@ __device__ void rows_direct(const Ctx& c, const DevWorld& w, const DevParams& P,
                                 if (a > last && a < best) best = a;
                             }
                             last = best;
-                            visit(global_nb(c, best, s0 + t));
+                            visit(global_nb<kFluid>(c, best, s0 + t));
                         }
                     }
             });
@@ -720,15 +740,20 @@ __global__ void __launch_bounds__(kRowThreads) k_neighbor_rows(Ctx c) {
     }
     if (__any(tooDense)) {
         rows_direct<kFluid>(c, w, P, wi, g, y0, IY);
+        // the owner-counted part of the region's bins is empty
+        for (int r = 0; r < IY; ++r)
+            for (int x = lane; x < g.IX; x += kRowThreads) c.privNext[g.binOff + (y0 + r) * g.nbx + g.x0 + x] = 0u;
         return;
     }
+    const Region R{S.hist, g.x0, y0, g.IX, IY};
+    for (int b = lane; b < g.IX * IY; b += kRowThreads) S.hist[b] = 0u;
 
     RowFlight F, G;
     // rows y0-1, y0, y0+1 into ring slots 0, 1, 2
     for (int r = 0; r < 3; ++r) {
         row_issue_counts(c, g, y0 - 1 + r, F);
         row_issue_entries(c, g, S, F);
-        row_commit(g, S, F, r);
+        row_commit<kFluid>(g, S, F, r);
     }
     if (IY >= 2) row_issue_counts(c, g, y0 + 2, F);
     for (int y = y0; y < y0 + IY; ++y) {
@@ -739,9 +764,9 @@ __global__ void __launch_bounds__(kRowThreads) k_neighbor_rows(Ctx c) {
             row_issue_entries(c, g, S, F);
             if (y + 3 <= y0 + IY) row_issue_counts(c, g, y + 3, G);
         }
-        row_compute<kFluid>(c, w, P, wi, g, S, rm_, rc, rp);
+        row_compute<kFluid>(c, w, P, wi, g, S, R, rm_, rc, rp);
         if (next) {
-            row_commit(g, S, F, rm_);  // row y+2 takes the slot of row y-1
+            row_commit<kFluid>(g, S, F, rm_);  // row y+2 takes the slot of row y-1
 #pragma unroll
             for (int q = 0; q < 2; ++q) {
                 F.s0[q] = G.s0[q];
@@ -749,6 +774,11 @@ __global__ void __launch_bounds__(kRowThreads) k_neighbor_rows(Ctx c) {
             }
         }
     }
+    // the region's owner-counted bin sizes for the next step (every bin written, zeros included)
+    __syncthreads();
+    for (int r = 0; r < IY; ++r)
+        for (int x = lane; x < g.IX; x += kRowThreads)
+            c.privNext[g.binOff + (y0 + r) * g.nbx + g.x0 + x] = S.hist[r * g.IX + x];
 }
 
 void launch_neighbor(const Ctx& c, const LaunchShape& sh, hipStream_t s, bool fluid, bool collision) {

@@ -41,7 +41,9 @@ constexpr int kScanThreads = 256;
 constexpr int kScanItems = 8;
 constexpr int kScanTile = kScanThreads * kScanItems;
 
-__device__ __forceinline__ void load8(const uint32_t* count, size_t base, int n, uint32_t v[8]) {
+// bin sizes base..base+7: shared count plus, for cell bins, the owner-counted part
+__device__ __forceinline__ void load8(const uint32_t* count, const uint32_t* priv, int nCell, size_t base, int n,
+                                      uint32_t v[8]) {
     if (base + 8 <= size_t(n)) {
         const uint4 a = *reinterpret_cast<const uint4*>(count + base);
         const uint4 b = *reinterpret_cast<const uint4*>(count + base + 4);
@@ -50,12 +52,22 @@ __device__ __forceinline__ void load8(const uint32_t* count, size_t base, int n,
 #pragma unroll
         for (int k = 0; k < 8; ++k) v[k] = base + k < size_t(n) ? count[base + k] : 0u;
     }
+    if (base + 8 <= size_t(nCell)) {
+        const uint4 a = *reinterpret_cast<const uint4*>(priv + base);
+        const uint4 b = *reinterpret_cast<const uint4*>(priv + base + 4);
+        v[0] += a.x, v[1] += a.y, v[2] += a.z, v[3] += a.w, v[4] += b.x, v[5] += b.y, v[6] += b.z, v[7] += b.w;
+    } else {
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+            if (base + k < size_t(nCell)) v[k] += priv[base + k];
+    }
 }
 
-__global__ void __launch_bounds__(kScanThreads) k_scan_reduce(const uint32_t* count, uint32_t* blockSums, int n) {
+__global__ void __launch_bounds__(kScanThreads) k_scan_reduce(const uint32_t* count, const uint32_t* priv, int nCell,
+                                                              uint32_t* blockSums, int n) {
     const size_t base = size_t(blockIdx.x) * kScanTile + size_t(threadIdx.x) * kScanItems;
     uint32_t v[8];
-    load8(count, base, n, v);
+    load8(count, priv, nCell, base, n, v);
     uint32_t s = v[0] + v[1] + v[2] + v[3] + v[4] + v[5] + v[6] + v[7];
     uint32_t total;
     block_exclusive_scan(s, total);
@@ -90,11 +102,11 @@ __global__ void __launch_bounds__(1024) k_scan_top(uint32_t* blockSums, int numT
 
 // writes start[]; `count` stays: the scatter counts it back down to zero, which leaves it cleared
 // for the step after next (the two histograms alternate)
-__global__ void __launch_bounds__(kScanThreads) k_scan_down(const uint32_t* count, const uint32_t* blockSums,
-                                                            uint32_t* start, int n) {
+__global__ void __launch_bounds__(kScanThreads) k_scan_down(const uint32_t* count, const uint32_t* priv, int nCell,
+                                                            const uint32_t* blockSums, uint32_t* start, int n) {
     const size_t base = size_t(blockIdx.x) * kScanTile + size_t(threadIdx.x) * kScanItems;
     uint32_t v[8];
-    load8(count, base, n, v);
+    load8(count, priv, nCell, base, n, v);
     uint32_t s = v[0] + v[1] + v[2] + v[3] + v[4] + v[5] + v[6] + v[7];
     uint32_t total;
     uint32_t off = block_exclusive_scan(s, total) + blockSums[blockIdx.x];
@@ -126,8 +138,15 @@ __global__ void k_cell_scatter(Ctx c) {
     const float d = c.A.dens[i];
     const uint32_t f = c.A.flags[i];
     const int sl = c.A.slot[i];
-    int gb = c.binOf[i];
-    int dest = w.cellOff + int(c.start[gb] - c.start[w.binOff]) + int(atomicSub(&c.count[gb], 1u)) - 1;
+    const int gb = c.binOf[i];
+    const int arr = c.arrival[i];
+    int rank = arr;
+    if (arr & kArrivalShared) {
+        // shared arrivals come after the owner-counted ones; counting down clears the histogram
+        rank = int(c.priv[gb]) + (arr & ~kArrivalShared);
+        atomicSub(&c.count[gb], 1u);
+    }
+    const int dest = w.cellOff + int(c.start[gb] - c.start[w.binOff]) + rank;
     c.sorted[dest] = i;
     c.mpos[dest] = p;
     c.mvel[dest] = v;
@@ -226,15 +245,19 @@ void launch_fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t s) {
 }
 
 void launch_seed_bins(const Ctx& c, const LaunchShape& sh, hipStream_t s) {
+    // every cell is counted in the shared histogram; the owner-counted part starts empty
+    launch_fill_u32(c.priv, 0u, size_t(c.numCellBins), s);
     if (sh.cellBlocks) hipLaunchKernelGGL(k_cell_bin, dim3(sh.cellBlocks), dim3(sh.cellThreads), 0, s, c);
     if (sh.partBlocks) hipLaunchKernelGGL(k_part_bin, dim3(sh.partGrid), dim3(sh.partThreads), 0, s, c);
 }
 
 void launch_scan(const Ctx& c, hipStream_t s, StepCounters* sc) {
     const int tiles = (c.numBins + kScanTile - 1) / kScanTile;
-    hipLaunchKernelGGL(k_scan_reduce, dim3(tiles), dim3(kScanThreads), 0, s, c.count, c.blockSums, c.numBins);
+    hipLaunchKernelGGL(k_scan_reduce, dim3(tiles), dim3(kScanThreads), 0, s, c.count, c.priv, c.numCellBins,
+                       c.blockSums, c.numBins);
     hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, s, c.blockSums, tiles, c.start, c.numBins, sc);
-    hipLaunchKernelGGL(k_scan_down, dim3(tiles), dim3(kScanThreads), 0, s, c.count, c.blockSums, c.start, c.numBins);
+    hipLaunchKernelGGL(k_scan_down, dim3(tiles), dim3(kScanThreads), 0, s, c.count, c.priv, c.numCellBins,
+                       c.blockSums, c.start, c.numBins);
 }
 
 void launch_cell_scatter(const Ctx& c, const LaunchShape& sh, hipStream_t s) {

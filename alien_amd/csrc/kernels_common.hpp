@@ -50,14 +50,22 @@ __device__ __forceinline__ int rank_in_bin(const int* sorted, int s, int e, int 
     return r;
 }
 
-// counting-sort bookkeeping for the next step: bin of the end-of-step position, counted into
-// `hist` with an atomic whose result is not needed (the scatter ranks entries later)
-__device__ __forceinline__ void bin_for_next_step(const Ctx& c, const DevWorld& w, int k, f2 p, uint32_t* hist) {
+// counting-sort bookkeeping: the arrival of a cell counted in the shared (atomic) histogram
+constexpr int kArrivalShared = 0x40000000;
+
+// cell bin of a position (global bin index)
+__device__ __forceinline__ int cell_bin_of(const DevWorld& w, f2 p) {
     int sx, sy;
     slot_of(p, w.W, w.H, sx, sy);
-    int gb = w.binOff + cell_bin_local(w, sx, sy);
+    return w.binOff + cell_bin_local(w, sx, sy);
+}
+
+// counting-sort bookkeeping for the next step: bin of the end-of-step position, counted into the
+// shared histogram `hist`; the returned count is the cell's place among the shared arrivals
+__device__ __forceinline__ void bin_for_next_step(const Ctx& c, const DevWorld& w, int k, f2 p, uint32_t* hist) {
+    const int gb = cell_bin_of(w, p);
     c.binOf[k] = gb;
-    atomicAdd(&hist[gb], 1u);
+    c.arrival[k] = kArrivalShared | int(atomicAdd(&hist[gb], 1u));
 }
 __device__ __forceinline__ void pbin_for_next_step(const Ctx& c, const DevWorld& w, int k, f2 p, uint32_t* hist) {
     int sx, sy;

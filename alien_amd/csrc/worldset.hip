@@ -60,6 +60,17 @@ static float sqrt_threshold(float R) {
     return s;
 }
 
+// largest float s with sqrtf(s) < R (-1 if there is none)
+static float sqrt_below(float R) {
+    if (!(R > 0.0f)) return -1.0f;
+    if (std::isinf(R)) return std::numeric_limits<float>::max();
+    float s = R * R;
+    while (s > 0.0f && !(std::sqrt(s) < R)) s = std::nextafter(s, 0.0f);
+    while (std::sqrt(std::nextafter(s, std::numeric_limits<float>::infinity())) < R)
+        s = std::nextafter(s, std::numeric_limits<float>::infinity());
+    return std::sqrt(s) < R ? s : -1.0f;
+}
+
 DevParams to_dev_params(const alien_params& p) {
     DevParams d{};
     d.dt = p.timestepSize;
@@ -79,6 +90,12 @@ DevParams to_dev_params(const alien_params& p) {
         d.absorption[c] = p.radiationAbsorption[c];
     }
     d.reachSq = sqrt_threshold(d.motion == 0 ? d.smoothing * 2.0f : d.collisionDistance);
+    d.maxVelSq = sqrt_threshold(d.maxVelocity);
+    d.fastSq = sqrt_threshold(0.5f);
+    d.fusionBelowSq = sqrt_below(d.fusionVelocity);
+    d.invH = 1.0f / d.smoothing;
+    d.invH2 = 1.0f / (d.smoothing * d.smoothing);
+    d.invH3 = 1.0f / (d.smoothing * d.smoothing * d.smoothing);
     return d;
 }
 
@@ -360,11 +377,11 @@ void WorldSet::synchronize() { check_hip(hipStreamSynchronize(stream_), "hipStre
 
 uint64_t WorldSet::estimateBytes(uint64_t cells, uint64_t particles, int W, int H, int binSize) {
     const uint64_t perCell = 2 * (8 + 8 + 4 + 4 + 4 + kMaxBonds * 12) + 8 + 4 * 4 + 8 + 3 * 8 + 2 * kMaxBonds * 8 + 4 +
-                             2 + 8 + 4 * 16 + 4 + (8 + 8 + 4 + 4 + 4);
+                             2 + 8 + 4 * 16 + 4 + (8 + 8 + 4 + 4 + 4) + 4;
     const uint64_t perParticle = 2 * (8 + 8 + 4 + 4 + 4 + 1) + 6 * 4;
     const uint64_t bins = uint64_t(std::max(1, W / binSize)) * uint64_t(std::max(1, H / binSize)) +
                           uint64_t(std::max(1, W / kParticleBin)) * uint64_t(std::max(1, H / kParticleBin));
-    return cells * perCell + particles * perParticle + bins * 8 + (1u << 20);
+    return cells * perCell + particles * perParticle + bins * 16 + (1u << 20);
 }
 
 void WorldSet::clearAll() {
@@ -452,6 +469,7 @@ void WorldSet::build(const std::vector<Source>& sources) {
         stagedHint.push_back(hint);
         bondWork.push_back(work ? 1 : 0);
     }
+    const int cellBins = bins;  // cell bins of all worlds come first, then the particle bins
     for (auto& w : worlds) {
         w.pbinOff = bins;
         bins += w.pnbx * w.pnby;
@@ -501,6 +519,11 @@ void WorldSet::build(const std::vector<Source>& sources) {
     pbinOf_ = static_cast<int*>(A(sizeof(int) * NP));
     count_[0] = static_cast<uint32_t*>(A(sizeof(uint32_t) * NB));
     count_[1] = static_cast<uint32_t*>(A(sizeof(uint32_t) * NB));
+    numCellBins_ = cellBins;
+    const size_t NCB = size_t(std::max(cellBins, 1));
+    priv_[0] = static_cast<uint32_t*>(A(sizeof(uint32_t) * NCB));
+    priv_[1] = static_cast<uint32_t*>(A(sizeof(uint32_t) * NCB));
+    arrival_ = static_cast<int*>(A(sizeof(int) * NC));
     start_ = static_cast<uint32_t*>(A(sizeof(uint32_t) * (size_t(NB) + 1)));
     blockSums_ = static_cast<uint32_t*>(A(sizeof(uint32_t) * (size_t(NB) / 2048 + 2)));
     sorted_ = static_cast<int*>(A(sizeof(int) * NC));
@@ -622,6 +645,8 @@ void WorldSet::build(const std::vector<Source>& sources) {
     check_hip(hipMemcpy(dSc_, &sc, sizeof(sc), hipMemcpyHostToDevice), "H2D");
     launch_fill_u32(count_[0], 0u, size_t(NB), stream_);
     launch_fill_u32(count_[1], 0u, size_t(NB), stream_);
+    launch_fill_u32(priv_[0], 0u, NCB, stream_);
+    launch_fill_u32(priv_[1], 0u, NCB, stream_);
     launch_fill_u32(reinterpret_cast<uint32_t*>(fclaim_), uint32_t(kClaimFree), size_t(NP), stream_);
     launch_fill_u32(reinterpret_cast<uint32_t*>(cclaim_), uint32_t(kClaimFree), size_t(NC), stream_);
     launch_fill_u32(reinterpret_cast<uint32_t*>(claim_), 0xffffffffu, size_t(NC) * 2, stream_);
@@ -676,7 +701,7 @@ int choose_strip(const std::vector<float2>& pos, int W, int H, int binSize) {
 void strip_shape(DevWorld& w, int strip) {
     w.tileX = std::max(1, std::min(strip, w.nbx));
     w.ntx = (w.nbx + w.tileX - 1) / w.tileX;
-    w.tileY = std::max(2, std::min(16, int((int64_t(w.ntx) * w.nby) / 4096)));
+    w.tileY = std::max(2, std::min(kChunkMax, int((int64_t(w.ntx) * w.nby) / 4096)));
     w.tileY = std::min(w.tileY, w.nby);
     w.nty = (w.nby + w.tileY - 1) / w.tileY;
 }
@@ -776,6 +801,10 @@ Ctx WorldSet::makeCtx(int parity) const {
     c.cellCap = cellCap_;
     c.partCap = partCap_;
     c.numBins = numBins_;
+    c.numCellBins = numCellBins_;
+    c.priv = priv_[parity];
+    c.privNext = priv_[parity ^ 1];
+    c.arrival = arrival_;
     c.worlds = dWorlds_;
     c.params = dParams_;
     c.ctr = dCtr_;

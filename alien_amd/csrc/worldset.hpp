@@ -165,6 +165,9 @@ private:
     float* stiff_ = nullptr;
     int *binOf_ = nullptr, *pbinOf_ = nullptr;
     uint32_t* count_[2] = {nullptr, nullptr};  // bin histograms, alternating with the state buffers
+    uint32_t* priv_[2] = {nullptr, nullptr};   // owner-counted part of the cell-bin histograms
+    int* arrival_ = nullptr;
+    int numCellBins_ = 0;
     uint32_t *start_ = nullptr, *blockSums_ = nullptr;
     int *sorted_ = nullptr, *newidx_ = nullptr, *psorted_ = nullptr;
     float2 *mpos_ = nullptr, *mvel_ = nullptr;

@@ -30,12 +30,18 @@
 //   R10 dead particles are removed with a stable compaction.
 //   R12 SPH sums gathered per cell in visiting order.
 //   R13 collision neighbours use the uncorrected (seam-blind) distance like Map.cuh:189.
-// Two rules change only the rounding of the FP32 build (in exact arithmetic they are the
-// reference's formulas), so that FP32 stays within 1e-5 of FP64 also on large worlds:
+// Three rules change only the rounding of the FP32 build (in exact arithmetic they are the
+// reference's formulas); R14 and R15 keep FP32 within 1e-5 of FP64 also on large worlds, R16 saves
+// divisions:
 //   R14 torus differences subtract the period from the operand beyond it first (exact), instead
 //       of rounding a - b and then taking remainderf (Map.cuh:35-39).
 //   R15 the second connection-force pass and checkConnections evaluate bond vectors at the moved
 //       positions as (start difference) + (shift difference); positions are moved after it.
+//   R16 unit vectors, the SPH normalisations (1/h, 1/h^2, 1/h^3), the density terms
+//       (density/density^2 = 1/density) and the collision and viscosity denominators are applied
+//       as products with the correctly rounded reciprocal (one division instead of several).
+// Cells are never detached here (detached cells come from cell functions), so the detached rule
+// of the neighbour predicate never applies and is not restated.
 // Parity status: pinned by tests/test_oracle.py against closed forms, invariants, brute force
 // and the reference's own test scenarios (see the docstring there for which test pins what).
 #pragma once
@@ -91,9 +97,12 @@ template <class R> inline R dot(V2<R> a, V2<R> b) { return a.x * b.x + a.y * b.y
 template <class R> inline R lengthSquared(V2<R> v) { return v.x * v.x + v.y * v.y; }          // Math.cuh:288
 template <class R> inline R length(V2<R> v) { return std::sqrt(v.x * v.x + v.y * v.y); }     // Math.cuh:273
 // Math.cuh:246-261: unit vector, (1,0) below NEAR_ZERO
-template <class R> inline V2<R> normalized(V2<R> v) {
+template <class R> inline V2<R> normalized(V2<R> v) {  // R16: times the reciprocal length
     R len = std::sqrt(v.x * v.x + v.y * v.y);
-    if (len > K<R>::NEAR_ZERO) return {v.x / len, v.y / len};
+    if (len > K<R>::NEAR_ZERO) {
+        const R inv = R(1) / len;
+        return {v.x * inv, v.y * inv};
+    }
     return {R(1), R(0)};
 }
 template <class R> inline void rotateQuarterClockwise(V2<R>& v) {  // Math.cuh:157-162
@@ -274,8 +283,7 @@ template <class R> struct Cell {  // Object.cuh:201-263, physics subset
     uint64_t id;
     V2<R> pos, vel;
     R energy, stiffness, density;
-    uint8_t color, maxConn, numConn, barrier, detached, livingState, cellFunction, ancestorMutationId,
-        selected;
+    uint8_t color, maxConn, numConn, barrier, livingState, cellFunction, ancestorMutationId, selected;
     uint32_t age, creatureId, mutationId, activationTime;
     float genomeComplexity;
     Bond<R> bonds[ALIEN_MAX_CELL_BONDS];
@@ -380,7 +388,6 @@ public:
             d.maxConn = s.maxConnections;
             d.numConn = s.numConnections;
             d.barrier = s.barrier ? 1 : 0;
-            d.detached = 0;
             d.livingState = s.livingState;
             d.cellFunction = s.cellFunction;
             d.ancestorMutationId = s.ancestorMutationId;
@@ -552,7 +559,6 @@ public:
         if (i == j) return false;
         const auto& a = cells[i];
         const auto& b = cells[j];
-        if (a.detached + b.detached == 1) return false;
         if (rawParams.motionType == ALIEN_MOTION_COLLISION) {
             return length(b.pos - a.pos) <= p.cellMaxCollisionDistance;
         }
@@ -589,7 +595,7 @@ public:
                 R bf = c.barrier ? R(2) : R(1);
                 if (length(c.vel) > R(0.5f) && approaching) {
                     R dsq = distance * distance + R(0.25f);
-                    fij = posDelta * dot(velDelta, posDelta) / (R(-2) * dsq) * bf;
+                    fij = posDelta * dot(velDelta, posDelta) * (R(1) / (R(-2) * dsq)) * bf;  // R16
                 } else {
                     fij = normalized(posDelta) * (r - length(posDelta)) * p.cellRepulsionStrength * bf;
                 }
@@ -602,7 +608,7 @@ public:
                 R bf = o.barrier ? R(2) : R(1);
                 if (length(o.vel) > R(0.5f) && approaching) {
                     R dsq = distance * distance + R(0.25f);
-                    fji = pd * dot(vd, pd) / (R(-2) * dsq) * bf;
+                    fji = pd * dot(vd, pd) * (R(1) / (R(-2) * dsq)) * bf;  // R16
                 } else {
                     fji = normalized(pd) * (r - length(pd)) * p.cellRepulsionStrength * bf;
                 }
@@ -623,6 +629,9 @@ public:
         const auto& c = cells[i];
         const R h = p.smoothingLength;
         const R cmd = p.cellMinDistance;
+        // R16: divisions by h, h^2, h^3 and by the densities as products with reciprocals
+        const R invH = R(1) / h, invH2 = R(1) / (h * h), invH3 = R(1) / (h * h * h);
+        const R invDi = R(1) / c.density;
         V2<R> Fp{R(0), R(0)}, Fv{R(0), R(0)}, D{R(0), R(0)};
         R rho = R(0);
         int barrierList[10];
@@ -631,22 +640,23 @@ public:
             const auto& o = cells[j];
             V2<R> posDelta = torus.delta(c.pos, o.pos);
             R distance = length(posDelta);
-            if (!(distance <= h * R(2)) || c.detached + o.detached == 1) return;
+            if (!(distance <= h * R(2))) return;
             if (o.barrier) {
                 if (numBarrier < 10) barrierList[numBarrier] = j;
                 ++numBarrier;
                 return;
             }
-            rho = rho + calcKernel(distance / h) / (h * h);
+            rho = rho + calcKernel(distance * invH) * invH2;
             if (j == i) return;
             if (!c.barrier && distance < cmd) D = D + posDelta * cmd / R(5);
             if (bonded(i, j)) return;
             V2<R> velDelta = c.vel - o.vel;
-            R factor = c.density / (c.density * c.density) + o.density / (o.density * o.density);
+            const R invDj = R(1) / o.density;
+            R factor = invDi + invDj;
             if (std::abs(distance) > K<R>::NEAR_ZERO) {
-                R kd = calcKernel_d(distance / h) / (h * h * h);
-                Fp = Fp + posDelta / (-distance) * factor * kd;
-                Fv = Fv + velDelta / o.density * distance * kd / (distance * distance + R(0.25f));
+                R kd = calcKernel_d(distance * invH) * invH3;
+                Fp = Fp + posDelta * (R(1) / (-distance)) * factor * kd;
+                Fv = Fv + velDelta * invDj * distance * kd * (R(1) / (distance * distance + R(0.25f)));
             }
             if (i < j && length(velDelta) >= p.cellFusionVelocity && c.numConn < c.maxConn && o.numConn < o.maxConn &&
                 c.energy <= p.cellMaxBindingEnergy && o.energy <= p.cellMaxBindingEnergy && !c.barrier && !o.barrier)
