@@ -17,10 +17,12 @@ constexpr int kColors = 7;
 constexpr int kParticleBin = 8;  // edge of the energy-particle bins, in world units
 constexpr int kNone = -1;
 // neighbour pass: one wave per strip of bin columns; entries of one staged bin row (strip plus two
-// halo columns) in LDS; widest strip
+// halo columns) in LDS; widest strip.  The LDS footprint sets how many waves a CU keeps in flight,
+// and the pass is latency bound, so the row capacity is sized for strips of 32 columns at fluid
+// density (about 62 entries in the densest row of a 4M-cell 4096^2 world).
 constexpr int kRowThreads = 64;
-constexpr int kRowCap = 128;
-constexpr int kStripMax = 64;
+constexpr int kRowCap = 96;
+constexpr int kStripMax = 32;
 constexpr int kChunkMax = 16;  // bin rows per neighbour-pass chunk
 constexpr int kClaimFree = 0x7fffffff;
 
@@ -142,7 +144,6 @@ struct Ctx {
     float2* mvel;
     float* mdens;
     uint32_t* mflags;
-    int* mslot;
     int* newidx;   // [cellCap] A index -> new index
     int* psorted;  // [partCap]
     int2* staged;  // [cellCap] (new index, world)

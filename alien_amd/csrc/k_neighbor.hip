@@ -29,7 +29,6 @@ struct Nb {
     uint32_t flags;
     int aidx;  // index in the start-of-step arrays (A)
     int lidx;  // index in the end-of-step arrays (B), the logical index of this step
-    int slot;  // cold slot (stable identity)
 };
 
 __device__ __forceinline__ void push_candidate(const Ctx& c, int a, int b, int w) {
@@ -206,7 +205,9 @@ __device__ __forceinline__ bool is_bonded(const int bt[kMaxBonds], int jLocal) {
 }
 
 // The bins a neighbour-pass wave owns (x0..x0+IX-1, y0..y0+IY-1) and its LDS histogram of the
-// next step's cells in them; hist == nullptr counts every cell in the shared histogram.
+// next step's cells in them, two 16-bit counters per word (bin b in half b & 1 of word b >> 1; a
+// wave counts at most its own cells, far below 2^16); hist == nullptr counts every cell in the
+// shared histogram.
 struct Region {
     uint32_t* hist;
     int x0, y0, IX, IY;
@@ -221,10 +222,16 @@ __device__ __forceinline__ void bin_next(const Ctx& c, const DevWorld& w, int k,
     const int gb = w.binOff + by * w.nbx + bx;
     c.binOf[k] = gb;
     const int lx = bx - R.x0, ly = by - R.y0;
-    if (R.hist && lx >= 0 && lx < R.IX && ly >= 0 && ly < R.IY)
-        c.arrival[k] = int(atomicAdd(&R.hist[ly * R.IX + lx], 1u));
-    else
+    if (R.hist && lx >= 0 && lx < R.IX && ly >= 0 && ly < R.IY) {
+        const int b = ly * R.IX + lx;
+        const uint32_t sh = uint32_t(b & 1) << 4;
+        c.arrival[k] = int((atomicAdd(&R.hist[b >> 1], 1u << sh) >> sh) & 0xFFFFu);
+    } else
         c.arrival[k] = kArrivalShared | int(atomicAdd(&c.countNext[gb], 1u));
+}
+
+__device__ __forceinline__ uint32_t hist_count(const uint32_t* hist, int b) {
+    return (hist[b >> 1] >> (uint32_t(b & 1) << 4)) & 0xFFFFu;
 }
 
 // Barrier reflection, applyForces, and the end-of-step writes of one cell.
@@ -278,7 +285,7 @@ __device__ __forceinline__ void cell_finish(const Ctx& c, const DevWorld& w, con
     // the cell in the end-of-step order; bond targets stay in start-of-step indices until the
     // staged cells are renumbered (k_remap)
     c.B.flags[k] = fi;
-    c.B.slot[k] = s.slot;
+    c.B.slot[k] = c.A.slot[i];
     c.B.dens[k] = kFluid ? a.rho : s.dens;
     for (int b = 0; b < nci; ++b) {
         c.B.btgt[b * cap + k] = bt[b];
@@ -343,33 +350,36 @@ constexpr int kPairCap = 256;  // pair records per evaluation window
 
 struct RowShared {
     int rs[3][kRowBins + 1];  // per staged row: first slot of every halo-column bin, then the total
+    int rm[3][kRowBins];      // per staged row: map position of every halo-column bin's first entry
     float2 pos[3][kRowCap];   // entries by slot
     float2 vel[3][kRowCap];
     float dens[3][kRowCap];   // fluid: 1 / density; collision: density
     uint32_t flags[3][kRowCap];
     int aidx[3][kRowCap];
-    int slot[3][kRowCap];
-    int lidx[3][kRowCap];      // logical index of this step (position in the cell map)
     uint8_t sbin[3][kRowCap];  // slot -> halo column
     // the row in flight
     int nrs[kRowBins + 1];
     int nrm[kRowBins];
     uint8_t nebin[kRowCap];  // arrival index -> halo column
-    int naidx[kRowCap];      // arrival index -> previous index
-    // pair records of the current window: the cell's slot in the middle row, and the neighbour as
-    // bonded << 15 | stencil row << 8 | slot
-    uint8_t pself[kPairCap];
-    uint16_t pnb[kPairCap];
-    float term[kPairTerms][kRowThreads];
-    uint32_t pflag[kRowThreads];
-    uint32_t hist[kStripMax * kChunkMax];  // next-step cells of the wave's own bins
+    union {
+        int naidx[kRowCap];  // while a row is committed: arrival index -> previous index
+        struct {
+            // while a row is computed, the pair records of the current window: the cell's slot in
+            // the middle row, and the neighbour as bonded << 15 | stencil row << 8 | slot
+            uint8_t pself[kPairCap];
+            uint16_t pnb[kPairCap];
+            float term[kPairTerms][kRowThreads];
+            uint32_t pflag[kRowThreads];
+        } pr;
+    } u;
+    uint32_t hist[kStripMax * kChunkMax / 2];  // next-step cells of the wave's own bins (16-bit pairs)
 };
 
 // registers of a row in flight: bin counters of the halo columns lane and lane + 64, then the
 // entries lane and lane + 64 in arrival order
 struct RowFlight {
     uint32_t s0[2], s1[2];
-    int aidx[2], slot[2];
+    int aidx[2];
     float2 pos[2], vel[2];
     float dens[2];
     uint32_t flags[2];
@@ -383,8 +393,8 @@ __device__ __forceinline__ Nb row_nb(const RowShared& S, int r, int sl) {
     o.dens = S.dens[r][sl];
     o.flags = S.flags[r][sl];
     o.aidx = S.aidx[r][sl];
-    o.slot = S.slot[r][sl];
-    o.lidx = S.lidx[r][sl];
+    const int hx = S.sbin[r][sl];
+    o.lidx = S.rm[r][hx] + (sl - S.rs[r][hx]);
     return o;
 }
 
@@ -397,7 +407,6 @@ __device__ __forceinline__ Nb global_nb(const Ctx& c, int a, int l) {
     o.flags = c.A.flags[a];
     o.aidx = a;
     o.lidx = l;
-    o.slot = c.A.slot[a];
     return o;
 }
 
@@ -455,7 +464,6 @@ __device__ __forceinline__ void row_issue_entries(const Ctx& c, const StripGeom&
             const int hx = S.nebin[e];
             const int m = S.nrm[hx] + (e - S.nrs[hx]);
             F.aidx[q] = c.sorted[m];
-            F.slot[q] = c.mslot[m];
             F.pos[q] = c.mpos[m];
             F.vel[q] = c.mvel[m];
             F.dens[q] = c.mdens[m];
@@ -468,11 +476,14 @@ __device__ __forceinline__ void row_issue_entries(const Ctx& c, const StripGeom&
 template <bool kFluid>
 __device__ __forceinline__ void row_commit(const StripGeom& g, RowShared& S, const RowFlight& F, int r) {
     const int lane = threadIdx.x;
-    for (int i = lane; i <= g.HX; i += kRowThreads) S.rs[r][i] = S.nrs[i];
+    for (int i = lane; i <= g.HX; i += kRowThreads) {
+        S.rs[r][i] = S.nrs[i];
+        if (i < g.HX) S.rm[r][i] = S.nrm[i];
+    }
 #pragma unroll
     for (int q = 0; q < 2; ++q) {
         const int e = lane + q * kRowThreads;
-        if (e < F.total) S.naidx[e] = F.aidx[q];
+        if (e < F.total) S.u.naidx[e] = F.aidx[q];
     }
     __syncthreads();
 #pragma unroll
@@ -483,16 +494,14 @@ __device__ __forceinline__ void row_commit(const StripGeom& g, RowShared& S, con
             const int s0 = S.nrs[hx], s1 = S.nrs[hx + 1];
             const int a = F.aidx[q];
             int rank = 0;
-            for (int u = s0; u < s1; ++u) rank += S.naidx[u] < a ? 1 : 0;
+            for (int u = s0; u < s1; ++u) rank += S.u.naidx[u] < a ? 1 : 0;
             const int sl = s0 + rank;
             S.pos[r][sl] = F.pos[q];
             S.vel[r][sl] = F.vel[q];
             S.dens[r][sl] = kFluid ? 1.0f / F.dens[q] : F.dens[q];
             S.flags[r][sl] = F.flags[q];
             S.aidx[r][sl] = a;
-            S.slot[r][sl] = F.slot[q];
             S.sbin[r][sl] = uint8_t(hx);
-            S.lidx[r][sl] = S.nrm[hx] + rank;
         }
     }
     __syncthreads();
@@ -601,8 +610,8 @@ __device__ __forceinline__ void row_compute(const Ctx& c, const DevWorld& w, con
                     }
                     const int nslot = a0s[r == 0 ? 0 : (r == 1 ? 1 : 2)] + rest;
                     const uint32_t bonded = uint32_t((bmask >> ci) & 1ull);
-                    S.pself[p - W0] = uint8_t(sl);
-                    S.pnb[p - W0] = uint16_t((bonded << 15) | (uint32_t(r) << 8) | uint32_t(nslot));
+                    S.u.pr.pself[p - W0] = uint8_t(sl);
+                    S.u.pr.pnb[p - W0] = uint16_t((bonded << 15) | (uint32_t(r) << 8) | uint32_t(nslot));
                 }
             }
             __syncthreads();
@@ -610,16 +619,16 @@ __device__ __forceinline__ void row_compute(const Ctx& c, const DevWorld& w, con
                 // pair terms, one pair per lane
                 const int p = off + lane;
                 if (p < wend) {
-                    const uint32_t rec = S.pnb[p - W0];
+                    const uint32_t rec = S.u.pr.pnb[p - W0];
                     const int r = int((rec >> 8) & 3u);
                     const int rr = r == 0 ? rm_ : (r == 1 ? rc : rp);
-                    const Nb s = row_nb(S, rc, S.pself[p - W0]);
+                    const Nb s = row_nb(S, rc, S.u.pr.pself[p - W0]);
                     const Nb o = row_nb(S, rr, int(rec & 0xFFu));
                     float t[kPairTerms];
                     const uint32_t f = pair_term<kFluid>(c, w, P, wi, s, o, (rec >> 15) != 0u, t);
 #pragma unroll
-                    for (int v = 0; v < kPairTerms; ++v) S.term[v][lane] = t[v];
-                    S.pflag[lane] = f;
+                    for (int v = 0; v < kPairTerms; ++v) S.u.pr.term[v][lane] = t[v];
+                    S.u.pr.pflag[lane] = f;
                 }
                 __syncthreads();
                 // the owner adds its pairs of this round in visiting order
@@ -630,8 +639,8 @@ __device__ __forceinline__ void row_compute(const Ctx& c, const DevWorld& w, con
                         if (p2 - pbase == selfIdx) a.rho = a.rho + w0;
                         float t[kPairTerms];
 #pragma unroll
-                        for (int v = 0; v < kPairTerms; ++v) t[v] = S.term[v][k];
-                        acc_add<kFluid>(a, S.pflag[k], t);
+                        for (int v = 0; v < kPairTerms; ++v) t[v] = S.u.pr.term[v][k];
+                        acc_add<kFluid>(a, S.u.pr.pflag[k], t);
                     }
                 }
                 __syncthreads();
@@ -746,7 +755,7 @@ __global__ void __launch_bounds__(kRowThreads) k_neighbor_rows(Ctx c) {
         return;
     }
     const Region R{S.hist, g.x0, y0, g.IX, IY};
-    for (int b = lane; b < g.IX * IY; b += kRowThreads) S.hist[b] = 0u;
+    for (int b = lane; b < (g.IX * IY + 1) / 2; b += kRowThreads) S.hist[b] = 0u;
 
     RowFlight F, G;
     // rows y0-1, y0, y0+1 into ring slots 0, 1, 2
@@ -778,7 +787,7 @@ __global__ void __launch_bounds__(kRowThreads) k_neighbor_rows(Ctx c) {
     __syncthreads();
     for (int r = 0; r < IY; ++r)
         for (int x = lane; x < g.IX; x += kRowThreads)
-            c.privNext[g.binOff + (y0 + r) * g.nbx + g.x0 + x] = S.hist[r * g.IX + x];
+            c.privNext[g.binOff + (y0 + r) * g.nbx + g.x0 + x] = hist_count(S.hist, r * g.IX + x);
 }
 
 void launch_neighbor(const Ctx& c, const LaunchShape& sh, hipStream_t s, bool fluid, bool collision) {

@@ -137,7 +137,6 @@ __global__ void k_cell_scatter(Ctx c) {
     const float2 p = c.A.pos[i], v = c.A.vel[i];
     const float d = c.A.dens[i];
     const uint32_t f = c.A.flags[i];
-    const int sl = c.A.slot[i];
     const int gb = c.binOf[i];
     const int arr = c.arrival[i];
     int rank = arr;
@@ -152,7 +151,6 @@ __global__ void k_cell_scatter(Ctx c) {
     c.mvel[dest] = v;
     c.mdens[dest] = d;
     c.mflags[dest] = f;
-    c.mslot[dest] = sl;
 }
 
 __global__ void k_part_scatter(Ctx c) {

@@ -377,7 +377,7 @@ void WorldSet::synchronize() { check_hip(hipStreamSynchronize(stream_), "hipStre
 
 uint64_t WorldSet::estimateBytes(uint64_t cells, uint64_t particles, int W, int H, int binSize) {
     const uint64_t perCell = 2 * (8 + 8 + 4 + 4 + 4 + kMaxBonds * 12) + 8 + 4 * 4 + 8 + 3 * 8 + 2 * kMaxBonds * 8 + 4 +
-                             2 + 8 + 4 * 16 + 4 + (8 + 8 + 4 + 4 + 4) + 4;
+                             2 + 8 + 4 * 16 + 4 + (8 + 8 + 4 + 4) + 4;
     const uint64_t perParticle = 2 * (8 + 8 + 4 + 4 + 4 + 1) + 6 * 4;
     const uint64_t bins = uint64_t(std::max(1, W / binSize)) * uint64_t(std::max(1, H / binSize)) +
                           uint64_t(std::max(1, W / kParticleBin)) * uint64_t(std::max(1, H / kParticleBin));
@@ -531,7 +531,6 @@ void WorldSet::build(const std::vector<Source>& sources) {
     mvel_ = static_cast<float2*>(A(sizeof(float2) * NC));
     mdens_ = static_cast<float*>(A(sizeof(float) * NC));
     mflags_ = static_cast<uint32_t*>(A(sizeof(uint32_t) * NC));
-    mslot_ = static_cast<int*>(A(sizeof(int) * NC));
     newidx_ = static_cast<int*>(A(sizeof(int) * NC));
     psorted_ = static_cast<int*>(A(sizeof(int) * NP));
     staged_ = static_cast<int2*>(A(sizeof(int2) * NC));
@@ -687,7 +686,7 @@ int choose_strip(const std::vector<float2>& pos, int W, int H, int binSize) {
     };
     // widest strip whose staged rows (strip plus halo columns, one bin row) stay within 3/4 of the
     // LDS row capacity everywhere; denser rows take the slower global path
-    for (int sw : {kStripMax, 32, 16, 8}) {
+    for (int sw : {kStripMax, 16, 8}) {
         int64_t maxRow = 0;
         for (int y = 0; y < nby; ++y)
             for (int x0 = 0; x0 < nbx; x0 += sw) maxRow = std::max(maxRow, window(x0 - 1, y, x0 + sw + 1, y + 1));
@@ -770,8 +769,8 @@ void WorldSet::setBlockSize(int threads) {
 }
 
 void WorldSet::setStripCap(int cap) {
-    if (cap != 8 && cap != 16 && cap != 32 && cap != kStripMax)
-        throw Error(ALIEN_E_ARG, "strip cap must be 8, 16, 32 or 64");
+    if (cap != 4 && cap != 8 && cap != 16 && cap != kStripMax)
+        throw Error(ALIEN_E_ARG, "strip cap must be 4, 8, 16 or 32");
     stripCap_ = cap;
     if (worlds_.empty()) return;
     for (size_t i = 0; i < worlds_.size(); ++i) strip_shape(worlds_[i], std::min(stripChoice_[i], cap));
@@ -824,7 +823,6 @@ Ctx WorldSet::makeCtx(int parity) const {
     c.mvel = mvel_;
     c.mdens = mdens_;
     c.mflags = mflags_;
-    c.mslot = mslot_;
     c.newidx = newidx_;
     c.psorted = psorted_;
     c.staged = staged_;

@@ -173,7 +173,6 @@ private:
     float2 *mpos_ = nullptr, *mvel_ = nullptr;
     float* mdens_ = nullptr;
     uint32_t* mflags_ = nullptr;
-    int* mslot_ = nullptr;
     int2* staged_ = nullptr;
     float2 *own_ = nullptr, *shift_ = nullptr, *s2_ = nullptr, *out1_ = nullptr, *out2_ = nullptr;
     uint32_t* valid_ = nullptr;
