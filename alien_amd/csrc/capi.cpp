@@ -475,7 +475,7 @@ int alien_batch_autotune(alien_batch* b, uint32_t trialSteps, int32_t* blockSize
         double best = 1e300;
         int bestBlock = b->blockSize, bestStrip = b->set->stripCap();
         for (int blk : {64, 128, 256})
-            for (int sc : {kStripCapMax, 16, 8}) {
+            for (int sc : {kStripCapMax, 32, 16}) {
                 trial.setBlockSize(blk);
                 trial.setStripCap(sc);
                 trial.step(1);  // captures the graphs of this shape

@@ -16,13 +16,13 @@ constexpr int kMaxBonds = 6;
 constexpr int kColors = 7;
 constexpr int kParticleBin = 8;  // edge of the energy-particle bins, in world units
 constexpr int kNone = -1;
-// neighbour pass: one wave per strip of bin columns; entries of one staged bin row (strip plus two
-// halo columns) in LDS; widest strip.  The LDS footprint sets how many waves a CU keeps in flight,
-// and the pass is latency bound, so the row capacity is sized for strips of 32 columns at fluid
-// density (about 62 entries in the densest row of a 4M-cell 4096^2 world).
+// neighbour pass: one wave per strip of bin columns, three staged bin rows (strip plus two halo
+// columns each) in LDS; at most kRowCap entries per staged row (two per lane while in flight);
+// widest strip
 constexpr int kRowThreads = 64;
-constexpr int kRowCap = 96;
-constexpr int kStripMax = 32;
+constexpr int kRowCap = 128;
+constexpr int kRowPool = 256;  // entries of three consecutive staged rows
+constexpr int kStripMax = 64;
 constexpr int kChunkMax = 16;  // bin rows per neighbour-pass chunk
 constexpr int kClaimFree = 0x7fffffff;
 

@@ -339,37 +339,46 @@ __device__ __forceinline__ void cell_direct(const Ctx& c, const DevWorld& w, con
 
 // ---------------------------------------------------------------------------------------------
 // Row streaming.  A workgroup (one wave) owns a strip of up to kStripMax bin columns and a chunk of
-// bin rows, and walks the rows top to bottom with rows y-1, y, y+1 staged in LDS (a ring of three
-// rows).  While row y is computed, the entries of row y+2 and the bin counters of row y+3 are on
-// their way into registers; row y+2 is committed into the ring slot row y-1 leaves.  A staged
-// row covers the strip plus one halo column on each side, its entries stored bin by bin in rank
-// order (R1), so the three stencil bins of a cell in one row are one contiguous range.
+// bin rows, and walks the rows top to bottom with rows y-1, y, y+1 staged in LDS.  While row y is
+// computed, the entries of row y+2 and the bin counters of row y+3 are on their way into
+// registers; row y+2 is committed once row y-1 is no longer needed.  A staged row covers the
+// strip plus one halo column on each side, its entries stored bin by bin in rank order (R1), so
+// the three stencil bins of a cell in one row are one contiguous range.  The three rows share one
+// ring pool of entries (positions counted up without wrapping, stored modulo kPool): a row takes
+// the pool space right after the previous one, so any three consecutive rows of up to kPool
+// entries together fit, however they are distributed.
 // ---------------------------------------------------------------------------------------------
 constexpr int kRowBins = kStripMax + 2;
-constexpr int kPairCap = 256;  // pair records per evaluation window
+constexpr int kPool = kRowPool;
+constexpr int kPoolMask = kPool - 1;
+constexpr int kPairCap = 128;  // pair records per evaluation window
+static_assert((kPool & kPoolMask) == 0 && kPool <= 256, "pair records keep an 8-bit pool position");
+static_assert(kRowCap <= 2 * kRowThreads, "a row in flight holds at most two entries per lane");
 
 struct RowShared {
-    int rs[3][kRowBins + 1];  // per staged row: first slot of every halo-column bin, then the total
-    int rm[3][kRowBins];      // per staged row: map position of every halo-column bin's first entry
-    float2 pos[3][kRowCap];   // entries by slot
-    float2 vel[3][kRowCap];
-    float dens[3][kRowCap];   // fluid: 1 / density; collision: density
-    uint32_t flags[3][kRowCap];
-    int aidx[3][kRowCap];
-    uint8_t sbin[3][kRowCap];  // slot -> halo column
+    // per ring row: pool position of the first entry of every halo-column bin, then the end; map
+    // positions of the first entries of the left halo bin, the first interior bin and the right
+    // halo bin (the interior bins of a row are contiguous in the map)
+    int rs[3][kRowBins + 1];
+    int rm[3][3];
+    // entries of the staged rows, by pool position modulo kPool
+    float2 pos[kPool];
+    float2 vel[kPool];
+    float dens[kPool];  // fluid: 1 / density; collision: density
+    uint32_t flags[kPool];
+    int aidx[kPool];
+    uint8_t sbin[kPool];  // halo column
     // the row in flight
     int nrs[kRowBins + 1];
-    int nrm[kRowBins];
+    int nrm[3];
     uint8_t nebin[kRowCap];  // arrival index -> halo column
     union {
         int naidx[kRowCap];  // while a row is committed: arrival index -> previous index
         struct {
-            // while a row is computed, the pair records of the current window: the cell's slot in
-            // the middle row, and the neighbour as bonded << 15 | stencil row << 8 | slot
+            // while a row is computed, the pair records of the current window: the cell's pool
+            // position, and the neighbour as bonded << 15 | stencil row << 8 | pool position
             uint8_t pself[kPairCap];
             uint16_t pnb[kPairCap];
-            float term[kPairTerms][kRowThreads];
-            uint32_t pflag[kRowThreads];
         } pr;
     } u;
     uint32_t hist[kStripMax * kChunkMax / 2];  // next-step cells of the wave's own bins (16-bit pairs)
@@ -386,15 +395,27 @@ struct RowFlight {
     int total;
 };
 
-__device__ __forceinline__ Nb row_nb(const RowShared& S, int r, int sl) {
+struct StripGeom {
+    int x0, IX, HX;  // first interior bin column, interior columns, halo columns (IX + 2)
+    int nbx, nby, binOff, cellOff;
+    uint32_t base;   // start[] value of the world's first bin
+};
+
+// map segment of halo column hx: 0 the left halo bin, 1 the interior, 2 the right halo bin; and
+// the segment's first halo column
+__device__ __forceinline__ int seg_of(int hx, int HX) { return hx == 0 ? 0 : (hx == HX - 1 ? 2 : 1); }
+__device__ __forceinline__ int seg_first(int seg, int HX) { return seg == 0 ? 0 : (seg == 2 ? HX - 1 : 1); }
+
+// the staged entry at pool position pp (already reduced modulo kPool) of ring row r
+__device__ __forceinline__ Nb row_nb(const RowShared& S, const StripGeom& g, int r, int pp) {
     Nb o;
-    o.pos = to_f2(S.pos[r][sl]);
-    o.vel = to_f2(S.vel[r][sl]);
-    o.dens = S.dens[r][sl];
-    o.flags = S.flags[r][sl];
-    o.aidx = S.aidx[r][sl];
-    const int hx = S.sbin[r][sl];
-    o.lidx = S.rm[r][hx] + (sl - S.rs[r][hx]);
+    o.pos = to_f2(S.pos[pp]);
+    o.vel = to_f2(S.vel[pp]);
+    o.dens = S.dens[pp];
+    o.flags = S.flags[pp];
+    o.aidx = S.aidx[pp];
+    const int seg = seg_of(S.sbin[pp], g.HX);
+    o.lidx = S.rm[r][seg] + ((pp - S.rs[r][seg_first(seg, g.HX)]) & kPoolMask);
     return o;
 }
 
@@ -410,14 +431,14 @@ __device__ __forceinline__ Nb global_nb(const Ctx& c, int a, int l) {
     return o;
 }
 
-struct StripGeom {
-    int x0, IX, HX;  // first interior bin column, interior columns, halo columns (IX + 2)
-    int nbx, nby, binOff, cellOff;
-    uint32_t base;   // start[] value of the world's first bin
-};
-
 __device__ __forceinline__ int strip_bin(const StripGeom& g, int yy, int hx) {
     return g.binOff + wrap_small(yy, g.nby) * g.nbx + wrap_small(g.x0 - 1 + hx, g.nbx);
+}
+
+__device__ __forceinline__ int wave_max(int v) {
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v = max(v, __shfl_xor(v, d, kRowThreads));
+    return v;
 }
 
 // A: bin counters of row yy
@@ -444,16 +465,15 @@ __device__ __forceinline__ void row_issue_entries(const Ctx& c, const StripGeom&
     const int tot0 = __shfl(inc0, kRowThreads - 1);
     const int inc1 = int(wave_inclusive_scan(uint32_t(n1)));
     F.total = tot0 + __shfl(inc1, kRowThreads - 1);
-    const int off0 = inc0 - n0, off1 = tot0 + inc1 - n1;
-    if (lane < g.HX) {
-        S.nrs[lane] = off0;
-        S.nrm[lane] = g.cellOff + int(F.s0[0] - g.base);
-        for (int t = 0; t < n0; ++t) S.nebin[off0 + t] = uint8_t(lane);
-    }
-    if (lane + kRowThreads < g.HX) {
-        S.nrs[lane + kRowThreads] = off1;
-        S.nrm[lane + kRowThreads] = g.cellOff + int(F.s0[1] - g.base);
-        for (int t = 0; t < n1; ++t) S.nebin[off1 + t] = uint8_t(lane + kRowThreads);
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+        const int hx = lane + q * kRowThreads;
+        if (hx >= g.HX) continue;
+        const int off = q == 0 ? inc0 - n0 : tot0 + inc1 - n1;
+        const int n = q == 0 ? n0 : n1;
+        S.nrs[hx] = off;
+        if (hx <= 1 || hx == g.HX - 1) S.nrm[seg_of(hx, g.HX)] = g.cellOff + int(F.s0[q] - g.base);
+        for (int t = 0; t < n; ++t) S.nebin[off + t] = uint8_t(hx);
     }
     if (lane == 0) S.nrs[g.HX] = F.total;
     __syncthreads();
@@ -461,8 +481,8 @@ __device__ __forceinline__ void row_issue_entries(const Ctx& c, const StripGeom&
     for (int q = 0; q < 2; ++q) {
         const int e = lane + q * kRowThreads;
         if (e < F.total) {
-            const int hx = S.nebin[e];
-            const int m = S.nrm[hx] + (e - S.nrs[hx]);
+            const int seg = seg_of(S.nebin[e], g.HX);
+            const int m = S.nrm[seg] + (e - S.nrs[seg_first(seg, g.HX)]);
             F.aidx[q] = c.sorted[m];
             F.pos[q] = c.mpos[m];
             F.vel[q] = c.mvel[m];
@@ -472,14 +492,13 @@ __device__ __forceinline__ void row_issue_entries(const Ctx& c, const StripGeom&
     }
 }
 
-// C: the row into ring slot r, every entry at its rank by previous index inside its bin (R1)
+// C: the row into ring row r at pool position head, every entry at its rank by previous index
+// inside its bin (R1)
 template <bool kFluid>
-__device__ __forceinline__ void row_commit(const StripGeom& g, RowShared& S, const RowFlight& F, int r) {
+__device__ __forceinline__ void row_commit(const StripGeom& g, RowShared& S, const RowFlight& F, int r, int head) {
     const int lane = threadIdx.x;
-    for (int i = lane; i <= g.HX; i += kRowThreads) {
-        S.rs[r][i] = S.nrs[i];
-        if (i < g.HX) S.rm[r][i] = S.nrm[i];
-    }
+    for (int i = lane; i <= g.HX; i += kRowThreads) S.rs[r][i] = head + S.nrs[i];
+    if (lane < 3) S.rm[r][lane] = S.nrm[lane];
 #pragma unroll
     for (int q = 0; q < 2; ++q) {
         const int e = lane + q * kRowThreads;
@@ -495,19 +514,19 @@ __device__ __forceinline__ void row_commit(const StripGeom& g, RowShared& S, con
             const int a = F.aidx[q];
             int rank = 0;
             for (int u = s0; u < s1; ++u) rank += S.u.naidx[u] < a ? 1 : 0;
-            const int sl = s0 + rank;
-            S.pos[r][sl] = F.pos[q];
-            S.vel[r][sl] = F.vel[q];
-            S.dens[r][sl] = kFluid ? 1.0f / F.dens[q] : F.dens[q];
-            S.flags[r][sl] = F.flags[q];
-            S.aidx[r][sl] = a;
-            S.sbin[r][sl] = uint8_t(hx);
+            const int pp = (head + s0 + rank) & kPoolMask;
+            S.pos[pp] = F.pos[q];
+            S.vel[pp] = F.vel[q];
+            S.dens[pp] = kFluid ? 1.0f / F.dens[q] : F.dens[q];
+            S.flags[pp] = F.flags[q];
+            S.aidx[pp] = a;
+            S.sbin[pp] = uint8_t(hx);
         }
     }
     __syncthreads();
 }
 
-// neighbour predicate on staged data (is_neighbor, with the cell itself recognised by its slot)
+// neighbour predicate on staged data (is_neighbor, with the cell itself recognised by position)
 template <bool kFluid>
 __device__ __forceinline__ bool staged_neighbor(const Nb& s, f2 opos, bool isSelf, float Wf, float Hf, float reachSq) {
     if constexpr (kFluid)
@@ -516,7 +535,7 @@ __device__ __forceinline__ bool staged_neighbor(const Nb& s, f2 opos, bool isSel
         return !isSelf && lensq(sub(opos, s.pos)) <= reachSq;
 }
 
-// The cells of row y (ring slot rc; rows y-1 and y+1 in rm_ and rp).
+// The cells of row y (ring row rc; rows y-1 and y+1 in rm_ and rp).
 template <bool kFluid>
 __device__ __forceinline__ void row_compute(const Ctx& c, const DevWorld& w, const DevParams& P, int wi,
                                             const StripGeom& g, RowShared& S, const Region& R, int rm_, int rc,
@@ -527,7 +546,7 @@ __device__ __forceinline__ void row_compute(const Ctx& c, const DevWorld& w, con
     const int ncell = S.rs[rc][g.IX + 1] - cs;
     const float w0 = kFluid ? sph_w(0.0f) * P.invH2 : 0.0f;  // a fluid cell's own density term
     for (int q0 = 0; q0 < ncell; q0 += kRowThreads) {
-        // candidates of this lane's cell: three contiguous slot ranges, one per stencil row
+        // candidates of this lane's cell: three contiguous pool ranges, one per stencil row
         const int q = q0 + lane;
         const bool active = q < ncell;
         Nb self;
@@ -536,11 +555,12 @@ __device__ __forceinline__ void row_compute(const Ctx& c, const DevWorld& w, con
         unsigned long long mask = 0ull, bmask = 0ull;
         bool direct = false;
         int selfIdx = -1;  // fluid: number of the cell's pairs that come before its own density term
-        const int sl = cs + q;
+        const int su = cs + q;  // pool position, not wrapped
+        const int sp = su & kPoolMask;
         if (active) {
-            self = row_nb(S, rc, sl);
+            self = row_nb(S, g, rc, sp);
             load_bonds(c, self, bt);
-            const int hx = S.sbin[rc][sl];
+            const int hx = S.sbin[sp];
             a0s[0] = S.rs[rm_][hx - 1];
             ns[0] = S.rs[rm_][hx + 2] - a0s[0];
             a0s[1] = S.rs[rc][hx - 1];
@@ -557,25 +577,25 @@ __device__ __forceinline__ void row_compute(const Ctx& c, const DevWorld& w, con
                 int cidx = 0;
 #pragma unroll
                 for (int r = 0; r < 3; ++r) {
-                    const int rr = r == 0 ? rm_ : (r == 1 ? rc : rp);
                     const int a0 = a0s[r], n = ns[r];
                     for (int u = 0; u < n; u += 2) {
                         // two candidates per iteration: their loads overlap
-                        const f2 p0 = to_f2(S.pos[rr][a0 + u]);
+                        const f2 p0 = to_f2(S.pos[(a0 + u) & kPoolMask]);
                         const bool two = u + 1 < n;
-                        const f2 p1 = two ? to_f2(S.pos[rr][a0 + u + 1]) : p0;
+                        const f2 p1 = two ? to_f2(S.pos[(a0 + u + 1) & kPoolMask]) : p0;
 #pragma unroll
                         for (int h = 0; h < 2; ++h) {
                             if (h == 1 && !two) break;
                             const int cu = a0 + u + h;
-                            const bool isSelf = r == 1 && cu == sl;
+                            const bool isSelf = r == 1 && cu == su;
                             if (elideSelf && isSelf) {
                                 selfIdx = __builtin_popcountll(mask);
                                 continue;
                             }
                             if (staged_neighbor<kFluid>(self, h == 0 ? p0 : p1, isSelf, Wf, Hf, P.reachSq)) {
                                 mask |= 1ull << (cidx + u + h);
-                                if (hasBonds && is_bonded(bt, S.aidx[rr][cu] - w.cellOff)) bmask |= 1ull << (cidx + u + h);
+                                if (hasBonds && is_bonded(bt, S.aidx[cu & kPoolMask] - w.cellOff))
+                                    bmask |= 1ull << (cidx + u + h);
                             }
                         }
                     }
@@ -608,43 +628,46 @@ __device__ __forceinline__ void row_compute(const Ctx& c, const DevWorld& w, con
                             r = 2;
                         }
                     }
-                    const int nslot = a0s[r == 0 ? 0 : (r == 1 ? 1 : 2)] + rest;
+                    const int np = (a0s[r == 0 ? 0 : (r == 1 ? 1 : 2)] + rest) & kPoolMask;
                     const uint32_t bonded = uint32_t((bmask >> ci) & 1ull);
-                    S.u.pr.pself[p - W0] = uint8_t(sl);
-                    S.u.pr.pnb[p - W0] = uint16_t((bonded << 15) | (uint32_t(r) << 8) | uint32_t(nslot));
+                    S.u.pr.pself[p - W0] = uint8_t(sp);
+                    S.u.pr.pnb[p - W0] = uint16_t((bonded << 15) | (uint32_t(r) << 8) | uint32_t(np));
                 }
             }
             __syncthreads();
             for (int off = W0; off < wend; off += kRowThreads) {
                 // pair terms, one pair per lane
                 const int p = off + lane;
+                float t[kPairTerms];
+#pragma unroll
+                for (int v = 0; v < kPairTerms; ++v) t[v] = 0.0f;
+                uint32_t f = 0u;
                 if (p < wend) {
                     const uint32_t rec = S.u.pr.pnb[p - W0];
                     const int r = int((rec >> 8) & 3u);
                     const int rr = r == 0 ? rm_ : (r == 1 ? rc : rp);
-                    const Nb s = row_nb(S, rc, S.u.pr.pself[p - W0]);
-                    const Nb o = row_nb(S, rr, int(rec & 0xFFu));
-                    float t[kPairTerms];
-                    const uint32_t f = pair_term<kFluid>(c, w, P, wi, s, o, (rec >> 15) != 0u, t);
-#pragma unroll
-                    for (int v = 0; v < kPairTerms; ++v) S.u.pr.term[v][lane] = t[v];
-                    S.u.pr.pflag[lane] = f;
+                    const Nb s = row_nb(S, g, rc, S.u.pr.pself[p - W0]);
+                    const Nb o = row_nb(S, g, rr, int(rec & 0xFFu));
+                    f = pair_term<kFluid>(c, w, P, wi, s, o, (rec >> 15) != 0u, t);
                 }
-                __syncthreads();
-                // the owner adds its pairs of this round in visiting order
-                if (count > 0) {
-                    const int lo = max(pbase, off), hi = min(pbase + count, min(off + kRowThreads, wend));
-                    for (int p2 = lo; p2 < hi; ++p2) {
-                        const int k = p2 - off;
-                        if (p2 - pbase == selfIdx) a.rho = a.rho + w0;
-                        float t[kPairTerms];
+                // the owner adds its pairs of this round in visiting order, reading each from the
+                // lane that evaluated it (every lane takes part in every exchange)
+                const int lo = max(pbase, off), hi = min(pbase + count, min(off + kRowThreads, wend));
+                const int n = count > 0 ? max(0, hi - lo) : 0;
+                const int nmax = wave_max(n);
+                for (int j = 0; j < nmax; ++j) {
+                    const int src = (lo + j - off) & (kRowThreads - 1);
+                    float tt[kPairTerms];
 #pragma unroll
-                        for (int v = 0; v < kPairTerms; ++v) t[v] = S.u.pr.term[v][k];
-                        acc_add<kFluid>(a, S.u.pr.pflag[k], t);
+                    for (int v = 0; v < kPairTerms; ++v) tt[v] = __shfl(t[v], src, kRowThreads);
+                    const uint32_t ff = uint32_t(__shfl(int(f), src, kRowThreads));
+                    if (j < n) {
+                        if (lo + j - pbase == selfIdx) a.rho = a.rho + w0;
+                        acc_add<kFluid>(a, ff, tt);
                     }
                 }
-                __syncthreads();
             }
+            __syncthreads();
         }
         if (active && selfIdx == count) a.rho = a.rho + w0;  // the own term after all pairs
         if (active) {
@@ -653,7 +676,7 @@ __device__ __forceinline__ void row_compute(const Ctx& c, const DevWorld& w, con
 #pragma unroll
                     for (int r = 0; r < 3; ++r) {
                         const int rr = r == 0 ? rm_ : (r == 1 ? rc : rp);
-                        for (int u = a0s[r]; u < a0s[r] + ns[r]; ++u) visit(row_nb(S, rr, u));
+                        for (int u = a0s[r]; u < a0s[r] + ns[r]; ++u) visit(row_nb(S, g, rr, u & kPoolMask));
                     }
                 });
             } else {
@@ -737,16 +760,17 @@ __global__ void __launch_bounds__(kRowThreads) k_neighbor_rows(Ctx c) {
     const int IY = min(w.tileY, w.nby - y0);
     const int lane = threadIdx.x;
 
-    // rows y0-1 .. y0+IY must each fit the LDS row capacity
-    bool tooDense = false;
+    // rows y0-1 .. y0+IY must each fit a row in flight, and every three consecutive ones the pool
+    int total = 0;
     if (lane < IY + 2) {
         const int yy = y0 - 1 + lane;
         const int rowFirst = g.binOff + wrap_small(yy, g.nby) * g.nbx + g.x0;
-        int total = int(c.start[rowFirst + g.IX] - c.start[rowFirst]);
+        total = int(c.start[rowFirst + g.IX] - c.start[rowFirst]);
         const int gl = strip_bin(g, yy, 0), gr = strip_bin(g, yy, g.IX + 1);
         total += int(c.start[gl + 1] - c.start[gl]) + int(c.start[gr + 1] - c.start[gr]);
-        tooDense = total > kRowCap;
     }
+    const int three = total + __shfl_down(total, 1, kRowThreads) + __shfl_down(total, 2, kRowThreads);
+    const bool tooDense = total > kRowCap || (lane < IY && three > kPool);
     if (__any(tooDense)) {
         rows_direct<kFluid>(c, w, P, wi, g, y0, IY);
         // the owner-counted part of the region's bins is empty
@@ -758,11 +782,13 @@ __global__ void __launch_bounds__(kRowThreads) k_neighbor_rows(Ctx c) {
     for (int b = lane; b < (g.IX * IY + 1) / 2; b += kRowThreads) S.hist[b] = 0u;
 
     RowFlight F, G;
-    // rows y0-1, y0, y0+1 into ring slots 0, 1, 2
+    int head = 0;  // pool position of the next committed row
+    // rows y0-1, y0, y0+1 into ring rows 0, 1, 2
     for (int r = 0; r < 3; ++r) {
         row_issue_counts(c, g, y0 - 1 + r, F);
         row_issue_entries(c, g, S, F);
-        row_commit<kFluid>(g, S, F, r);
+        row_commit<kFluid>(g, S, F, r, head);
+        head += F.total;
     }
     if (IY >= 2) row_issue_counts(c, g, y0 + 2, F);
     for (int y = y0; y < y0 + IY; ++y) {
@@ -775,7 +801,8 @@ __global__ void __launch_bounds__(kRowThreads) k_neighbor_rows(Ctx c) {
         }
         row_compute<kFluid>(c, w, P, wi, g, S, R, rm_, rc, rp);
         if (next) {
-            row_commit<kFluid>(g, S, F, rm_);  // row y+2 takes the slot of row y-1
+            row_commit<kFluid>(g, S, F, rm_, head);  // row y+2 replaces row y-1
+            head += F.total;
 #pragma unroll
             for (int q = 0; q < 2; ++q) {
                 F.s0[q] = G.s0[q];
@@ -788,6 +815,13 @@ __global__ void __launch_bounds__(kRowThreads) k_neighbor_rows(Ctx c) {
     for (int r = 0; r < IY; ++r)
         for (int x = lane; x < g.IX; x += kRowThreads)
             c.privNext[g.binOff + (y0 + r) * g.nbx + g.x0 + x] = hist_count(S.hist, r * g.IX + x);
+}
+
+int neighbor_blocks_per_cu(bool fluid) {
+    int n = 0;
+    const hipError_t e = fluid ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_neighbor_rows<true>, kRowThreads, 0)
+                               : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_neighbor_rows<false>, kRowThreads, 0);
+    return e == hipSuccess ? std::max(1, n) : 1;
 }
 
 void launch_neighbor(const Ctx& c, const LaunchShape& sh, hipStream_t s, bool fluid, bool collision) {

@@ -31,6 +31,8 @@ struct Recorder {
 void launch_fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t s);
 // bins of the current state (A) for the next sort; the step keeps them up to date itself
 void launch_seed_bins(const Ctx& c, const LaunchShape& sh, hipStream_t s);
+// neighbour-pass workgroups one CU keeps resident (fluid or collision variant)
+int neighbor_blocks_per_cu(bool fluid);
 void launch_step(const Ctx& c, const LaunchShape& sh, hipStream_t s, Recorder* rec = nullptr);
 // cell map of the current state in logical order (sorted/newidx), for introspection; consumes
 // the seeded bins and seeds them again afterwards

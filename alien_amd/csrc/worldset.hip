@@ -620,6 +620,7 @@ void WorldSet::build(const std::vector<Source>& sources) {
     stagedHint_ = std::move(stagedHint);
     bondWork_ = std::move(bondWork);
     stripChoice_ = std::move(stripChoice);
+    chunk_shape(worlds_, params_);
     shape_.hasBondWork = false;
     for (uint8_t b : bondWork_) shape_.hasBondWork = shape_.hasBondWork || b != 0;
     parity_ = 0;
@@ -671,38 +672,76 @@ int choose_strip(const std::vector<float2>& pos, int W, int H, int binSize) {
         const int bx = std::min(sx / binSize, nbx - 1), by = std::min(sy / binSize, nby - 1);
         ++cnt[size_t(by) * nbx + bx];
     }
-    // 2D prefix sums for window counts (torus windows handled by clamping to the grid; the edge
-    // tiles see a little less than their true halo, the 25% headroom covers it)
-    std::vector<int64_t> pre(size_t(nbx + 1) * (nby + 1), 0);
-    for (int y = 0; y < nby; ++y)
-        for (int x = 0; x < nbx; ++x)
-            pre[size_t(y + 1) * (nbx + 1) + x + 1] = cnt[size_t(y) * nbx + x] + pre[size_t(y) * (nbx + 1) + x + 1] +
-                                                    pre[size_t(y + 1) * (nbx + 1) + x] - pre[size_t(y) * (nbx + 1) + x];
-    auto window = [&](int x0, int y0, int x1, int y1) {  // [x0,x1) x [y0,y1), clamped
-        x0 = std::max(x0, 0), y0 = std::max(y0, 0), x1 = std::min(x1, nbx), y1 = std::min(y1, nby);
-        if (x1 <= x0 || y1 <= y0) return int64_t(0);
-        return pre[size_t(y1) * (nbx + 1) + x1] - pre[size_t(y0) * (nbx + 1) + x1] - pre[size_t(y1) * (nbx + 1) + x0] +
-               pre[size_t(y0) * (nbx + 1) + x0];
-    };
-    // widest strip whose staged rows (strip plus halo columns, one bin row) stay within 3/4 of the
-    // LDS row capacity everywhere; denser rows take the slower global path
-    for (int sw : {kStripMax, 16, 8}) {
-        int64_t maxRow = 0;
-        for (int y = 0; y < nby; ++y)
-            for (int x0 = 0; x0 < nbx; x0 += sw) maxRow = std::max(maxRow, window(x0 - 1, y, x0 + sw + 1, y + 1));
-        if (maxRow * 4 <= int64_t(kRowCap) * 3) return sw;
+    // entries of every staged row as the neighbour pass stages it (interior plus the two halo
+    // columns, both wrapped around the torus), for each candidate width; a chunk whose rows exceed
+    // the row capacity, or whose three consecutive rows exceed the LDS pool, takes the slower
+    // global path, so the widest strip where no row does that at the start is taken
+    for (int sw : {kStripMax, 32, 16, 8}) {
+        bool fits = true;
+        std::vector<int> rows(static_cast<size_t>(nby));
+        for (int x0 = 0; x0 < nbx && fits; x0 += sw) {
+            const int HX = std::min(sw, nbx - x0) + 2;
+            for (int y = 0; y < nby; ++y) {
+                int n = 0;
+                for (int hx = 0; hx < HX; ++hx) n += cnt[size_t(y) * nbx + (x0 - 1 + hx + nbx) % nbx];
+                rows[size_t(y)] = n;
+            }
+            for (int y = 0; y < nby && fits; ++y) {
+                const int three = rows[size_t(y)] + rows[size_t((y + 1) % nby)] + rows[size_t((y + 2) % nby)];
+                fits = rows[size_t(y)] <= kRowCap && three <= kRowPool;
+            }
+        }
+        if (fits) return sw;
     }
     return 4;
 }
 
-// Strip width (capped by the world's columns), rows per chunk (long enough to reuse the staged
-// rows, short enough to give the whole chip work) and the strip/chunk counts.
+// Strip width (capped by the world's columns) and strip count; chunk_shape sets the rows.
 void strip_shape(DevWorld& w, int strip) {
     w.tileX = std::max(1, std::min(strip, w.nbx));
     w.ntx = (w.nbx + w.tileX - 1) / w.tileX;
-    w.tileY = std::max(2, std::min(kChunkMax, int((int64_t(w.ntx) * w.nby) / 4096)));
-    w.tileY = std::min(w.tileY, w.nby);
+    w.tileY = std::min(2, w.nby);
     w.nty = (w.nby + w.tileY - 1) / w.tileY;
+}
+
+// Rows per neighbour-pass chunk, one value for all worlds of a set.  A chunk stages two halo rows
+// besides its own (about kHaloCost computed rows' worth of work); its workgroups run in rounds of
+// as many as the chip keeps resident, and a partly filled last round costs a full one.  The rows
+// minimise rounds x (rows + kHaloCost).
+void chunk_shape(std::vector<DevWorld>& worlds, const std::vector<DevParams>& params) {
+    constexpr double kHaloCost = 0.6;
+    static const int numCUs = [] {
+        int dev = 0, n = 0;
+        if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+            return 256;
+        return std::max(1, n);
+    }();
+    bool fluid = false, collision = false;
+    for (size_t i = 0; i < worlds.size(); ++i) (params[i].motion == 0 ? fluid : collision) = true;
+    int perCU = 1 << 30;
+    if (fluid) perCU = std::min(perCU, neighbor_blocks_per_cu(true));
+    if (collision) perCU = std::min(perCU, neighbor_blocks_per_cu(false));
+    if (!fluid && !collision) perCU = 1;
+    const int64_t slots = int64_t(perCU) * numCUs;
+    int best = 2;
+    double bestCost = 1e300;
+    for (int rows = 2; rows <= kChunkMax; ++rows) {
+        int64_t tiles = 0;
+        for (const DevWorld& w : worlds) {
+            if (w.numCells == 0) continue;
+            const int ty = std::min(rows, w.nby);
+            tiles += int64_t(w.ntx) * ((w.nby + ty - 1) / ty);
+        }
+        const double cost = double((tiles + slots - 1) / slots) * (rows + kHaloCost);
+        if (cost < bestCost) {
+            bestCost = cost;
+            best = rows;
+        }
+    }
+    for (DevWorld& w : worlds) {
+        w.tileY = std::min(best, w.nby);
+        w.nty = (w.nby + w.tileY - 1) / w.tileY;
+    }
 }
 
 void WorldSet::buildTables() {
@@ -769,11 +808,12 @@ void WorldSet::setBlockSize(int threads) {
 }
 
 void WorldSet::setStripCap(int cap) {
-    if (cap != 4 && cap != 8 && cap != 16 && cap != kStripMax)
-        throw Error(ALIEN_E_ARG, "strip cap must be 4, 8, 16 or 32");
+    if (cap != 8 && cap != 16 && cap != 32 && cap != kStripMax)
+        throw Error(ALIEN_E_ARG, "strip cap must be 8, 16, 32 or 64");
     stripCap_ = cap;
     if (worlds_.empty()) return;
     for (size_t i = 0; i < worlds_.size(); ++i) strip_shape(worlds_[i], std::min(stripChoice_[i], cap));
+    chunk_shape(worlds_, params_);
     check_hip(hipMemcpy(dWorlds_, worlds_.data(), sizeof(DevWorld) * worlds_.size(), hipMemcpyHostToDevice), "H2D");
     buildTables();
 }

@@ -35,6 +35,7 @@ void validate_world(const alien_world_desc& d, const alien_params& p);
 DevParams to_dev_params(const alien_params& p);
 int choose_strip(const std::vector<float2>& pos, int W, int H, int binSize);
 void strip_shape(DevWorld& w, int strip);
+void chunk_shape(std::vector<DevWorld>& worlds, const std::vector<DevParams>& params);
 constexpr int kXcds = 8;  // accelerator complex dies of an MI355X
 
 // host-side identity and pass-through data of one world (fields the physics path never touches)

@@ -243,7 +243,7 @@ def main():
                     traffic = json.load(f).get("bytes_per_launch")
             roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                         "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                        "kernel": "k_neighbor", "avg_launch_us": avg_s * 1e6,
+                        "kernel": "k_neighbor_rows<true>", "avg_launch_us": avg_s * 1e6,
                         "bytes_per_launch": per_launch}
 
     step_bytes = BYTES_PER_FLUID_CELL * n + BYTES_PER_PARTICLE * args.particles

@@ -218,7 +218,7 @@ int alien_batch_set_block_size(alien_batch* b, int32_t threads);
 int alien_batch_set_use_graph(alien_batch* b, int32_t enable);
 /* Autotuner (no counterpart in ALIEN): times trialSteps steps of a device copy of the batch for
  * every candidate launch shape (cell-kernel block size 64/128/256 x neighbour-pass strip cap
- * 32/16/8 bins) and keeps the fastest.  The batch's own worlds do not advance and no result
+ * 64/32/16 bins) and keeps the fastest.  The batch's own worlds do not advance and no result
  * depends on the choice.  Outputs may be NULL. */
 int alien_batch_autotune(alien_batch* b, uint32_t trialSteps, int32_t* blockSize, int32_t* stripCap,
                          double* msPerStep);
