@@ -325,6 +325,14 @@ int alien_get_kernel_timing(alien_world* w, alien_kernel_time* out, int32_t cap,
     });
 }
 
+int alien_get_path_counts(alien_world* w, uint64_t* fallbackChunks, uint64_t* directCells) {
+    return guarded([&] {
+        requireHealthy(w);
+        if (!fallbackChunks || !directCells) throw Error(ALIEN_E_ARG, "null output pointer");
+        w->set->pathCounts(*fallbackChunks, *directCells);
+    });
+}
+
 // ---- batched worlds ---------------------------------------------------------------------------
 }  // extern "C"
 

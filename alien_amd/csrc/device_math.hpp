@@ -71,6 +71,12 @@ __device__ __forceinline__ float torus_delta1(float a, float b, float size) {
 __device__ __forceinline__ f2 torus_delta(f2 a, f2 b, float W, float H) {
     return {torus_delta1(a.x, b.x, W), torus_delta1(a.y, b.y, H)};
 }
+// torus_delta for points known not to straddle the seam (|a - b| <= size / 2 per axis) unless
+// `seam` is set: then the plain difference is the same value
+__device__ __forceinline__ f2 stencil_delta(f2 a, f2 b, float W, float H, bool seam) {
+    if (seam) return torus_delta(a, b, W, H);
+    return {a.x - b.x, a.y - b.y};
+}
 
 // BaseMap::correctPosition(float2&) (Map.cuh:20-26)
 __device__ __forceinline__ f2 torus_pos(f2 p, int W, int H) {

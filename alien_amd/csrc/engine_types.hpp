@@ -90,6 +90,10 @@ struct StepCounters {
     int candidates;  // fusion candidate pairs
     int errors;      // bit 0: candidate list overflow
     int pad;
+    // neighbour-pass paths since the data was set (not reset per step): chunks that walked their
+    // stencils in global memory, and cells of staged rows that walked theirs alone
+    unsigned long long fallbackChunks;
+    unsigned long long directCells;
 };
 
 // hot per-cell / per-particle state; two of these exist and alternate every step
@@ -142,7 +146,7 @@ struct Ctx {
     // that the neighbour pass reads each row of bins as one contiguous range
     float2* mpos;
     float2* mvel;
-    float* mdens;
+    float* mdens;  // fluid: 1 / density
     uint32_t* mflags;
     int* newidx;   // [cellCap] A index -> new index
     int* psorted;  // [partCap]

@@ -61,11 +61,11 @@ __device__ __forceinline__ bool is_neighbor(const Nb& s, const Nb& o, float Wf, 
 // The contribution of neighbour o to cell s, for a pair that passed is_neighbor.
 template <bool kFluid>
 __device__ __forceinline__ uint32_t pair_term(const Ctx& c, const DevWorld& w, const DevParams& P, int wi, const Nb& s,
-                                              const Nb& o, bool bonded, float* t) {
+                                              const Nb& o, bool bonded, float* t, bool seam = true) {
     const float Wf = float(w.W), Hf = float(w.H);
     const float cmd = P.minDistance;
     const bool bari = flags_barrier(s.flags);
-    const f2 pd = torus_delta(s.pos, o.pos, Wf, Hf);
+    const f2 pd = stencil_delta(s.pos, o.pos, Wf, Hf, seam);
     const float d = len2(pd);
     uint32_t f = 0u;
     if constexpr (kFluid) {
@@ -399,6 +399,7 @@ struct StripGeom {
     int x0, IX, HX;  // first interior bin column, interior columns, halo columns (IX + 2)
     int nbx, nby, binOff, cellOff;
     uint32_t base;   // start[] value of the world's first bin
+    bool seam;       // the staged rows or columns wrap around the torus
 };
 
 // map segment of halo column hx: 0 the left halo bin, 1 the interior, 2 the right halo bin; and
@@ -517,7 +518,7 @@ __device__ __forceinline__ void row_commit(const StripGeom& g, RowShared& S, con
             const int pp = (head + s0 + rank) & kPoolMask;
             S.pos[pp] = F.pos[q];
             S.vel[pp] = F.vel[q];
-            S.dens[pp] = kFluid ? 1.0f / F.dens[q] : F.dens[q];
+            S.dens[pp] = F.dens[q];
             S.flags[pp] = F.flags[q];
             S.aidx[pp] = a;
             S.sbin[pp] = uint8_t(hx);
@@ -528,9 +529,10 @@ __device__ __forceinline__ void row_commit(const StripGeom& g, RowShared& S, con
 
 // neighbour predicate on staged data (is_neighbor, with the cell itself recognised by position)
 template <bool kFluid>
-__device__ __forceinline__ bool staged_neighbor(const Nb& s, f2 opos, bool isSelf, float Wf, float Hf, float reachSq) {
+__device__ __forceinline__ bool staged_neighbor(const Nb& s, f2 opos, bool isSelf, float Wf, float Hf, float reachSq,
+                                                bool seam) {
     if constexpr (kFluid)
-        return lensq(torus_delta(s.pos, opos, Wf, Hf)) <= reachSq;
+        return lensq(stencil_delta(s.pos, opos, Wf, Hf, seam)) <= reachSq;
     else
         return !isSelf && lensq(sub(opos, s.pos)) <= reachSq;
 }
@@ -592,7 +594,7 @@ __device__ __forceinline__ void row_compute(const Ctx& c, const DevWorld& w, con
                                 selfIdx = __builtin_popcountll(mask);
                                 continue;
                             }
-                            if (staged_neighbor<kFluid>(self, h == 0 ? p0 : p1, isSelf, Wf, Hf, P.reachSq)) {
+                            if (staged_neighbor<kFluid>(self, h == 0 ? p0 : p1, isSelf, Wf, Hf, P.reachSq, g.seam)) {
                                 mask |= 1ull << (cidx + u + h);
                                 if (hasBonds && is_bonded(bt, S.aidx[cu & kPoolMask] - w.cellOff))
                                     bmask |= 1ull << (cidx + u + h);
@@ -648,7 +650,7 @@ __device__ __forceinline__ void row_compute(const Ctx& c, const DevWorld& w, con
                     const int rr = r == 0 ? rm_ : (r == 1 ? rc : rp);
                     const Nb s = row_nb(S, g, rc, S.u.pr.pself[p - W0]);
                     const Nb o = row_nb(S, g, rr, int(rec & 0xFFu));
-                    f = pair_term<kFluid>(c, w, P, wi, s, o, (rec >> 15) != 0u, t);
+                    f = pair_term<kFluid>(c, w, P, wi, s, o, (rec >> 15) != 0u, t, g.seam);
                 }
                 // the owner adds its pairs of this round in visiting order, reading each from the
                 // lane that evaluated it (every lane takes part in every exchange)
@@ -672,6 +674,7 @@ __device__ __forceinline__ void row_compute(const Ctx& c, const DevWorld& w, con
         if (active && selfIdx == count) a.rho = a.rho + w0;  // the own term after all pairs
         if (active) {
             if (direct) {
+                atomicAdd(&c.sc->directCells, 1ull);
                 cell_direct<kFluid>(c, w, P, wi, self, R, [&](auto&& visit) {
 #pragma unroll
                     for (int r = 0; r < 3; ++r) {
@@ -759,6 +762,10 @@ __global__ void __launch_bounds__(kRowThreads) k_neighbor_rows(Ctx c) {
     const int y0 = chunk * w.tileY;
     const int IY = min(w.tileY, w.nby - y0);
     const int lane = threadIdx.x;
+    // Away from the seam no staged bin wraps, and two stencil neighbours lie less than three bin
+    // edges apart (the last column and row absorb less than one edge of remainder) while half the
+    // world is at least four: the torus difference is the plain difference there, bit for bit.
+    g.seam = g.x0 == 0 || g.x0 + g.IX >= g.nbx || y0 == 0 || y0 + IY >= g.nby || g.nbx < 8 || g.nby < 8;
 
     // rows y0-1 .. y0+IY must each fit a row in flight, and every three consecutive ones the pool
     int total = 0;
@@ -772,6 +779,7 @@ __global__ void __launch_bounds__(kRowThreads) k_neighbor_rows(Ctx c) {
     const int three = total + __shfl_down(total, 1, kRowThreads) + __shfl_down(total, 2, kRowThreads);
     const bool tooDense = total > kRowCap || (lane < IY && three > kPool);
     if (__any(tooDense)) {
+        if (lane == 0) atomicAdd(&c.sc->fallbackChunks, 1ull);
         rows_direct<kFluid>(c, w, P, wi, g, y0, IY);
         // the owner-counted part of the region's bins is empty
         for (int r = 0; r < IY; ++r)

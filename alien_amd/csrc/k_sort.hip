@@ -135,7 +135,8 @@ __global__ void k_cell_scatter(Ctx c) {
     int i = blk.y + int(threadIdx.x);
     if (i >= w.cellOff + w.numCells) return;
     const float2 p = c.A.pos[i], v = c.A.vel[i];
-    const float d = c.A.dens[i];
+    // the neighbour pass of a fluid world reads 1 / density
+    const float d = c.params[blk.x].motion == 0 ? 1.0f / c.A.dens[i] : c.A.dens[i];
     const uint32_t f = c.A.flags[i];
     const int gb = c.binOf[i];
     const int arr = c.arrival[i];

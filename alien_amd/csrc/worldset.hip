@@ -823,6 +823,16 @@ void WorldSet::setUseGraph(bool on) {
     invalidateGraphs();
 }
 
+void WorldSet::pathCounts(uint64_t& fallbackChunks, uint64_t& directCells) {
+    fallbackChunks = directCells = 0;
+    if (!dSc_) return;
+    check_hip(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+    StepCounters sc{};
+    check_hip(hipMemcpy(&sc, dSc_, sizeof(sc), hipMemcpyDeviceToHost), "D2H");
+    fallbackChunks = sc.fallbackChunks;
+    directCells = sc.directCells;
+}
+
 void WorldSet::setKernelTiming(bool on) {
     timing_ = on;
     if (on && !recorder_) recorder_ = std::make_unique<EventRecorder>(stream_);

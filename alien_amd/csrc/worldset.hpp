@@ -121,6 +121,8 @@ public:
     void setUseGraph(bool on);
     // kernel timing mode: no graphs, every launch bracketed by events (see alien_set_kernel_timing)
     void setKernelTiming(bool on);
+    // neighbour-pass fallback counts since the data was set: chunks, cells
+    void pathCounts(uint64_t& fallbackChunks, uint64_t& directCells);
     struct KernelTime {
         uint64_t launches = 0;
         double totalMs = 0.0;

@@ -86,6 +86,7 @@ def lib():
                                            ctypes.POINTER(ctypes.c_double)]
         L.alien_set_kernel_timing.argtypes = [vp, i32]
         L.alien_get_kernel_timing.argtypes = [vp, ctypes.POINTER(KernelTime), i32, ctypes.POINTER(i32)]
+        L.alien_get_path_counts.argtypes = [vp, pu64, pu64]
         L.alien_batch_set_kernel_timing.argtypes = [vp, i32]
         L.alien_batch_get_kernel_timing.argtypes = [vp, ctypes.POINTER(KernelTime), i32, ctypes.POINTER(i32)]
         L.alien_server_create.argtypes = [u64, ctypes.c_uint32, ctypes.POINTER(vp)]
@@ -229,6 +230,12 @@ class Simulation:
     def kernel_times(self):
         """{kernel name: (launches, total ms)} since the last call (kernel timing mode)."""
         return _kernel_times(lib().alien_get_kernel_timing, self._h)
+
+    def path_counts(self):
+        """(chunks on the global-memory path, cells on the single-cell path) since set_data."""
+        a, b = ctypes.c_uint64(), ctypes.c_uint64()
+        check(lib().alien_get_path_counts(self._h, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
 
     def get_map(self):
         n, _ = self.counts()

@@ -382,3 +382,33 @@ def random_mix(seed, size=(64, 64), num_free=300, patches=6, num_particles=200, 
     assign_ids(cells, parts)
     params = default_params(motionType=motion, smoothingLength=0.65)
     return World(f"mix_{seed}", size, params, cells, parts, 50, seed)
+
+
+def crowded_bins(seed, motion=MOTION_COLLISION, size=(96, 96), background=400):
+    """Free cells crowded into a few bins so that the neighbour pass takes each of its slower
+    paths: one bin of 140 cells (a staged bin row above 128 entries), three vertically stacked
+    bins of 95 cells (three consecutive rows above 256 entries together) and a 3x3 block of bins
+    with 9 cells each (cells with more than 64 candidates), over a sparse background.  Bins are 2
+    units wide at these parameters (interaction radius 1.3)."""
+    rng = np.random.default_rng(seed)
+    params = default_params(motionType=motion, smoothingLength=0.65)
+
+    def in_bin(bx, by, n):
+        p = np.empty((n, 2))
+        p[:, 0] = 2.0 * bx + rng.uniform(0.05, 1.95, n)
+        p[:, 1] = 2.0 * by + rng.uniform(0.05, 1.95, n)
+        return p
+
+    groups = [in_bin(10, 10, 140), in_bin(30, 20, 95), in_bin(30, 21, 95), in_bin(30, 22, 95)]
+    groups += [in_bin(5 + dx, 40 + dy, 9) for dx in range(3) for dy in range(3)]
+    crowd_pos = np.concatenate(groups)
+    crowd = np.zeros(len(crowd_pos), dtype=CELL_DTYPE)
+    crowd["pos"] = crowd_pos.astype(np.float32)
+    crowd["vel"] = rng.normal(0.0, 0.05, size=(len(crowd), 2)).astype(np.float32)
+    crowd["energy"] = np.float32(100.0)
+    crowd["stiffness"] = np.float32(1.0)
+    free = free_cells(rng, size, background, vel_sigma=0.2)
+    cells = concat_cells([crowd, free])
+    parts = particles(rng, size, 50, vel_sigma=0.5)
+    assign_ids(cells, parts)
+    return World(f"crowded_{seed}", size, params, cells, parts, 4, seed)

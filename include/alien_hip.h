@@ -191,6 +191,12 @@ typedef struct alien_kernel_time {
 int alien_set_kernel_timing(alien_world* w, int32_t enable);
 int alien_get_kernel_timing(alien_world* w, alien_kernel_time* out, int32_t cap, int32_t* count);
 
+/* Neighbour-pass path counts since the world's data was last set (no counterpart in ALIEN; a
+ * performance diagnostic, results never depend on the path): chunks of bin rows too crowded for
+ * the LDS staging (a row above 128 entries or three consecutive rows above 256), which walk their
+ * stencils in global memory, and cells with more than 64 candidates, which walk theirs alone. */
+int alien_get_path_counts(alien_world* w, uint64_t* fallbackChunks, uint64_t* directCells);
+
 /* ---- batched worlds ------------------------------------------------------------------------ */
 
 typedef struct alien_batch alien_batch;

@@ -79,6 +79,18 @@ def test_mix_dense_fusion_bit_exact():
     np.testing.assert_allclose(H.angle_sums(c), 360.0, atol=2e-3)
 
 
+@pytest.mark.parametrize("motion", [MOTION_FLUID, MOTION_COLLISION])
+def test_crowded_bins_take_the_fallback_paths_bit_exact(motion):
+    """A staged row above 128 entries and three consecutive rows above 256 (whole chunks walk
+    their stencils in global memory) and cells with more than 64 candidates (one cell walks its
+    stencil alone): same results as the oracle."""
+    w = worlds.crowded_bins(11 + motion, motion=motion)
+    say(f"crowded bins motion={motion} cells={len(w.cells)}")
+    sim, _ = run_both(w, [1, 2, 4])
+    chunks, cells = sim.path_counts()
+    assert chunks > 0 and cells > 0, (chunks, cells)
+
+
 def test_plain_soft_body_bit_exact():
     w = worlds.plain_soft_body()
     say(f"plain soft body cells={len(w.cells)}")
