@@ -18,6 +18,8 @@
 // collision and viscosity denominators are products with the correctly rounded reciprocal (R16
 // of the oracle), and the length comparisons |v| > c are tests of |v|^2 against the exact
 // threshold of the correctly rounded square root.
+#include <type_traits>
+
 #include "kernels_common.hpp"
 #include "kernels.hpp"
 
@@ -226,8 +228,10 @@ __device__ __forceinline__ void bin_next(const Ctx& c, const DevWorld& w, int k,
         const int b = ly * R.IX + lx;
         const uint32_t sh = uint32_t(b & 1) << 4;
         c.arrival[k] = int((atomicAdd(&R.hist[b >> 1], 1u << sh) >> sh) & 0xFFFFu);
-    } else
-        c.arrival[k] = kArrivalShared | int(atomicAdd(&c.countNext[gb], 1u));
+    } else {
+        atomicAdd(&c.countNext[gb], 1u);
+        c.arrival[k] = kArrivalShared;
+    }
 }
 
 __device__ __forceinline__ uint32_t hist_count(const uint32_t* hist, int b) {
@@ -237,7 +241,7 @@ __device__ __forceinline__ uint32_t hist_count(const uint32_t* hist, int b) {
 // Barrier reflection, applyForces, and the end-of-step writes of one cell.
 template <bool kFluid>
 __device__ __forceinline__ void cell_finish(const Ctx& c, const DevWorld& w, const DevParams& P, int wi, const Nb& s,
-                                            const int bt[kMaxBonds], Acc& a, const Region& R) {
+                                            int slot, const int bt[kMaxBonds], Acc& a, const Region& R) {
     const int cap = c.cellCap;
     const float Wf = float(w.W), Hf = float(w.H);
     const int i = s.aidx, k = s.lidx;
@@ -285,7 +289,7 @@ __device__ __forceinline__ void cell_finish(const Ctx& c, const DevWorld& w, con
     // the cell in the end-of-step order; bond targets stay in start-of-step indices until the
     // staged cells are renumbered (k_remap)
     c.B.flags[k] = fi;
-    c.B.slot[k] = c.A.slot[i];
+    c.B.slot[k] = slot;
     c.B.dens[k] = kFluid ? a.rho : s.dens;
     for (int b = 0; b < nci; ++b) {
         c.B.btgt[b * cap + k] = bt[b];
@@ -333,7 +337,7 @@ __device__ __forceinline__ void cell_direct(const Ctx& c, const DevWorld& w, con
         const uint32_t f = pair_term<kFluid>(c, w, P, wi, s, o, is_bonded(bt, o.aidx - w.cellOff), t);
         acc_add<kFluid>(a, f, t);
     });
-    cell_finish<kFluid>(c, w, P, wi, s, bt, a, R);
+    cell_finish<kFluid>(c, w, P, wi, s, c.A.slot[s.aidx], bt, a, R);
 }
 
 
@@ -352,6 +356,7 @@ constexpr int kRowBins = kStripMax + 2;
 constexpr int kPool = kRowPool;
 constexpr int kPoolMask = kPool - 1;
 constexpr int kPairCap = 128;  // pair records per evaluation window
+constexpr int kPosMirror = 64;
 static_assert((kPool & kPoolMask) == 0 && kPool <= 256, "pair records keep an 8-bit pool position");
 static_assert(kRowCap <= 2 * kRowThreads, "a row in flight holds at most two entries per lane");
 
@@ -361,8 +366,9 @@ struct RowShared {
     // halo bin (the interior bins of a row are contiguous in the map)
     int rs[3][kRowBins + 1];
     int rm[3][3];
-    // entries of the staged rows, by pool position modulo kPool
-    float2 pos[kPool];
+    // entries of the staged rows, by pool position modulo kPool; positions 0..kPosMirror-1 are
+    // repeated after the end so that a stencil range of up to 64 entries reads without wrapping
+    float2 pos[kPool + kPosMirror];
     float2 vel[kPool];
     float dens[kPool];  // fluid: 1 / density; collision: density
     uint32_t flags[kPool];
@@ -517,6 +523,7 @@ __device__ __forceinline__ void row_commit(const StripGeom& g, RowShared& S, con
             for (int u = s0; u < s1; ++u) rank += S.u.naidx[u] < a ? 1 : 0;
             const int pp = (head + s0 + rank) & kPoolMask;
             S.pos[pp] = F.pos[q];
+            if (pp < kPosMirror) S.pos[kPool + pp] = F.pos[q];
             S.vel[pp] = F.vel[q];
             S.dens[pp] = F.dens[q];
             S.flags[pp] = F.flags[q];
@@ -525,16 +532,6 @@ __device__ __forceinline__ void row_commit(const StripGeom& g, RowShared& S, con
         }
     }
     __syncthreads();
-}
-
-// neighbour predicate on staged data (is_neighbor, with the cell itself recognised by position)
-template <bool kFluid>
-__device__ __forceinline__ bool staged_neighbor(const Nb& s, f2 opos, bool isSelf, float Wf, float Hf, float reachSq,
-                                                bool seam) {
-    if constexpr (kFluid)
-        return lensq(stencil_delta(s.pos, opos, Wf, Hf, seam)) <= reachSq;
-    else
-        return !isSelf && lensq(sub(opos, s.pos)) <= reachSq;
 }
 
 // The cells of row y (ring row rc; rows y-1 and y+1 in rm_ and rp).
@@ -559,8 +556,10 @@ __device__ __forceinline__ void row_compute(const Ctx& c, const DevWorld& w, con
         int selfIdx = -1;  // fluid: number of the cell's pairs that come before its own density term
         const int su = cs + q;  // pool position, not wrapped
         const int sp = su & kPoolMask;
+        int slot = 0;
         if (active) {
             self = row_nb(S, g, rc, sp);
+            slot = c.A.slot[self.aidx];  // needed only at the end: loaded early
             load_bonds(c, self, bt);
             const int hx = S.sbin[sp];
             a0s[0] = S.rs[rm_][hx - 1];
@@ -574,34 +573,53 @@ __device__ __forceinline__ void row_compute(const Ctx& c, const DevWorld& w, con
             } else {
                 const bool hasBonds = flags_num_conn(self.flags) > 0;
                 // fluid: the cell's own density term needs no pair evaluation; only its place in
-                // the visiting order is kept (a barrier cell meets itself as a barrier instead)
-                const bool elideSelf = kFluid && !flags_barrier(self.flags);
-                int cidx = 0;
+                // the visiting order is kept (a barrier cell meets itself as a barrier instead);
+                // collision: the cell is not its own neighbour
+                const bool keepSelf = kFluid && flags_barrier(self.flags);
+                const int selfFlat = ns[0] + (su - a0s[1]);  // the cell's own visiting index
+                // predicated tests, one mask bit per candidate in visiting order (one copy of the
+                // loop for chunks on the torus seam, one for the rest)
+                auto test = [&](auto seamTag) {
+                    constexpr bool kSeam = decltype(seamTag)::value;
+                    auto dsq = [&](f2 o) {
+                        return kFluid ? lensq(kSeam ? torus_delta(self.pos, o, Wf, Hf) : sub(self.pos, o))
+                                      : lensq(sub(o, self.pos));  // R13
+                    };
+                    int cidx = 0;
 #pragma unroll
-                for (int r = 0; r < 3; ++r) {
-                    const int a0 = a0s[r], n = ns[r];
-                    for (int u = 0; u < n; u += 2) {
-                        // two candidates per iteration: their loads overlap
-                        const f2 p0 = to_f2(S.pos[(a0 + u) & kPoolMask]);
-                        const bool two = u + 1 < n;
-                        const f2 p1 = two ? to_f2(S.pos[(a0 + u + 1) & kPoolMask]) : p0;
-#pragma unroll
-                        for (int h = 0; h < 2; ++h) {
-                            if (h == 1 && !two) break;
-                            const int cu = a0 + u + h;
-                            const bool isSelf = r == 1 && cu == su;
-                            if (elideSelf && isSelf) {
-                                selfIdx = __builtin_popcountll(mask);
-                                continue;
-                            }
-                            if (staged_neighbor<kFluid>(self, h == 0 ? p0 : p1, isSelf, Wf, Hf, P.reachSq, g.seam)) {
-                                mask |= 1ull << (cidx + u + h);
-                                if (hasBonds && is_bonded(bt, S.aidx[cu & kPoolMask] - w.cellOff))
-                                    bmask |= 1ull << (cidx + u + h);
-                            }
+                    for (int r = 0; r < 3; ++r) {
+                        const int pb = a0s[r] & kPoolMask, n = ns[r];
+                        unsigned long long rowMask = 0ull;
+                        for (int u = 0; u < n; u += 2) {
+                            // two candidates per iteration; the second is read even past the
+                            // range end (within the mirrored pool) and trimmed below
+                            const f2 p0 = to_f2(S.pos[pb + u]);
+                            const f2 p1 = to_f2(S.pos[pb + u + 1]);
+                            rowMask |= dsq(p0) <= P.reachSq ? (1ull << u) : 0ull;
+                            rowMask |= dsq(p1) <= P.reachSq ? (2ull << u) : 0ull;
                         }
+                        rowMask &= n >= 64 ? ~0ull : (1ull << n) - 1ull;
+                        mask |= rowMask << cidx;
+                        cidx += n;
                     }
-                    cidx += n;
+                };
+                if (g.seam)
+                    test(std::true_type{});
+                else
+                    test(std::false_type{});
+                if (!keepSelf) mask &= ~(1ull << selfFlat);
+                if (kFluid && !keepSelf) selfIdx = __builtin_popcountll(mask & ((1ull << selfFlat) - 1ull));
+                if (hasBonds) {
+                    // which of the neighbours are bonded
+                    unsigned long long m = mask;
+                    while (m) {
+                        const int ci = __builtin_ctzll(m);
+                        m &= m - 1ull;
+                        const int r = ci < ns[0] ? 0 : (ci < ns[0] + ns[1] ? 1 : 2);
+                        const int rest = ci - (r == 0 ? 0 : (r == 1 ? ns[0] : ns[0] + ns[1]));
+                        const int np = (a0s[r == 0 ? 0 : (r == 1 ? 1 : 2)] + rest) & kPoolMask;
+                        if (is_bonded(bt, S.aidx[np] - w.cellOff)) bmask |= 1ull << ci;
+                    }
                 }
             }
         }
@@ -683,7 +701,7 @@ __device__ __forceinline__ void row_compute(const Ctx& c, const DevWorld& w, con
                     }
                 });
             } else {
-                cell_finish<kFluid>(c, w, P, wi, self, bt, a, R);
+                cell_finish<kFluid>(c, w, P, wi, self, slot, bt, a, R);
             }
         }
     }

@@ -126,9 +126,10 @@ __global__ void __launch_bounds__(kScanThreads) k_scan_down(const uint32_t* coun
     }
 }
 
-// entries land in their bin in arrival order (the in-bin order is fixed later by ranking), with
-// a copy of the state the neighbour pass reads; counting the bin down to zero hands every entry
-// its own position and clears the histogram
+// entries land in their bin (the in-bin order is fixed later by ranking), with a copy of the
+// state the neighbour pass reads: owner-counted cells at the rank they got when counted, the
+// shared ones after them, each at the place its count-down of the shared histogram returns
+// (which also clears the histogram)
 __global__ void k_cell_scatter(Ctx c) {
     int2 blk = c.cellBlocks[blockIdx.x];
     const DevWorld w = c.worlds[blk.x];
@@ -141,11 +142,7 @@ __global__ void k_cell_scatter(Ctx c) {
     const int gb = c.binOf[i];
     const int arr = c.arrival[i];
     int rank = arr;
-    if (arr & kArrivalShared) {
-        // shared arrivals come after the owner-counted ones; counting down clears the histogram
-        rank = int(c.priv[gb]) + (arr & ~kArrivalShared);
-        atomicSub(&c.count[gb], 1u);
-    }
+    if (arr & kArrivalShared) rank = int(c.priv[gb]) + int(atomicSub(&c.count[gb], 1u)) - 1;
     const int dest = w.cellOff + int(c.start[gb] - c.start[w.binOff]) + rank;
     c.sorted[dest] = i;
     c.mpos[dest] = p;
