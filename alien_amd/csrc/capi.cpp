@@ -453,6 +453,15 @@ int alien_batch_set_kernel_timing(alien_batch* b, int32_t enable) {
     });
 }
 
+int alien_batch_get_path_counts(alien_batch* b, uint64_t* fallbackChunks, uint64_t* directCells) {
+    return guarded([&] {
+        require(b != nullptr, "null batch");
+        if (!fallbackChunks || !directCells) throw Error(ALIEN_E_ARG, "null output pointer");
+        *fallbackChunks = *directCells = 0;
+        if (b->set) b->set->pathCounts(*fallbackChunks, *directCells);
+    });
+}
+
 int alien_batch_get_kernel_timing(alien_batch* b, alien_kernel_time* out, int32_t cap, int32_t* count) {
     return guarded([&] {
         require(b != nullptr, "null batch");

@@ -59,8 +59,9 @@ struct DevParams {
     // largest float whose correctly rounded square root does not exceed R
     float reachSq;
     // the same for the other length comparisons of the pass: |v| > maxVelocity <=> |v|^2 > maxVelSq,
-    // |v| > 0.5 <=> |v|^2 > fastSq, |dv| < fusionVelocity <=> |dv|^2 <= fusionBelowSq
-    float maxVelSq, fastSq, fusionBelowSq;
+    // |v| > 0.5 <=> |v|^2 > fastSq, |dv| < fusionVelocity <=> |dv|^2 <= fusionBelowSq,
+    // |d| < minDistance <=> |d|^2 <= overlapBelowSq
+    float maxVelSq, fastSq, fusionBelowSq, overlapBelowSq;
     float invH, invH2, invH3;  // 1/h, 1/h^2, 1/h^3 (fluid)
 };
 
@@ -153,6 +154,7 @@ struct Ctx {
     int2* staged;  // [cellCap] (new index, world)
     float2* own;    // [cellCap]
     float2* shift;  // [cellCap] Verlet displacement of a staged cell, applied after the second force pass
+    float* bstiff;  // [cellCap] stiffness of a bonded cell in end-of-step order (written by the neighbour pass)
     float2* s2;     // [cellCap] connection force of the first pass
     float2* out1;  // [kMaxBonds][cellCap]
     float2* out2;

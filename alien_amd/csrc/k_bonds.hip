@@ -60,7 +60,7 @@ __global__ void k_conn_own(Ctx c, int pass) {
         const bool considerAngles = (c.ctr[wi].timestep % 3ull) == 0ull;
         const f2 px = to_f2(c.B.pos[x]);
         const f2 shx = pass == 0 ? f2{0.0f, 0.0f} : to_f2(c.shift[x]);
-        const float sx = c.stiff[w.cellOff + c.B.slot[x]];
+        const float sx = c.bstiff[x];
         const float kx2 = sx * sx;
         int tg[kMaxBonds];
 #pragma unroll
@@ -72,7 +72,7 @@ __global__ void k_conn_own(Ctx c, int pass) {
         const float maxBind = P.maxBindingDistance[flags_color(fx) % kColors];
         for (int b = 0; b < nc; ++b) {
             const int o = tg[b];
-            const float so = c.stiff[w.cellOff + c.B.slot[o]];
+            const float so = c.bstiff[o];
             const float kc2 = so * so;
             const f2 disp = bond_vector(c, o, px, shx, pass, Wf, Hf);
             const float ad = len2(disp);
@@ -131,6 +131,8 @@ __device__ __forceinline__ f2 gather_conn(const Ctx& c, const DevWorld& w, int x
     const int xl = x - w.cellOff;
     for (int q = 0; q < nc; ++q) {
         const int p = w.cellOff + c.B.btgt[q * cap + x];
+        const uint32_t vp = c.valid[p];
+        if (vp == 0u) continue;  // the partner wrote no angle force this pass (2 of 3 steps)
         const int np = flags_num_conn(c.B.flags[p]);
         int b = -1;
         for (int kk = 0; kk < np; ++kk)
@@ -139,7 +141,6 @@ __device__ __forceinline__ f2 gather_conn(const Ctx& c, const DevWorld& w, int x
                 break;
             }
         if (b < 0) continue;
-        const uint32_t vp = c.valid[p];
         if (vp & (1u << b)) acc = add(acc, to_f2(c.out1[b * cap + p]));
         const int b2 = (b + 1) % np;
         if (vp & (1u << (6 + b2))) acc = add(acc, to_f2(c.out2[b2 * cap + p]));

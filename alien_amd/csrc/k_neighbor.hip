@@ -296,7 +296,10 @@ __device__ __forceinline__ void cell_finish(const Ctx& c, const DevWorld& w, con
         c.B.brest[b * cap + k] = c.A.brest[b * cap + i];
         c.B.bang[b * cap + k] = c.A.bang[b * cap + i];
     }
-    if (nci > 0) c.newidx[i] = k;
+    if (nci > 0) {
+        c.newidx[i] = k;
+        c.bstiff[k] = c.stiff[w.cellOff + slot];
+    }
     f2 v = vi;
     if (!bari) {  // applyForces
         v = add(v, F);
@@ -442,12 +445,6 @@ __device__ __forceinline__ int strip_bin(const StripGeom& g, int yy, int hx) {
     return g.binOff + wrap_small(yy, g.nby) * g.nbx + wrap_small(g.x0 - 1 + hx, g.nbx);
 }
 
-__device__ __forceinline__ int wave_max(int v) {
-#pragma unroll
-    for (int d = 32; d >= 1; d >>= 1) v = max(v, __shfl_xor(v, d, kRowThreads));
-    return v;
-}
-
 // A: bin counters of row yy
 __device__ __forceinline__ void row_issue_counts(const Ctx& c, const StripGeom& g, int yy, RowFlight& F) {
     const int lane = threadIdx.x;
@@ -469,9 +466,9 @@ __device__ __forceinline__ void row_issue_entries(const Ctx& c, const StripGeom&
     const int n0 = lane < g.HX ? int(F.s1[0] - F.s0[0]) : 0;
     const int n1 = lane + kRowThreads < g.HX ? int(F.s1[1] - F.s0[1]) : 0;
     const int inc0 = int(wave_inclusive_scan(uint32_t(n0)));
-    const int tot0 = __shfl(inc0, kRowThreads - 1);
+    const int tot0 = wave_last(inc0);
     const int inc1 = int(wave_inclusive_scan(uint32_t(n1)));
-    F.total = tot0 + __shfl(inc1, kRowThreads - 1);
+    F.total = tot0 + wave_last(inc1);
 #pragma unroll
     for (int q = 0; q < 2; ++q) {
         const int hx = lane + q * kRowThreads;
@@ -611,6 +608,7 @@ __device__ __forceinline__ void row_compute(const Ctx& c, const DevWorld& w, con
                 if (kFluid && !keepSelf) selfIdx = __builtin_popcountll(mask & ((1ull << selfFlat) - 1ull));
                 if (hasBonds) {
                     // which of the neighbours are bonded
+                    const bool bari = flags_barrier(self.flags);
                     unsigned long long m = mask;
                     while (m) {
                         const int ci = __builtin_ctzll(m);
@@ -618,14 +616,26 @@ __device__ __forceinline__ void row_compute(const Ctx& c, const DevWorld& w, con
                         const int r = ci < ns[0] ? 0 : (ci < ns[0] + ns[1] ? 1 : 2);
                         const int rest = ci - (r == 0 ? 0 : (r == 1 ? ns[0] : ns[0] + ns[1]));
                         const int np = (a0s[r == 0 ? 0 : (r == 1 ? 1 : 2)] + rest) & kPoolMask;
-                        if (is_bonded(bt, S.aidx[np] - w.cellOff)) bmask |= 1ull << ci;
+                        if (!is_bonded(bt, S.aidx[np] - w.cellOff)) continue;
+                        if constexpr (!kFluid) {
+                            // a bonded collision pair adds only its overlap push (closer than
+                            // minDistance, not for a barrier cell): otherwise it adds nothing and
+                            // leaves the list (the distance as the pair term takes it)
+                            const f2 o = to_f2(S.pos[np]);
+                            const float d2 = lensq(g.seam ? torus_delta(self.pos, o, Wf, Hf) : sub(self.pos, o));
+                            if (bari || d2 > P.overlapBelowSq) {
+                                mask &= ~(1ull << ci);
+                                continue;
+                            }
+                        }
+                        bmask |= 1ull << ci;
                     }
                 }
             }
         }
         const int count = __builtin_popcountll(mask);
         const int inc = int(wave_inclusive_scan(uint32_t(count)));
-        const int numPairs = __shfl(inc, kRowThreads - 1);
+        const int numPairs = wave_last(inc);
         const int pbase = inc - count;
 
         Acc a;
@@ -674,8 +684,7 @@ __device__ __forceinline__ void row_compute(const Ctx& c, const DevWorld& w, con
                 // lane that evaluated it (every lane takes part in every exchange)
                 const int lo = max(pbase, off), hi = min(pbase + count, min(off + kRowThreads, wend));
                 const int n = count > 0 ? max(0, hi - lo) : 0;
-                const int nmax = wave_max(n);
-                for (int j = 0; j < nmax; ++j) {
+                for (int j = 0; __any(j < n); ++j) {
                     const int src = (lo + j - off) & (kRowThreads - 1);
                     float tt[kPairTerms];
 #pragma unroll

@@ -77,15 +77,23 @@ __device__ __forceinline__ void pbin_for_next_step(const Ctx& c, const DevWorld&
     atomicAdd(&hist[gb], 1u);
 }
 
-__device__ __forceinline__ uint32_t wave_inclusive_scan(uint32_t v) {
-    const int lane = threadIdx.x & 63;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        uint32_t t = __shfl_up(v, d, 64);
-        if (lane >= d) v += t;
-    }
+// Inclusive sum over the 64 lanes of a wave (all lanes active), with DPP row shifts and row
+// broadcasts instead of LDS permutes: sums of up to four lanes, of eight, of the 16-lane row, then
+// row 0's total into row 1 and row 2's into row 3, then lane 31's prefix into rows 2 and 3.
+__device__ __forceinline__ uint32_t wave_inclusive_scan(uint32_t x) {
+    uint32_t v = x;
+    v += uint32_t(__builtin_amdgcn_update_dpp(0, int(x), 0x111, 0xf, 0xf, true));   // row_shr:1
+    v += uint32_t(__builtin_amdgcn_update_dpp(0, int(x), 0x112, 0xf, 0xf, true));   // row_shr:2
+    v += uint32_t(__builtin_amdgcn_update_dpp(0, int(x), 0x113, 0xf, 0xf, true));   // row_shr:3
+    v += uint32_t(__builtin_amdgcn_update_dpp(0, int(v), 0x114, 0xf, 0xe, false));  // row_shr:4, lanes 4-15
+    v += uint32_t(__builtin_amdgcn_update_dpp(0, int(v), 0x118, 0xf, 0xc, false));  // row_shr:8, lanes 8-15
+    v += uint32_t(__builtin_amdgcn_update_dpp(0, int(v), 0x142, 0xa, 0xf, false));  // row_bcast:15, rows 1, 3
+    v += uint32_t(__builtin_amdgcn_update_dpp(0, int(v), 0x143, 0xc, 0xf, false));  // row_bcast:31, rows 2, 3
     return v;
 }
+
+// the value of lane 63 (a wave total after an inclusive scan), read as a scalar
+__device__ __forceinline__ int wave_last(int v) { return __builtin_amdgcn_readlane(v, 63); }
 
 // exclusive scan across the block (blockDim multiple of 64, at most 1024); block total in `total`
 __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t& total) {

@@ -93,6 +93,7 @@ DevParams to_dev_params(const alien_params& p) {
     d.maxVelSq = sqrt_threshold(d.maxVelocity);
     d.fastSq = sqrt_threshold(0.5f);
     d.fusionBelowSq = sqrt_below(d.fusionVelocity);
+    d.overlapBelowSq = sqrt_below(d.minDistance);
     d.invH = 1.0f / d.smoothing;
     d.invH2 = 1.0f / (d.smoothing * d.smoothing);
     d.invH3 = 1.0f / (d.smoothing * d.smoothing * d.smoothing);
@@ -377,7 +378,7 @@ void WorldSet::synchronize() { check_hip(hipStreamSynchronize(stream_), "hipStre
 
 uint64_t WorldSet::estimateBytes(uint64_t cells, uint64_t particles, int W, int H, int binSize) {
     const uint64_t perCell = 2 * (8 + 8 + 4 + 4 + 4 + kMaxBonds * 12) + 8 + 4 * 4 + 8 + 3 * 8 + 2 * kMaxBonds * 8 + 4 +
-                             2 + 8 + 4 * 16 + 4 + (8 + 8 + 4 + 4) + 4;
+                             2 + 8 + 4 * 16 + 4 + (8 + 8 + 4 + 4) + 4 + 4;
     const uint64_t perParticle = 2 * (8 + 8 + 4 + 4 + 4 + 1) + 6 * 4;
     const uint64_t bins = uint64_t(std::max(1, W / binSize)) * uint64_t(std::max(1, H / binSize)) +
                           uint64_t(std::max(1, W / kParticleBin)) * uint64_t(std::max(1, H / kParticleBin));
@@ -536,6 +537,7 @@ void WorldSet::build(const std::vector<Source>& sources) {
     staged_ = static_cast<int2*>(A(sizeof(int2) * NC));
     own_ = static_cast<float2*>(A(sizeof(float2) * NC));
     shift_ = static_cast<float2*>(A(sizeof(float2) * NC));
+    bstiff_ = static_cast<float*>(A(sizeof(float) * NC));
     s2_ = static_cast<float2*>(A(sizeof(float2) * NC));
     out1_ = static_cast<float2*>(A(sizeof(float2) * kMaxBonds * size_t(NC)));
     out2_ = static_cast<float2*>(A(sizeof(float2) * kMaxBonds * size_t(NC)));
@@ -878,6 +880,7 @@ Ctx WorldSet::makeCtx(int parity) const {
     c.staged = staged_;
     c.own = own_;
     c.shift = shift_;
+    c.bstiff = bstiff_;
     c.s2 = s2_;
     c.out1 = out1_;
     c.out2 = out2_;

@@ -178,6 +178,7 @@ private:
     uint32_t* mflags_ = nullptr;
     int2* staged_ = nullptr;
     float2 *own_ = nullptr, *shift_ = nullptr, *s2_ = nullptr, *out1_ = nullptr, *out2_ = nullptr;
+    float* bstiff_ = nullptr;
     uint32_t* valid_ = nullptr;
     uint8_t *delFlag_ = nullptr, *mark_ = nullptr;
     unsigned long long* claim_ = nullptr;

@@ -89,6 +89,7 @@ def lib():
         L.alien_get_path_counts.argtypes = [vp, pu64, pu64]
         L.alien_batch_set_kernel_timing.argtypes = [vp, i32]
         L.alien_batch_get_kernel_timing.argtypes = [vp, ctypes.POINTER(KernelTime), i32, ctypes.POINTER(i32)]
+        L.alien_batch_get_path_counts.argtypes = [vp, pu64, pu64]
         L.alien_server_create.argtypes = [u64, ctypes.c_uint32, ctypes.POINTER(vp)]
         L.alien_server_destroy.argtypes = [vp]
         L.alien_server_submit.argtypes = [vp, ctypes.POINTER(WorldDesc), ctypes.POINTER(Params), vp, u64, vp, u64, u64,
@@ -330,6 +331,12 @@ class Batch:
 
     def kernel_times(self):
         return _kernel_times(lib().alien_batch_get_kernel_timing, self._h)
+
+    def path_counts(self):
+        """(chunks on the global-memory path, cells on the single-cell path) since the last rebuild."""
+        a, b = ctypes.c_uint64(), ctypes.c_uint64()
+        check(lib().alien_batch_get_path_counts(self._h, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
 
 
 class Server:

@@ -96,6 +96,10 @@ def cpu_baseline(world, seconds):
     }
 
 
+def progress(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
 def gen_processes():
     try:
         n = len(os.sched_getaffinity(0))
@@ -114,6 +118,7 @@ def bench_batch(args):
     ws = worlds.batched_worlds(count=args.batch_worlds, steps=args.steps, processes=gen_processes())
     gen_s = time.perf_counter() - t0
     cells = sum(len(w.cells) for w in ws)
+    progress(f"generated {len(ws)} worlds, {cells} cells in {gen_s:.1f} s")
     b = engine.Batch()
     for w in ws:
         b.add(w)
@@ -123,6 +128,7 @@ def bench_batch(args):
     b.step(args.steps)
     batched_s = time.perf_counter() - t0
     b.close()
+    progress(f"batched: {args.steps} steps in {batched_s:.2f} s (autotune block {tune[0]}, strip cap {tune[1]})")
     # per-world sequential: each world alone, same steps, same library
     seq_s = 0.0
     for w in ws:
@@ -132,6 +138,8 @@ def bench_batch(args):
         sim.step(args.steps)
         seq_s += time.perf_counter() - t0
         sim.close()
+        if (ws.index(w) + 1) % 32 == 0:
+            progress(f"sequential: {ws.index(w) + 1} worlds, {seq_s:.2f} s of stepping")
     value = cells * args.steps / batched_s
     seq_value = cells * args.steps / seq_s
     print(json.dumps({
@@ -158,6 +166,7 @@ def bench_serving(args):
     t0 = time.perf_counter()
     jobs = worlds.serving_jobs(rate=args.serving_rate, duration=args.serving_seconds, processes=gen_processes())
     gen_s = time.perf_counter() - t0
+    progress(f"generated {len(jobs)} jobs in {gen_s:.1f} s")
     srv = engine.Server(budget_bytes=int(args.budget_gb * 2**30), chunk_steps=50)
     ids = []
     start = time.perf_counter()
@@ -166,8 +175,12 @@ def bench_serving(args):
         if delay > 0:
             time.sleep(delay)
         ids.append(srv.submit(w.size, w.params, w.cells, w.particles, w.steps))
-    for j in ids:
+        if len(ids) % 100 == 0:
+            progress(f"submitted {len(ids)} jobs")
+    for n, j in enumerate(ids):
         srv.wait(j, timeout=3600)
+        if (n + 1) % 100 == 0:
+            progress(f"{n + 1} jobs done")
     infos = [srv.poll(j) for j in ids]
     lat = sorted(i.finishTime - i.submitTime for i in infos)
     work = sum(i.numCells * i.stepsDone for i in infos)

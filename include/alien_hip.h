@@ -230,6 +230,8 @@ int alien_batch_autotune(alien_batch* b, uint32_t trialSteps, int32_t* blockSize
                          double* msPerStep);
 int alien_batch_set_kernel_timing(alien_batch* b, int32_t enable);
 int alien_batch_get_kernel_timing(alien_batch* b, alien_kernel_time* out, int32_t cap, int32_t* count);
+/* alien_get_path_counts for all worlds of a batch together (no counterpart in ALIEN). */
+int alien_batch_get_path_counts(alien_batch* b, uint64_t* fallbackChunks, uint64_t* directCells);
 
 /* ---- serving ------------------------------------------------------------------------------- */
 
