@@ -557,10 +557,14 @@ struct alien_server {
 
     uint64_t usedBytes() const {
         uint64_t s = 0;
-        for (auto id : running) s += jobs.at(id)->bytes;
+        for (auto id : running)
+            if (id) s += jobs.at(id)->bytes;
         return s;
     }
 
+    // `running` lists the jobs in world order of the resident set; 0 marks a finished world that
+    // stays in the set, inactive, until the set is next rebuilt (when jobs are admitted or
+    // cancelled), so a completion costs no copy of the other worlds.
     void loop() {
         int device = 0;
         (void)hipGetDevice(&device);
@@ -572,6 +576,7 @@ struct alien_server {
             std::vector<uint64_t> keep;
             bool changed = false;
             for (auto id : running) {
+                if (!id) continue;
                 if (jobs.at(id)->cancel) {
                     jobs.erase(id);
                     changed = true;
@@ -618,6 +623,8 @@ struct alien_server {
             std::string err;
             try {
                 if (changed) {
+                    // rebuild: the live worlds (device copies) and the admitted ones, finished
+                    // worlds left out
                     auto fresh = std::make_unique<WorldSet>();
                     std::vector<WorldSet::Source> src;
                     for (size_t i = 0; i < members.size(); ++i) {
@@ -646,9 +653,10 @@ struct alien_server {
                 j.startTime = t;
                 j.image.reset();
             }
-            running = members;
+            running = changed ? members : oldRunning;
             if (!err.empty()) {
                 for (auto id : running) {
+                    if (!id) continue;
                     Job& j = *jobs.at(id);
                     j.state = ALIEN_JOB_FAILED;
                     j.error = err;
@@ -659,94 +667,60 @@ struct alien_server {
                 doneCv.notify_all();
                 continue;
             }
-            // finished jobs: snapshot and leave the resident set
-            std::vector<uint64_t> still;
+            // finished jobs: snapshot, then the world goes inactive in place
             std::vector<std::pair<uint64_t, int>> finished;
             for (size_t i = 0; i < running.size(); ++i) {
+                if (!running[i]) continue;
                 Job& j = *jobs.at(running[i]);
                 j.stepsDone += chunkSteps;
-                if (j.stepsDone >= j.stepsRequested)
-                    finished.push_back({running[i], int(i)});
-                else
-                    still.push_back(running[i]);
+                if (j.stepsDone >= j.stepsRequested) finished.push_back({running[i], int(i)});
             }
-            if (!finished.empty()) {
-                lk.unlock();
-                std::vector<std::pair<uint64_t, std::pair<std::vector<alien_cell_to>, std::vector<alien_particle_to>>>>
-                    snaps;
-                std::string snapErr;
-                try {
-                    for (auto& f : finished) {
-                        WorldImage img;
-                        set->extract(f.second, img);
-                        std::vector<alien_cell_to> c;
-                        std::vector<alien_particle_to> p;
-                        img.toTO(c, p);
-                        snaps.push_back({f.first, {std::move(c), std::move(p)}});
-                    }
-                } catch (const std::exception& e) {
-                    snapErr = e.what();
+            if (finished.empty()) continue;
+            lk.unlock();
+            std::vector<std::pair<uint64_t, std::pair<std::vector<alien_cell_to>, std::vector<alien_particle_to>>>>
+                snaps;
+            std::string snapErr;
+            try {
+                for (auto& f : finished) {
+                    WorldImage img;
+                    set->extract(f.second, img);
+                    std::vector<alien_cell_to> c;
+                    std::vector<alien_particle_to> p;
+                    img.toTO(c, p);
+                    snaps.push_back({f.first, {std::move(c), std::move(p)}});
+                    set->setActive(f.second, false);
                 }
-                lk.lock();
-                for (auto& s : snaps) {
-                    Job& j = *jobs.at(s.first);
-                    j.snapCells = std::move(s.second.first);
-                    j.snapParticles = std::move(s.second.second);
-                    j.numCells = j.snapCells.size();
-                    j.numParticles = j.snapParticles.size();
-                    j.state = ALIEN_JOB_DONE;
+            } catch (const std::exception& e) {
+                snapErr = e.what();
+            }
+            lk.lock();
+            for (auto& sn : snaps) {
+                Job& j = *jobs.at(sn.first);
+                j.snapCells = std::move(sn.second.first);
+                j.snapParticles = std::move(sn.second.second);
+                j.numCells = j.snapCells.size();
+                j.numParticles = j.snapParticles.size();
+                j.state = ALIEN_JOB_DONE;
+                j.finishTime = nowSeconds();
+            }
+            for (auto& f : finished) {
+                Job& j = *jobs.at(f.first);
+                if (j.state != ALIEN_JOB_DONE) {
+                    j.state = ALIEN_JOB_FAILED;
+                    j.error = snapErr;
                     j.finishTime = nowSeconds();
                 }
-                if (!snapErr.empty())
-                    for (auto& f : finished) {
-                        Job& j = *jobs.at(f.first);
-                        if (j.state != ALIEN_JOB_DONE) {
-                            j.state = ALIEN_JOB_FAILED;
-                            j.error = snapErr;
-                        }
-                    }
-                // keep the set as is; the next iteration rebuilds it without the finished worlds
-                std::vector<uint64_t> order = running;
-                running = order;
-                // mark membership change by removing finished ids (indices rebuilt next round)
-                std::vector<uint64_t> kept;
-                for (auto id : order)
-                    if (std::find_if(finished.begin(), finished.end(),
-                                     [&](const std::pair<uint64_t, int>& f) { return f.first == id; }) == finished.end())
-                        kept.push_back(id);
-                // rebuild now so that world indices of the set match `running`
-                lk.unlock();
-                try {
-                    if (kept.empty()) {
-                        set.reset();
-                    } else {
-                        auto fresh = std::make_unique<WorldSet>();
-                        std::vector<WorldSet::Source> src;
-                        for (auto id : kept) {
-                            WorldSet::Source s;
-                            s.from = set.get();
-                            s.fromIndex = int(std::find(order.begin(), order.end(), id) - order.begin());
-                            src.push_back(s);
-                        }
-                        fresh->build(src);
-                        set = std::move(fresh);
-                    }
-                } catch (const std::exception& e) {
-                    err = e.what();
-                }
-                lk.lock();
-                running = kept;
-                if (!err.empty()) {
-                    for (auto id : running) {
-                        Job& j = *jobs.at(id);
-                        j.state = ALIEN_JOB_FAILED;
-                        j.error = err;
-                    }
-                    running.clear();
-                    set.reset();
-                }
-                doneCv.notify_all();
+                running[size_t(f.second)] = 0;
             }
+            bool anyLive = false;
+            for (auto id : running) anyLive = anyLive || id != 0;
+            if (!anyLive) {
+                running.clear();
+                lk.unlock();
+                set.reset();
+                lk.lock();
+            }
+            doneCv.notify_all();
         }
     }
 };

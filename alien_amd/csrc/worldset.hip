@@ -621,6 +621,7 @@ void WorldSet::build(const std::vector<Source>& sources) {
     densityPending_ = std::move(pending);
     stagedHint_ = std::move(stagedHint);
     bondWork_ = std::move(bondWork);
+    active_.assign(worlds_.size(), 1);
     stripChoice_ = std::move(stripChoice);
     chunk_shape(worlds_, params_);
     shape_.hasBondWork = false;
@@ -752,6 +753,7 @@ void WorldSet::buildTables() {
     std::vector<int2> cells, bins, pbins, parts;
     const int ct = shape_.cellThreads, pt = shape_.partThreads, bt = shape_.binThreads;
     for (int wi = 0; wi < int(worlds_.size()); ++wi) {
+        if (!active_[wi]) continue;
         const DevWorld& w = worlds_[wi];
         for (int s = 0; s < w.numCells; s += ct) cells.push_back(make_int2(wi, w.cellOff + s));
         for (int s = 0; s < w.partCap; s += pt) parts.push_back(make_int2(wi, w.partOff + s));
@@ -771,7 +773,7 @@ void WorldSet::buildTables() {
     std::vector<int2> tiles;
     for (int wi = 0; wi < int(worlds_.size()); ++wi) {
         const DevWorld& w = worlds_[wi];
-        if (w.numCells == 0) continue;
+        if (w.numCells == 0 || !active_[wi]) continue;
         for (int t = 0; t < w.ntx * w.nty; ++t) tiles.push_back(make_int2(wi, t));
     }
     std::vector<int2> tileTable;
@@ -807,6 +809,16 @@ void WorldSet::setBlockSize(int threads) {
     shape_.partThreads = threads;
     shape_.binThreads = threads;
     if (!worlds_.empty()) buildTables();
+}
+
+// The kernels reach a world only through the launch tables, and every offset they derive from the
+// shared scan is relative to the world's own first bin, so the frozen counters of an inactive
+// world change nothing for the others.
+void WorldSet::setActive(int w, bool on) {
+    if (active_.at(w) == uint8_t(on)) return;
+    active_[w] = on;
+    buildTables();
+    invalidateGraphs();
 }
 
 void WorldSet::setStripCap(int cap) {

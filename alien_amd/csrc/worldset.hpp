@@ -117,6 +117,9 @@ public:
     int blockSize() const { return shape_.cellThreads; }
     // neighbour-pass strips no wider than cap bins (8, 16, 32 or 64); launch shape only
     void setStripCap(int cap);
+    // An inactive world stays in place but is left out of every launch (its state is frozen);
+    // all worlds are active after build.
+    void setActive(int w, bool on);
     int stripCap() const { return stripCap_; }
     void setUseGraph(bool on);
     // kernel timing mode: no graphs, every launch bracketed by events (see alien_set_kernel_timing)
@@ -189,6 +192,7 @@ private:
     int numPartBlocks_ = 0;
     std::vector<int> stagedHint_;    // per world: bonded + barrier cells, sizes the staged-list grids
     std::vector<uint8_t> bondWork_;  // per world: a bonded or barrier cell exists or can appear by fusion
+    std::vector<uint8_t> active_;    // per world: included in the launch tables
     std::vector<int> stripChoice_;   // per world: strip width chosen from its cell density
     int stripCap_ = kStripMax;
     void adaptGrids();

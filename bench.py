@@ -32,7 +32,8 @@ BYTES_PER_PARTICLE = 52
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--steps", type=int, default=None,
+                    help="timed steps (default 1000 for the large fluid world, 500 per call for the batch)")
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--cells", type=int, default=4_000_000)
     ap.add_argument("--particles", type=int, default=500_000)
@@ -49,7 +50,10 @@ def parse():
     ap.add_argument("--budget-gb", type=float, default=200.0)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain device copy of known size (calibrates the HBM byte counters)")
-    return ap.parse_args()
+    args = ap.parse_args()
+    if args.steps is None:
+        args.steps = 500 if args.workload == "batch" else 1000
+    return args
 
 
 def copy_calibration():
@@ -161,7 +165,7 @@ def bench_serving(args):
     """Poisson job arrivals (rate, window) of batched-workload worlds with log-uniform step
     counts, admitted by an HBM budget into the continuous-batching server; latency = submit to
     snapshot ready."""
-    from alien_amd import engine, worlds
+    from alien_amd import abi, engine, worlds
 
     t0 = time.perf_counter()
     jobs = worlds.serving_jobs(rate=args.serving_rate, duration=args.serving_seconds, processes=gen_processes())
@@ -177,10 +181,21 @@ def bench_serving(args):
         ids.append(srv.submit(w.size, w.params, w.cells, w.particles, w.steps))
         if len(ids) % 100 == 0:
             progress(f"submitted {len(ids)} jobs")
-    for n, j in enumerate(ids):
-        srv.wait(j, timeout=3600)
-        if (n + 1) % 100 == 0:
-            progress(f"{n + 1} jobs done")
+    pending, last, t_wait = set(ids), time.perf_counter(), time.perf_counter()
+    while pending:
+        time.sleep(1.0)
+        for j in list(pending):
+            st = srv.poll(j).state
+            if st == abi.JOB_FAILED:
+                raise RuntimeError(f"serving job {j} failed")
+            if st == abi.JOB_DONE:
+                pending.discard(j)
+        now = time.perf_counter()
+        if now - last >= 20.0:
+            progress(f"{len(ids) - len(pending)} of {len(ids)} jobs done")
+            last = now
+        if now - t_wait > 3600.0:
+            raise RuntimeError("serving jobs did not finish within an hour")
     infos = [srv.poll(j) for j in ids]
     lat = sorted(i.finishTime - i.submitTime for i in infos)
     work = sum(i.numCells * i.stepsDone for i in infos)
