@@ -239,7 +239,9 @@ def main():
     world = worlds.large_fluid(size=(args.size, args.size), num_cells=args.cells, num_particles=args.particles,
                                steps=args.steps)
     n = len(world.cells)
-    sim = engine.Simulation.from_world(world)
+    t0 = time.perf_counter()
+    sim = engine.Simulation.from_world(world)  # host arrays -> device (set_data)
+    set_s = time.perf_counter() - t0
     device = engine.gpu_name()
     sim.step(args.warmup)
 
@@ -250,6 +252,9 @@ def main():
     ms_per_step = 1000.0 * elapsed / args.steps
     value = n * args.steps / elapsed
     live_particles = sim.counts()[1]
+    t0 = time.perf_counter()
+    sim.get_data()  # device -> host arrays
+    get_s = time.perf_counter() - t0
 
     roofline = None
     kernels = {}
@@ -293,6 +298,9 @@ def main():
                    "device": device},
         "roofline": roofline,
         "step_bandwidth_GBs": step_bytes / (elapsed / args.steps) / 1e9,
+        # the host boundary: one set_data before and one get_data after the run (not per step)
+        "host_boundary": {"set_data_s": set_s, "get_data_s": get_s,
+                          "value_with_set_and_get": n * args.steps / (elapsed + set_s + get_s)},
         "live_particles_after": live_particles,
         "kernels": kernels,
     }
