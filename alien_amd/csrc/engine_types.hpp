@@ -74,8 +74,9 @@ struct DevWorld {
     int binOff;             // first cell bin in the combined bin arrays
     int pbinOff;            // first particle bin in the combined bin arrays
     int partOff, partCap;   // particle arrays
-    int tileX, tileY;       // neighbour pass: bins per strip, bin rows per chunk (launch shape only)
+    int tileX, tileY;       // neighbour pass: bins per strip, staged rows per chunk (launch shape only)
     int ntx, nty;           // strips, chunks
+    int rowsPerStage;       // bin rows per staged row of the neighbour pass (1, or 2 for sparse rows)
 };
 
 // per-world device counters

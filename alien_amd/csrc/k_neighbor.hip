@@ -346,29 +346,36 @@ __device__ __forceinline__ void cell_direct(const Ctx& c, const DevWorld& w, con
 
 // ---------------------------------------------------------------------------------------------
 // Row streaming.  A workgroup (one wave) owns a strip of up to kStripMax bin columns and a chunk of
-// bin rows, and walks the rows top to bottom with rows y-1, y, y+1 staged in LDS.  While row y is
-// computed, the entries of row y+2 and the bin counters of row y+3 are on their way into
-// registers; row y+2 is committed once row y-1 is no longer needed.  A staged row covers the
-// strip plus one halo column on each side, its entries stored bin by bin in rank order (R1), so
-// the three stencil bins of a cell in one row are one contiguous range.  The three rows share one
-// ring pool of entries (positions counted up without wrapping, stored modulo kPool): a row takes
-// the pool space right after the previous one, so any three consecutive rows of up to kPool
-// entries together fit, however they are distributed.
+// staged rows, and walks them top to bottom with staged rows Y-1, Y, Y+1 in LDS.  A staged row is
+// kSub consecutive bin rows (1, or 2 for sparse worlds, so that one pass has enough cells for the
+// 64 lanes); each bin row covers the strip plus one halo column on each side, its entries stored
+// bin by bin in rank order (R1), so the three stencil bins of a cell in one bin row are one
+// contiguous range.  While row Y is computed, the entries of row Y+2 and the bin counters of row
+// Y+3 are on their way into registers; row Y+2 is committed once row Y-1 is no longer needed.  The
+// three staged rows share one ring pool of entries (positions counted up without wrapping, stored
+// modulo kPool): a row takes the pool space right after the previous one, so any three
+// consecutive rows of up to kPool entries together fit, however they are distributed.
 // ---------------------------------------------------------------------------------------------
-constexpr int kRowBins = kStripMax + 2;
+constexpr int kRowBins = kStripMax + 2;            // halo columns of one bin row
+constexpr int kSubMax = 2;                         // bin rows per staged row, at most
 constexpr int kPool = kRowPool;
 constexpr int kPoolMask = kPool - 1;
 constexpr int kPairCap = 128;  // pair records per evaluation window
 constexpr int kPosMirror = 64;
 static_assert((kPool & kPoolMask) == 0 && kPool <= 256, "pair records keep an 8-bit pool position");
 static_assert(kRowCap <= 2 * kRowThreads, "a row in flight holds at most two entries per lane");
+static_assert(kSubMax * kRowBins <= 3 * kRowThreads && kSubMax * kRowBins < 256,
+              "bin slots: three per lane, 8-bit ids");
 
+// LDS of one wave, for staged rows of up to kMaxSub bin rows
+template <int kMaxSub>
 struct RowShared {
-    // per ring row: pool position of the first entry of every halo-column bin, then the end; map
-    // positions of the first entries of the left halo bin, the first interior bin and the right
-    // halo bin (the interior bins of a row are contiguous in the map)
-    int rs[3][kRowBins + 1];
-    int rm[3][3];
+    static constexpr int kStageBins = kMaxSub * kRowBins;  // bin slots of one staged row (row major)
+    // per ring row: pool position of the first entry of every bin slot, then the end; per ring
+    // row and bin row, map positions of the first entries of the left halo bin, the first interior
+    // bin and the right halo bin (the interior bins of a bin row are contiguous in the map)
+    int rs[3][kStageBins + 1];
+    int rm[3][kMaxSub][3];
     // entries of the staged rows, by pool position modulo kPool; positions 0..kPosMirror-1 are
     // repeated after the end so that a stencil range of up to 64 entries reads without wrapping
     float2 pos[kPool + kPosMirror];
@@ -376,16 +383,16 @@ struct RowShared {
     float dens[kPool];  // fluid: 1 / density; collision: density
     uint32_t flags[kPool];
     int aidx[kPool];
-    uint8_t sbin[kPool];  // halo column
+    uint8_t sbin[kPool];  // bin slot: bin row * HX + halo column
     // the row in flight
-    int nrs[kRowBins + 1];
-    int nrm[3];
-    uint8_t nebin[kRowCap];  // arrival index -> halo column
+    int nrs[kStageBins + 1];
+    int nrm[kMaxSub][3];
+    uint8_t nebin[kRowCap];  // arrival index -> bin slot
     union {
         int naidx[kRowCap];  // while a row is committed: arrival index -> previous index
         struct {
             // while a row is computed, the pair records of the current window: the cell's pool
-            // position, and the neighbour as bonded << 15 | stencil row << 8 | pool position
+            // position, and the neighbour as bonded << 15 | ring row << 8 | pool position
             uint8_t pself[kPairCap];
             uint16_t pnb[kPairCap];
         } pr;
@@ -393,10 +400,12 @@ struct RowShared {
     uint32_t hist[kStripMax * kChunkMax / 2];  // next-step cells of the wave's own bins (16-bit pairs)
 };
 
-// registers of a row in flight: bin counters of the halo columns lane and lane + 64, then the
-// entries lane and lane + 64 in arrival order
+// registers of a row in flight: bin counters of the bin slots lane, lane + 64, lane + 128, then
+// the entries lane and lane + 64 in arrival order
+template <int kSub>
 struct RowFlight {
-    uint32_t s0[2], s1[2];
+    static constexpr int kSlots = kSub == 2 ? 3 : 2;  // bin slots per lane
+    uint32_t s0[kSlots], s1[kSlots];
     int aidx[2];
     float2 pos[2], vel[2];
     float dens[2];
@@ -416,16 +425,23 @@ struct StripGeom {
 __device__ __forceinline__ int seg_of(int hx, int HX) { return hx == 0 ? 0 : (hx == HX - 1 ? 2 : 1); }
 __device__ __forceinline__ int seg_first(int seg, int HX) { return seg == 0 ? 0 : (seg == 2 ? HX - 1 : 1); }
 
+// bin row (0 or 1) of bin slot j
+template <int kSub>
+__device__ __forceinline__ int slot_row(int j, int HX) { return kSub == 2 && j >= HX ? 1 : 0; }
+
 // the staged entry at pool position pp (already reduced modulo kPool) of ring row r
-__device__ __forceinline__ Nb row_nb(const RowShared& S, const StripGeom& g, int r, int pp) {
+template <int kSub, class RS>
+__device__ __forceinline__ Nb row_nb(const RS& S, const StripGeom& g, int r, int pp) {
     Nb o;
     o.pos = to_f2(S.pos[pp]);
     o.vel = to_f2(S.vel[pp]);
     o.dens = S.dens[pp];
     o.flags = S.flags[pp];
     o.aidx = S.aidx[pp];
-    const int seg = seg_of(S.sbin[pp], g.HX);
-    o.lidx = S.rm[r][seg] + ((pp - S.rs[r][seg_first(seg, g.HX)]) & kPoolMask);
+    const int j = S.sbin[pp];
+    const int t = slot_row<kSub>(j, g.HX);
+    const int seg = seg_of(j - t * g.HX, g.HX);
+    o.lidx = S.rm[r][t][seg] + ((pp - S.rs[r][t * g.HX + seg_first(seg, g.HX)]) & kPoolMask);
     return o;
 }
 
@@ -441,19 +457,22 @@ __device__ __forceinline__ Nb global_nb(const Ctx& c, int a, int l) {
     return o;
 }
 
+// bin of halo column hx in bin row yy (both wrapped around the torus)
 __device__ __forceinline__ int strip_bin(const StripGeom& g, int yy, int hx) {
     return g.binOff + wrap_small(yy, g.nby) * g.nbx + wrap_small(g.x0 - 1 + hx, g.nbx);
 }
 
-// A: bin counters of row yy
-__device__ __forceinline__ void row_issue_counts(const Ctx& c, const StripGeom& g, int yy, RowFlight& F) {
+// A: bin counters of staged row Y
+template <int kSub>
+__device__ __forceinline__ void row_issue_counts(const Ctx& c, const StripGeom& g, int Y, RowFlight<kSub>& F) {
     const int lane = threadIdx.x;
 #pragma unroll
-    for (int q = 0; q < 2; ++q) {
-        const int hx = lane + q * kRowThreads;
+    for (int q = 0; q < RowFlight<kSub>::kSlots; ++q) {
+        const int j = lane + q * kRowThreads;
         F.s0[q] = F.s1[q] = 0u;
-        if (hx < g.HX) {
-            const int gb = strip_bin(g, yy, hx);
+        if (j < kSub * g.HX) {
+            const int t = slot_row<kSub>(j, g.HX);
+            const int gb = strip_bin(g, kSub * Y + t, j - t * g.HX);
             F.s0[q] = c.start[gb];
             F.s1[q] = c.start[gb + 1];
         }
@@ -461,32 +480,40 @@ __device__ __forceinline__ void row_issue_counts(const Ctx& c, const StripGeom& 
 }
 
 // B: slots of the row's bins, then the entry loads (map-order copy, contiguous per bin)
-__device__ __forceinline__ void row_issue_entries(const Ctx& c, const StripGeom& g, RowShared& S, RowFlight& F) {
+template <int kSub, class RS>
+__device__ __forceinline__ void row_issue_entries(const Ctx& c, const StripGeom& g, RS& S, RowFlight<kSub>& F) {
+    constexpr int kSlots = RowFlight<kSub>::kSlots;
     const int lane = threadIdx.x;
-    const int n0 = lane < g.HX ? int(F.s1[0] - F.s0[0]) : 0;
-    const int n1 = lane + kRowThreads < g.HX ? int(F.s1[1] - F.s0[1]) : 0;
-    const int inc0 = int(wave_inclusive_scan(uint32_t(n0)));
-    const int tot0 = wave_last(inc0);
-    const int inc1 = int(wave_inclusive_scan(uint32_t(n1)));
-    F.total = tot0 + wave_last(inc1);
+    const int SB = kSub * g.HX;
+    int n[kSlots], inc[kSlots];
 #pragma unroll
-    for (int q = 0; q < 2; ++q) {
-        const int hx = lane + q * kRowThreads;
-        if (hx >= g.HX) continue;
-        const int off = q == 0 ? inc0 - n0 : tot0 + inc1 - n1;
-        const int n = q == 0 ? n0 : n1;
-        S.nrs[hx] = off;
-        if (hx <= 1 || hx == g.HX - 1) S.nrm[seg_of(hx, g.HX)] = g.cellOff + int(F.s0[q] - g.base);
-        for (int t = 0; t < n; ++t) S.nebin[off + t] = uint8_t(hx);
+    for (int q = 0; q < kSlots; ++q) {
+        n[q] = lane + q * kRowThreads < SB ? int(F.s1[q] - F.s0[q]) : 0;
+        inc[q] = int(wave_inclusive_scan(uint32_t(n[q])));
     }
-    if (lane == 0) S.nrs[g.HX] = F.total;
+    const int tot0 = wave_last(inc[0]), tot1 = wave_last(inc[1]);
+    F.total = tot0 + tot1 + (kSlots > 2 ? wave_last(inc[kSlots - 1]) : 0);
+#pragma unroll
+    for (int q = 0; q < kSlots; ++q) {
+        const int j = lane + q * kRowThreads;
+        if (j >= SB) continue;
+        const int off = inc[q] - n[q] + (q >= 1 ? tot0 : 0) + (q >= 2 ? tot1 : 0);
+        const int t = slot_row<kSub>(j, g.HX);
+        const int hx = j - t * g.HX;
+        S.nrs[j] = off;
+        if (hx <= 1 || hx == g.HX - 1) S.nrm[t][seg_of(hx, g.HX)] = g.cellOff + int(F.s0[q] - g.base);
+        for (int e = 0; e < n[q]; ++e) S.nebin[off + e] = uint8_t(j);
+    }
+    if (lane == 0) S.nrs[SB] = F.total;
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < 2; ++q) {
         const int e = lane + q * kRowThreads;
         if (e < F.total) {
-            const int seg = seg_of(S.nebin[e], g.HX);
-            const int m = S.nrm[seg] + (e - S.nrs[seg_first(seg, g.HX)]);
+            const int j = S.nebin[e];
+            const int t = slot_row<kSub>(j, g.HX);
+            const int seg = seg_of(j - t * g.HX, g.HX);
+            const int m = S.nrm[t][seg] + (e - S.nrs[t * g.HX + seg_first(seg, g.HX)]);
             F.aidx[q] = c.sorted[m];
             F.pos[q] = c.mpos[m];
             F.vel[q] = c.mvel[m];
@@ -498,11 +525,12 @@ __device__ __forceinline__ void row_issue_entries(const Ctx& c, const StripGeom&
 
 // C: the row into ring row r at pool position head, every entry at its rank by previous index
 // inside its bin (R1)
-template <bool kFluid>
-__device__ __forceinline__ void row_commit(const StripGeom& g, RowShared& S, const RowFlight& F, int r, int head) {
+template <int kSub, class RS>
+__device__ __forceinline__ void row_commit(const StripGeom& g, RS& S, const RowFlight<kSub>& F, int r, int head) {
     const int lane = threadIdx.x;
-    for (int i = lane; i <= g.HX; i += kRowThreads) S.rs[r][i] = head + S.nrs[i];
-    if (lane < 3) S.rm[r][lane] = S.nrm[lane];
+    const int SB = kSub * g.HX;
+    for (int i = lane; i <= SB; i += kRowThreads) S.rs[r][i] = head + S.nrs[i];
+    if (lane < 3 * kSub) S.rm[r][lane / 3][lane % 3] = S.nrm[lane / 3][lane % 3];
 #pragma unroll
     for (int q = 0; q < 2; ++q) {
         const int e = lane + q * kRowThreads;
@@ -513,8 +541,8 @@ __device__ __forceinline__ void row_commit(const StripGeom& g, RowShared& S, con
     for (int q = 0; q < 2; ++q) {
         const int e = lane + q * kRowThreads;
         if (e < F.total) {
-            const int hx = S.nebin[e];
-            const int s0 = S.nrs[hx], s1 = S.nrs[hx + 1];
+            const int j = S.nebin[e];
+            const int s0 = S.nrs[j], s1 = S.nrs[j + 1];
             const int a = F.aidx[q];
             int rank = 0;
             for (int u = s0; u < s1; ++u) rank += S.u.naidx[u] < a ? 1 : 0;
@@ -525,46 +553,63 @@ __device__ __forceinline__ void row_commit(const StripGeom& g, RowShared& S, con
             S.dens[pp] = F.dens[q];
             S.flags[pp] = F.flags[q];
             S.aidx[pp] = a;
-            S.sbin[pp] = uint8_t(hx);
+            S.sbin[pp] = uint8_t(j);
         }
     }
     __syncthreads();
 }
 
-// The cells of row y (ring row rc; rows y-1 and y+1 in rm_ and rp).
-template <bool kFluid>
+// element r (0..2) of a three-element register array, without indexing it at run time (which
+// would place the array in scratch memory)
+__device__ __forceinline__ int pick3(const int (&a)[3], int r) { return r == 0 ? a[0] : (r == 1 ? a[1] : a[2]); }
+
+// The cells of staged row Y (ring row rc; rows Y-1 and Y+1 in rm_ and rp).
+template <bool kFluid, int kSub, class RS>
 __device__ __forceinline__ void row_compute(const Ctx& c, const DevWorld& w, const DevParams& P, int wi,
-                                            const StripGeom& g, RowShared& S, const Region& R, int rm_, int rc,
-                                            int rp) {
+                                            const StripGeom& g, RS& S, const Region& R, int rm_, int rc, int rp) {
     const int lane = threadIdx.x;
     const float Wf = float(w.W), Hf = float(w.H);
-    const int cs = S.rs[rc][1];
-    const int ncell = S.rs[rc][g.IX + 1] - cs;
+    // the cells: the interior bins of each bin row of the staged row
+    const int cs0 = S.rs[rc][1];
+    const int n0 = S.rs[rc][g.IX + 1] - cs0;
+    const int cs1 = kSub == 2 ? S.rs[rc][g.HX + 1] : 0;
+    const int ncell = n0 + (kSub == 2 ? S.rs[rc][g.HX + g.IX + 1] - cs1 : 0);
     const float w0 = kFluid ? sph_w(0.0f) * P.invH2 : 0.0f;  // a fluid cell's own density term
     for (int q0 = 0; q0 < ncell; q0 += kRowThreads) {
-        // candidates of this lane's cell: three contiguous pool ranges, one per stencil row
+        // candidates of this lane's cell: three contiguous pool ranges, one per stencil bin row
         const int q = q0 + lane;
         const bool active = q < ncell;
         Nb self;
         int bt[kMaxBonds];
-        int a0s[3] = {0, 0, 0}, ns[3] = {0, 0, 0};
+        int a0s[3] = {0, 0, 0}, ns[3] = {0, 0, 0}, ringOf[3] = {rm_, rc, rp};
         unsigned long long mask = 0ull, bmask = 0ull;
         bool direct = false;
         int selfIdx = -1;  // fluid: number of the cell's pairs that come before its own density term
-        const int su = cs + q;  // pool position, not wrapped
+        const int t = kSub == 2 && q >= n0 ? 1 : 0;    // the cell's bin row in the staged row
+        const int su = t == 0 ? cs0 + q : cs1 + (q - n0);  // pool position, not wrapped
         const int sp = su & kPoolMask;
         int slot = 0;
         if (active) {
-            self = row_nb(S, g, rc, sp);
+            self = row_nb<kSub>(S, g, rc, sp);
             slot = c.A.slot[self.aidx];  // needed only at the end: loaded early
             load_bonds(c, self, bt);
-            const int hx = S.sbin[sp];
-            a0s[0] = S.rs[rm_][hx - 1];
-            ns[0] = S.rs[rm_][hx + 2] - a0s[0];
-            a0s[1] = S.rs[rc][hx - 1];
-            ns[1] = S.rs[rc][hx + 2] - a0s[1];
-            a0s[2] = S.rs[rp][hx - 1];
-            ns[2] = S.rs[rp][hx + 2] - a0s[2];
+            const int hx = S.sbin[sp] - t * g.HX;
+            // stencil bin rows (ring row, bin row): above, own, below
+            int subOf[3] = {0, 0, 0};
+            if (kSub == 2) {
+                ringOf[0] = t == 0 ? rm_ : rc;
+                subOf[0] = t == 0 ? 1 : 0;
+                ringOf[1] = rc;
+                subOf[1] = t;
+                ringOf[2] = t == 0 ? rc : rp;
+                subOf[2] = t == 0 ? 1 : 0;
+            }
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                const int b0 = subOf[k] * g.HX + hx;
+                a0s[k] = S.rs[ringOf[k]][b0 - 1];
+                ns[k] = S.rs[ringOf[k]][b0 + 2] - a0s[k];
+            }
             if (ns[0] + ns[1] + ns[2] > 64) {
                 direct = true;
             } else {
@@ -615,7 +660,7 @@ __device__ __forceinline__ void row_compute(const Ctx& c, const DevWorld& w, con
                         m &= m - 1ull;
                         const int r = ci < ns[0] ? 0 : (ci < ns[0] + ns[1] ? 1 : 2);
                         const int rest = ci - (r == 0 ? 0 : (r == 1 ? ns[0] : ns[0] + ns[1]));
-                        const int np = (a0s[r == 0 ? 0 : (r == 1 ? 1 : 2)] + rest) & kPoolMask;
+                        const int np = (pick3(a0s, r) + rest) & kPoolMask;
                         if (!is_bonded(bt, S.aidx[np] - w.cellOff)) continue;
                         if constexpr (!kFluid) {
                             // a bonded collision pair adds only its overlap push (closer than
@@ -658,27 +703,25 @@ __device__ __forceinline__ void row_compute(const Ctx& c, const DevWorld& w, con
                             r = 2;
                         }
                     }
-                    const int np = (a0s[r == 0 ? 0 : (r == 1 ? 1 : 2)] + rest) & kPoolMask;
+                    const int np = (pick3(a0s, r) + rest) & kPoolMask;
                     const uint32_t bonded = uint32_t((bmask >> ci) & 1ull);
                     S.u.pr.pself[p - W0] = uint8_t(sp);
-                    S.u.pr.pnb[p - W0] = uint16_t((bonded << 15) | (uint32_t(r) << 8) | uint32_t(np));
+                    S.u.pr.pnb[p - W0] = uint16_t((bonded << 15) | (uint32_t(pick3(ringOf, r)) << 8) | uint32_t(np));
                 }
             }
             __syncthreads();
             for (int off = W0; off < wend; off += kRowThreads) {
                 // pair terms, one pair per lane
                 const int p = off + lane;
-                float t[kPairTerms];
+                float tm[kPairTerms];
 #pragma unroll
-                for (int v = 0; v < kPairTerms; ++v) t[v] = 0.0f;
+                for (int v = 0; v < kPairTerms; ++v) tm[v] = 0.0f;
                 uint32_t f = 0u;
                 if (p < wend) {
                     const uint32_t rec = S.u.pr.pnb[p - W0];
-                    const int r = int((rec >> 8) & 3u);
-                    const int rr = r == 0 ? rm_ : (r == 1 ? rc : rp);
-                    const Nb s = row_nb(S, g, rc, S.u.pr.pself[p - W0]);
-                    const Nb o = row_nb(S, g, rr, int(rec & 0xFFu));
-                    f = pair_term<kFluid>(c, w, P, wi, s, o, (rec >> 15) != 0u, t, g.seam);
+                    const Nb s = row_nb<kSub>(S, g, rc, S.u.pr.pself[p - W0]);
+                    const Nb o = row_nb<kSub>(S, g, int((rec >> 8) & 3u), int(rec & 0xFFu));
+                    f = pair_term<kFluid>(c, w, P, wi, s, o, (rec >> 15) != 0u, tm, g.seam);
                 }
                 // the owner adds its pairs of this round in visiting order, reading each from the
                 // lane that evaluated it (every lane takes part in every exchange)
@@ -688,7 +731,7 @@ __device__ __forceinline__ void row_compute(const Ctx& c, const DevWorld& w, con
                     const int src = (lo + j - off) & (kRowThreads - 1);
                     float tt[kPairTerms];
 #pragma unroll
-                    for (int v = 0; v < kPairTerms; ++v) tt[v] = __shfl(t[v], src, kRowThreads);
+                    for (int v = 0; v < kPairTerms; ++v) tt[v] = __shfl(tm[v], src, kRowThreads);
                     const uint32_t ff = uint32_t(__shfl(int(f), src, kRowThreads));
                     if (j < n) {
                         if (lo + j - pbase == selfIdx) a.rho = a.rho + w0;
@@ -704,10 +747,9 @@ __device__ __forceinline__ void row_compute(const Ctx& c, const DevWorld& w, con
                 atomicAdd(&c.sc->directCells, 1ull);
                 cell_direct<kFluid>(c, w, P, wi, self, R, [&](auto&& visit) {
 #pragma unroll
-                    for (int r = 0; r < 3; ++r) {
-                        const int rr = r == 0 ? rm_ : (r == 1 ? rc : rp);
-                        for (int u = a0s[r]; u < a0s[r] + ns[r]; ++u) visit(row_nb(S, g, rr, u & kPoolMask));
-                    }
+                    for (int r = 0; r < 3; ++r)
+                        for (int u = a0s[r]; u < a0s[r] + ns[r]; ++u)
+                            visit(row_nb<kSub>(S, g, ringOf[r], u & kPoolMask));
                 });
             } else {
                 cell_finish<kFluid>(c, w, P, wi, self, slot, bt, a, R);
@@ -767,9 +809,84 @@ __device__ void rows_direct(const Ctx& c, const DevWorld& w, const DevParams& P,
     }
 }
 
+// Whether staged rows y0-1 .. y0+IY (kSub bin rows each) each fit a row in flight and every three
+// consecutive ones the pool (the same answer on every lane).
+template <int kSub>
+__device__ __forceinline__ bool stage_fits(const Ctx& c, const StripGeom& g, int y0, int IY) {
+    const int lane = threadIdx.x;
+    int total = 0;
+    if (lane < IY + 2) {
+        const int Y = y0 - 1 + lane;
+#pragma unroll
+        for (int t = 0; t < kSub; ++t) {
+            const int yy = kSub * Y + t;
+            const int rowFirst = g.binOff + wrap_small(yy, g.nby) * g.nbx + g.x0;
+            total += int(c.start[rowFirst + g.IX] - c.start[rowFirst]);
+            const int gl = strip_bin(g, yy, 0), gr = strip_bin(g, yy, g.IX + 1);
+            total += int(c.start[gl + 1] - c.start[gl]) + int(c.start[gr + 1] - c.start[gr]);
+        }
+    }
+    const int three = total + __shfl_down(total, 1, kRowThreads) + __shfl_down(total, 2, kRowThreads);
+    return !__any(total > kRowCap || (lane < IY && three > kPool));
+}
+
+// One chunk of IY staged rows from staged row y0 (kSub bin rows each); false (and nothing done)
+// when its staged rows do not fit the LDS.
+template <bool kFluid, int kSub, class RS>
+__device__ __forceinline__ bool chunk_rows(const Ctx& c, const DevWorld& w, const DevParams& P, int wi,
+                                           StripGeom& g, RS& S, int y0, int IY) {
+    const int lane = threadIdx.x;
+    const int nbyS = g.nby / kSub;
+    // Away from the seam no staged bin wraps, and two stencil neighbours lie less than three bin
+    // edges apart (the last column and row absorb less than one edge of remainder) while half the
+    // world is at least four: the torus difference is the plain difference there, bit for bit.
+    g.seam = g.x0 == 0 || g.x0 + g.IX >= g.nbx || y0 == 0 || y0 + IY >= nbyS || g.nbx < 8 || g.nby < 8;
+    if (!stage_fits<kSub>(c, g, y0, IY)) return false;
+    const Region R{S.hist, g.x0, kSub * y0, g.IX, kSub * IY};
+    for (int b = lane; b < (g.IX * kSub * IY + 1) / 2; b += kRowThreads) S.hist[b] = 0u;
+
+    RowFlight<kSub> F, G;
+    int head = 0;  // pool position of the next committed row
+    // rows y0-1, y0, y0+1 into ring rows 0, 1, 2
+    for (int r = 0; r < 3; ++r) {
+        row_issue_counts<kSub>(c, g, y0 - 1 + r, F);
+        row_issue_entries<kSub>(c, g, S, F);
+        row_commit<kSub>(g, S, F, r, head);
+        head += F.total;
+    }
+    if (IY >= 2) row_issue_counts<kSub>(c, g, y0 + 2, F);
+    for (int y = y0; y < y0 + IY; ++y) {
+        const int rc = (y - y0 + 1) % 3;
+        const int rm_ = (rc + 2) % 3, rp = (rc + 1) % 3;
+        const bool next = y + 2 <= y0 + IY;  // row y+2 is the lower halo of row y+1
+        if (next) {
+            row_issue_entries<kSub>(c, g, S, F);
+            if (y + 3 <= y0 + IY) row_issue_counts<kSub>(c, g, y + 3, G);
+        }
+        row_compute<kFluid, kSub>(c, w, P, wi, g, S, R, rm_, rc, rp);
+        if (next) {
+            row_commit<kSub>(g, S, F, rm_, head);  // row y+2 replaces row y-1
+            head += F.total;
+#pragma unroll
+            for (int q = 0; q < RowFlight<kSub>::kSlots; ++q) {
+                F.s0[q] = G.s0[q];
+                F.s1[q] = G.s1[q];
+            }
+        }
+    }
+    // the region's owner-counted bin sizes for the next step (every bin written, zeros included)
+    __syncthreads();
+    for (int r = 0; r < kSub * IY; ++r)
+        for (int x = lane; x < g.IX; x += kRowThreads)
+            c.privNext[g.binOff + (kSub * y0 + r) * g.nbx + g.x0 + x] = hist_count(S.hist, r * g.IX + x);
+    return true;
+}
+
+// A collision world staged two bin rows at a time (sparse rows) runs a chunk whose two-row stages
+// do not fit as single bin rows over the same bins.
 template <bool kFluid>
 __global__ void __launch_bounds__(kRowThreads) k_neighbor_rows(Ctx c) {
-    __shared__ RowShared S;
+    __shared__ RowShared<kFluid ? 1 : kSubMax> S;
     const int2 tb = c.tileBlocks[blockIdx.x];
     if (tb.x < 0) return;
     const int wi = tb.x;
@@ -786,70 +903,25 @@ __global__ void __launch_bounds__(kRowThreads) k_neighbor_rows(Ctx c) {
     g.binOff = w.binOff;
     g.cellOff = w.cellOff;
     g.base = c.start[w.binOff];
-    const int y0 = chunk * w.tileY;
-    const int IY = min(w.tileY, w.nby - y0);
-    const int lane = threadIdx.x;
-    // Away from the seam no staged bin wraps, and two stencil neighbours lie less than three bin
-    // edges apart (the last column and row absorb less than one edge of remainder) while half the
-    // world is at least four: the torus difference is the plain difference there, bit for bit.
-    g.seam = g.x0 == 0 || g.x0 + g.IX >= g.nbx || y0 == 0 || y0 + IY >= g.nby || g.nbx < 8 || g.nby < 8;
-
-    // rows y0-1 .. y0+IY must each fit a row in flight, and every three consecutive ones the pool
-    int total = 0;
-    if (lane < IY + 2) {
-        const int yy = y0 - 1 + lane;
-        const int rowFirst = g.binOff + wrap_small(yy, g.nby) * g.nbx + g.x0;
-        total = int(c.start[rowFirst + g.IX] - c.start[rowFirst]);
-        const int gl = strip_bin(g, yy, 0), gr = strip_bin(g, yy, g.IX + 1);
-        total += int(c.start[gl + 1] - c.start[gl]) + int(c.start[gr + 1] - c.start[gr]);
+    g.seam = true;
+    const int y0 = chunk * w.tileY;  // in staged rows of w.rowsPerStage bin rows
+    const int IY = min(w.tileY, w.nby / w.rowsPerStage - y0);
+    const int by0 = w.rowsPerStage * y0, bIY = w.rowsPerStage * IY;  // the chunk in bin rows
+    bool done = false;
+    // (fluid worlds are always staged one bin row at a time: the two-row variant would cost the
+    // fluid kernel a third of its resident waves in registers)
+    if constexpr (!kFluid)
+        if (w.rowsPerStage == 2) done = chunk_rows<kFluid, 2>(c, w, P, wi, g, S, y0, IY);
+    if (!done) done = chunk_rows<kFluid, 1>(c, w, P, wi, g, S, by0, bIY);
+    if (!done) {
+        // too crowded for the LDS staging: every cell walks its stencil in global memory, and the
+        // owner-counted part of the region's bins is empty
+        if (threadIdx.x == 0) atomicAdd(&c.sc->fallbackChunks, 1ull);
+        rows_direct<kFluid>(c, w, P, wi, g, by0, bIY);
+        for (int r = 0; r < bIY; ++r)
+            for (int x = threadIdx.x; x < g.IX; x += kRowThreads)
+                c.privNext[g.binOff + (by0 + r) * g.nbx + g.x0 + x] = 0u;
     }
-    const int three = total + __shfl_down(total, 1, kRowThreads) + __shfl_down(total, 2, kRowThreads);
-    const bool tooDense = total > kRowCap || (lane < IY && three > kPool);
-    if (__any(tooDense)) {
-        if (lane == 0) atomicAdd(&c.sc->fallbackChunks, 1ull);
-        rows_direct<kFluid>(c, w, P, wi, g, y0, IY);
-        // the owner-counted part of the region's bins is empty
-        for (int r = 0; r < IY; ++r)
-            for (int x = lane; x < g.IX; x += kRowThreads) c.privNext[g.binOff + (y0 + r) * g.nbx + g.x0 + x] = 0u;
-        return;
-    }
-    const Region R{S.hist, g.x0, y0, g.IX, IY};
-    for (int b = lane; b < (g.IX * IY + 1) / 2; b += kRowThreads) S.hist[b] = 0u;
-
-    RowFlight F, G;
-    int head = 0;  // pool position of the next committed row
-    // rows y0-1, y0, y0+1 into ring rows 0, 1, 2
-    for (int r = 0; r < 3; ++r) {
-        row_issue_counts(c, g, y0 - 1 + r, F);
-        row_issue_entries(c, g, S, F);
-        row_commit<kFluid>(g, S, F, r, head);
-        head += F.total;
-    }
-    if (IY >= 2) row_issue_counts(c, g, y0 + 2, F);
-    for (int y = y0; y < y0 + IY; ++y) {
-        const int rc = (y - y0 + 1) % 3;
-        const int rm_ = (rc + 2) % 3, rp = (rc + 1) % 3;
-        const bool next = y + 2 <= y0 + IY;  // row y+2 is the lower halo of row y+1
-        if (next) {
-            row_issue_entries(c, g, S, F);
-            if (y + 3 <= y0 + IY) row_issue_counts(c, g, y + 3, G);
-        }
-        row_compute<kFluid>(c, w, P, wi, g, S, R, rm_, rc, rp);
-        if (next) {
-            row_commit<kFluid>(g, S, F, rm_, head);  // row y+2 replaces row y-1
-            head += F.total;
-#pragma unroll
-            for (int q = 0; q < 2; ++q) {
-                F.s0[q] = G.s0[q];
-                F.s1[q] = G.s1[q];
-            }
-        }
-    }
-    // the region's owner-counted bin sizes for the next step (every bin written, zeros included)
-    __syncthreads();
-    for (int r = 0; r < IY; ++r)
-        for (int x = lane; x < g.IX; x += kRowThreads)
-            c.privNext[g.binOff + (y0 + r) * g.nbx + g.x0 + x] = hist_count(S.hist, r * g.IX + x);
 }
 
 int neighbor_blocks_per_cu(bool fluid) {

@@ -407,14 +407,14 @@ void WorldSet::build(const std::vector<Source>& sources) {
     std::vector<uint8_t> pending;
     std::vector<int> stagedHint;
     std::vector<uint8_t> bondWork;
-    std::vector<int> stripChoice;
+    std::vector<int> stripChoice, stageChoice;
     int cellOff = 0, partOff = 0, bins = 0;
     for (const auto& src : sources) {
         HostWorld h;
         int live = 0;
         uint64_t ts = 0;
         bool pend = false;
-        int strip = kStripMax, hint = 0;
+        int strip = kStripMax, stage = 1, hint = 0;
         bool work = false;
         if (src.image) {
             if (src.from == this) throw Error(ALIEN_E_ARG, "a world set cannot be rebuilt from itself");
@@ -430,7 +430,8 @@ void WorldSet::build(const std::vector<Source>& sources) {
                     work = true;
             }
             const int bs = bin_size_for(h.params);
-            strip = choose_strip(im.pos, h.desc.worldSizeX, h.desc.worldSizeY, bs);
+            strip = choose_strip(im.pos, h.desc.worldSizeX, h.desc.worldSizeY, bs,
+                                 h.params.motionType == ALIEN_MOTION_COLLISION, stage);
         } else {
             if (src.from == this) throw Error(ALIEN_E_ARG, "a world set cannot be rebuilt from itself");
             h = src.from->hosts_.at(src.fromIndex);
@@ -438,6 +439,7 @@ void WorldSet::build(const std::vector<Source>& sources) {
             ts = src.from->timestep(src.fromIndex);
             pend = src.from->densityPending_.at(src.fromIndex) != 0;
             strip = src.from->stripChoice_.at(src.fromIndex);
+            stage = src.from->stageChoice_.at(src.fromIndex);
             hint = src.from->stagedHint_.at(src.fromIndex);
             work = src.from->bondWork_.at(src.fromIndex) != 0;
         }
@@ -456,8 +458,9 @@ void WorldSet::build(const std::vector<Source>& sources) {
         w.binOff = bins;
         w.partOff = partOff;
         w.partCap = live;
-        strip_shape(w, std::min(strip, stripCap_));
+        strip_shape(w, std::min(strip, stripCap_), stage);
         stripChoice.push_back(strip);
+        stageChoice.push_back(stage);
         cellOff += h.numCells;
         partOff += live;
         bins += w.nbx * w.nby;
@@ -623,6 +626,7 @@ void WorldSet::build(const std::vector<Source>& sources) {
     bondWork_ = std::move(bondWork);
     active_.assign(worlds_.size(), 1);
     stripChoice_ = std::move(stripChoice);
+    stageChoice_ = std::move(stageChoice);
     chunk_shape(worlds_, params_);
     shape_.hasBondWork = false;
     for (uint8_t b : bondWork_) shape_.hasBondWork = shape_.hasBondWork || b != 0;
@@ -665,7 +669,7 @@ void WorldSet::build(const std::vector<Source>& sources) {
 
 // Strip shape of the neighbour pass for one world, measured on the given positions.  Only
 // performance depends on it.
-int choose_strip(const std::vector<float2>& pos, int W, int H, int binSize) {
+int choose_strip(const std::vector<float2>& pos, int W, int H, int binSize, bool twoRowStages, int& rowsPerStage) {
     const int nbx = W / binSize, nby = H / binSize;
     std::vector<int> cnt(size_t(nbx) * nby, 0);
     for (const float2& p : pos) {
@@ -694,17 +698,25 @@ int choose_strip(const std::vector<float2>& pos, int W, int H, int binSize) {
                 fits = rows[size_t(y)] <= kRowCap && three <= kRowPool;
             }
         }
-        if (fits) return sw;
+        if (fits) {
+            // sparse rows (fewer than half a wave of cells per bin row of a strip on average) are
+            // staged two at a time; a chunk where that overflows runs single rows
+            const double perRow = double(pos.size()) / (double(nby) * double((nbx + sw - 1) / sw));
+            rowsPerStage = twoRowStages && perRow < 32.0 && nby % 2 == 0 && nby >= 6 ? 2 : 1;
+            return sw;
+        }
     }
+    rowsPerStage = 1;
     return 4;
 }
 
-// Strip width (capped by the world's columns) and strip count; chunk_shape sets the rows.
-void strip_shape(DevWorld& w, int strip) {
+// Strip width (capped by the world's columns), strip count and staging; chunk_shape sets the rows.
+void strip_shape(DevWorld& w, int strip, int rowsPerStage) {
     w.tileX = std::max(1, std::min(strip, w.nbx));
     w.ntx = (w.nbx + w.tileX - 1) / w.tileX;
-    w.tileY = std::min(2, w.nby);
-    w.nty = (w.nby + w.tileY - 1) / w.tileY;
+    w.rowsPerStage = rowsPerStage;
+    w.tileY = 1;
+    w.nty = w.nby / rowsPerStage;
 }
 
 // Rows per neighbour-pass chunk, one value for all worlds of a set.  A chunk stages two halo rows
@@ -728,12 +740,17 @@ void chunk_shape(std::vector<DevWorld>& worlds, const std::vector<DevParams>& pa
     const int64_t slots = int64_t(perCU) * numCUs;
     int best = 2;
     double bestCost = 1e300;
+    // staged rows per chunk of a world for `rows` bin rows (a chunk covers at most kChunkMax bin
+    // rows: the LDS histogram of its region)
+    auto stagedRows = [](const DevWorld& w, int rows) {
+        return std::max(1, std::min(rows, w.nby) / w.rowsPerStage);
+    };
     for (int rows = 2; rows <= kChunkMax; ++rows) {
         int64_t tiles = 0;
         for (const DevWorld& w : worlds) {
             if (w.numCells == 0) continue;
-            const int ty = std::min(rows, w.nby);
-            tiles += int64_t(w.ntx) * ((w.nby + ty - 1) / ty);
+            const int ty = stagedRows(w, rows), nyS = w.nby / w.rowsPerStage;
+            tiles += int64_t(w.ntx) * ((nyS + ty - 1) / ty);
         }
         const double cost = double((tiles + slots - 1) / slots) * (rows + kHaloCost);
         if (cost < bestCost) {
@@ -742,8 +759,9 @@ void chunk_shape(std::vector<DevWorld>& worlds, const std::vector<DevParams>& pa
         }
     }
     for (DevWorld& w : worlds) {
-        w.tileY = std::min(best, w.nby);
-        w.nty = (w.nby + w.tileY - 1) / w.tileY;
+        const int nyS = w.nby / w.rowsPerStage;
+        w.tileY = stagedRows(w, best);
+        w.nty = (nyS + w.tileY - 1) / w.tileY;
     }
 }
 
@@ -826,7 +844,8 @@ void WorldSet::setStripCap(int cap) {
         throw Error(ALIEN_E_ARG, "strip cap must be 8, 16, 32 or 64");
     stripCap_ = cap;
     if (worlds_.empty()) return;
-    for (size_t i = 0; i < worlds_.size(); ++i) strip_shape(worlds_[i], std::min(stripChoice_[i], cap));
+    for (size_t i = 0; i < worlds_.size(); ++i)
+        strip_shape(worlds_[i], std::min(stripChoice_[i], cap), stageChoice_[i]);
     chunk_shape(worlds_, params_);
     check_hip(hipMemcpy(dWorlds_, worlds_.data(), sizeof(DevWorld) * worlds_.size(), hipMemcpyHostToDevice), "H2D");
     buildTables();

@@ -33,8 +33,8 @@ int bin_size_for(const alien_params& p);
 void validate_params(const alien_params& p);
 void validate_world(const alien_world_desc& d, const alien_params& p);
 DevParams to_dev_params(const alien_params& p);
-int choose_strip(const std::vector<float2>& pos, int W, int H, int binSize);
-void strip_shape(DevWorld& w, int strip);
+int choose_strip(const std::vector<float2>& pos, int W, int H, int binSize, bool twoRowStages, int& rowsPerStage);
+void strip_shape(DevWorld& w, int strip, int rowsPerStage);
 void chunk_shape(std::vector<DevWorld>& worlds, const std::vector<DevParams>& params);
 constexpr int kXcds = 8;  // accelerator complex dies of an MI355X
 
@@ -194,6 +194,7 @@ private:
     std::vector<uint8_t> bondWork_;  // per world: a bonded or barrier cell exists or can appear by fusion
     std::vector<uint8_t> active_;    // per world: included in the launch tables
     std::vector<int> stripChoice_;   // per world: strip width chosen from its cell density
+    std::vector<int> stageChoice_;   // per world: bin rows per staged row (1 or 2)
     int stripCap_ = kStripMax;
     void adaptGrids();
     std::vector<Arr> tableAllocs_;
