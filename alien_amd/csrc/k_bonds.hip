@@ -24,10 +24,15 @@ __global__ void k_remap(Ctx c) {
         const int x = ent.x;
         const DevWorld& w = c.worlds[ent.y];
         const int nc = flags_num_conn(c.B.flags[x]);
-        for (int b = 0; b < nc; ++b) {
-            const int t = c.B.btgt[b * cap + x];
-            c.B.btgt[b * cap + x] = c.newidx[w.cellOff + t] - w.cellOff;
-        }
+        // all targets first, then all lookups, then the stores (so that the loads overlap)
+        int t[kMaxBonds], ni[kMaxBonds];
+#pragma unroll
+        for (int b = 0; b < kMaxBonds; ++b) t[b] = b < nc ? c.B.btgt[b * cap + x] : 0;
+#pragma unroll
+        for (int b = 0; b < kMaxBonds; ++b) ni[b] = b < nc ? c.newidx[w.cellOff + t[b]] : 0;
+#pragma unroll
+        for (int b = 0; b < kMaxBonds; ++b)
+            if (b < nc) c.B.btgt[b * cap + x] = ni[b] - w.cellOff;
     }
 }
 
@@ -41,7 +46,9 @@ __device__ __forceinline__ f2 bond_vector(const Ctx& c, int o, f2 px, f2 sx, int
 }
 
 // connection forces of the staged cells: own force plus per-bond angle forces for the partners;
-// pass 1 also runs checkConnections (CellProcessor.cuh:493-523) on the moved positions
+// pass 1 also runs checkConnections (CellProcessor.cuh:493-523) on the moved positions.  The
+// partners' data of all six bond slots is loaded up front (slots beyond the cell's bonds read the
+// cell itself and are not used), so the loads are in flight together.
 __global__ void k_conn_own(Ctx c, int pass) {
     const int n = c.sc->staged;
     const int cap = c.cellCap;
@@ -63,25 +70,43 @@ __global__ void k_conn_own(Ctx c, int pass) {
         const float sx = c.bstiff[x];
         const float kx2 = sx * sx;
         int tg[kMaxBonds];
+        f2 disp[kMaxBonds];
+        float kc2[kMaxBonds], rest[kMaxBonds];
 #pragma unroll
-        for (int b = 0; b < kMaxBonds; ++b) tg[b] = b < nc ? w.cellOff + c.B.btgt[b * cap + x] : -1;
-        f2 prevDisp = bond_vector(c, tg[nc - 1], px, shx, pass, Wf, Hf);
+        for (int b = 0; b < kMaxBonds; ++b) tg[b] = b < nc ? w.cellOff + c.B.btgt[b * cap + x] : x;
+#pragma unroll
+        for (int b = 0; b < kMaxBonds; ++b) {
+            const int o = tg[b];
+            const float so = c.bstiff[o];
+            kc2[b] = so * so;
+            rest[b] = c.B.brest[b * cap + x];
+            disp[b] = bond_vector(c, o, px, shx, pass, Wf, Hf);
+        }
+        // the bond before the first is the last one
+        f2 prevDisp = disp[0];
+        int lastTarget = tg[0];
+#pragma unroll
+        for (int b = 1; b < kMaxBonds; ++b)
+            if (b == nc - 1) {
+                prevDisp = disp[b];
+                lastTarget = tg[b];
+            }
         f2 force = {0.0f, 0.0f};
         uint32_t v = 0u;
         bool tooLong = false;
         const float maxBind = P.maxBindingDistance[flags_color(fx) % kColors];
-        for (int b = 0; b < nc; ++b) {
+#pragma unroll
+        for (int b = 0; b < kMaxBonds; ++b) {
+            if (b >= nc) break;
             const int o = tg[b];
-            const float so = c.bstiff[o];
-            const float kc2 = so * so;
-            const f2 disp = bond_vector(c, o, px, shx, pass, Wf, Hf);
-            const float ad = len2(disp);
+            const f2 d = disp[b];
+            const float ad = len2(d);
             if (pass == 1 && ad > maxBind) tooLong = true;  // checkConnections on new positions
-            const float dev = ad - c.B.brest[b * cap + x];
-            force = add(force, dvd(mul(mul(unit(disp), dev), kx2 + kc2), 6.0f));
+            const float dev = ad - rest[b];
+            force = add(force, dvd(mul(mul(unit(d), dev), kx2 + kc2[b]), 6.0f));
+            bool skipped = false;  // a bond whose actual angle is negative keeps the previous vector
             if (considerAngles && (nc > 2 || (nc == 2 && b == 0))) {
-                const int last = (b + nc - 1) % nc;
-                const int lc = tg[last];
+                const int lc = b == 0 ? lastTarget : tg[b > 0 ? b - 1 : 0];
                 const uint32_t fo = c.B.flags[o];
                 const int no = flags_num_conn(fo);
                 bool tri = false;
@@ -91,32 +116,35 @@ __global__ void k_conn_own(Ctx c, int pass) {
                         break;
                     }
                 if (!tri) {
-                    const float angle = angle_of(disp);
+                    const float angle = angle_of(d);
                     const float prevAngle = angle_of(prevDisp);
                     const float act = sub_angle(angle, prevAngle);
-                    if (act < 0.0f) continue;
-                    const float ref = c.B.bang[b * cap + x];
-                    const float strength = fabsf(ref - act) / 2000.0f * kx2;
-                    f2 f1 = mul(dvd(unit(disp), fmaxf(len2(disp), P.minDistance)), strength);
-                    rot_cw(f1);
-                    f2 fb = mul(dvd(unit(prevDisp), fmaxf(len2(prevDisp), P.minDistance)), strength);
-                    rot_ccw(fb);
-                    if (ref < act) {
-                        f1 = mul(f1, -1.0f);
-                        fb = mul(fb, -1.0f);
+                    if (act < 0.0f) {
+                        skipped = true;
+                    } else {
+                        const float ref = c.B.bang[b * cap + x];
+                        const float strength = fabsf(ref - act) / 2000.0f * kx2;
+                        f2 f1 = mul(dvd(unit(d), fmaxf(len2(d), P.minDistance)), strength);
+                        rot_cw(f1);
+                        f2 fb = mul(dvd(unit(prevDisp), fmaxf(len2(prevDisp), P.minDistance)), strength);
+                        rot_ccw(fb);
+                        if (ref < act) {
+                            f1 = mul(f1, -1.0f);
+                            fb = mul(fb, -1.0f);
+                        }
+                        if (!flags_barrier(fo)) {
+                            c.out1[b * cap + x] = to_float2(f1);
+                            v |= 1u << b;
+                        }
+                        if (!flags_barrier(c.B.flags[lc])) {
+                            c.out2[b * cap + x] = to_float2(fb);
+                            v |= 1u << (6 + b);
+                        }
+                        force = sub(force, add(f1, fb));
                     }
-                    if (!flags_barrier(fo)) {
-                        c.out1[b * cap + x] = to_float2(f1);
-                        v |= 1u << b;
-                    }
-                    if (!flags_barrier(c.B.flags[lc])) {
-                        c.out2[b * cap + x] = to_float2(fb);
-                        v |= 1u << (6 + b);
-                    }
-                    force = sub(force, add(f1, fb));
                 }
             }
-            prevDisp = disp;
+            if (!skipped) prevDisp = d;
         }
         c.own[x] = to_float2(force);
         c.valid[x] = v;
@@ -124,15 +152,24 @@ __global__ void k_conn_own(Ctx c, int pass) {
     }
 }
 
+// own force plus the angle forces the partners wrote for this cell, in its bond order (R7); the
+// partners' markers are loaded up front
 __device__ __forceinline__ f2 gather_conn(const Ctx& c, const DevWorld& w, int x, int nc) {
     const int cap = c.cellCap;
     f2 acc = {0.0f, 0.0f};
     acc = add(acc, to_f2(c.own[x]));
     const int xl = x - w.cellOff;
-    for (int q = 0; q < nc; ++q) {
-        const int p = w.cellOff + c.B.btgt[q * cap + x];
-        const uint32_t vp = c.valid[p];
+    int pt[kMaxBonds];
+    uint32_t vps[kMaxBonds];
+#pragma unroll
+    for (int q = 0; q < kMaxBonds; ++q) pt[q] = q < nc ? w.cellOff + c.B.btgt[q * cap + x] : x;
+#pragma unroll
+    for (int q = 0; q < kMaxBonds; ++q) vps[q] = q < nc ? c.valid[pt[q]] : 0u;
+#pragma unroll
+    for (int q = 0; q < kMaxBonds; ++q) {
+        const uint32_t vp = vps[q];
         if (vp == 0u) continue;  // the partner wrote no angle force this pass (2 of 3 steps)
+        const int p = pt[q];
         const int np = flags_num_conn(c.B.flags[p]);
         int b = -1;
         for (int kk = 0; kk < np; ++kk)
