@@ -679,6 +679,12 @@ int choose_strip(const std::vector<float2>& pos, int W, int H, int binSize, bool
         const int bx = std::min(sx / binSize, nbx - 1), by = std::min(sy / binSize, nby - 1);
         ++cnt[size_t(by) * nbx + bx];
     }
+    // sparse rows (fewer than half a wave of cells per bin row of a strip on average) are staged
+    // two at a time; a chunk where that overflows runs single rows
+    auto stages = [&](int sw) {
+        const double perRow = double(pos.size()) / (double(nby) * double((nbx + sw - 1) / sw));
+        return twoRowStages && perRow < 32.0 && nby % 2 == 0 && nby >= 6 ? 2 : 1;
+    };
     // entries of every staged row as the neighbour pass stages it (interior plus the two halo
     // columns, both wrapped around the torus), for each candidate width; a chunk whose rows exceed
     // the row capacity, or whose three consecutive rows exceed the LDS pool, takes the slower
@@ -699,14 +705,11 @@ int choose_strip(const std::vector<float2>& pos, int W, int H, int binSize, bool
             }
         }
         if (fits) {
-            // sparse rows (fewer than half a wave of cells per bin row of a strip on average) are
-            // staged two at a time; a chunk where that overflows runs single rows
-            const double perRow = double(pos.size()) / (double(nby) * double((nbx + sw - 1) / sw));
-            rowsPerStage = twoRowStages && perRow < 32.0 && nby % 2 == 0 && nby >= 6 ? 2 : 1;
+            rowsPerStage = stages(sw);
             return sw;
         }
     }
-    rowsPerStage = 1;
+    rowsPerStage = stages(4);
     return 4;
 }
 

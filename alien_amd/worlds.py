@@ -387,9 +387,10 @@ def random_mix(seed, size=(64, 64), num_free=300, patches=6, num_particles=200, 
 def crowded_bins(seed, motion=MOTION_COLLISION, size=(96, 96), background=400):
     """Free cells crowded into a few bins so that the neighbour pass takes each of its slower
     paths: one bin of 140 cells (a staged bin row above 128 entries), three vertically stacked
-    bins of 95 cells (three consecutive rows above 256 entries together) and a 3x3 block of bins
-    with 9 cells each (cells with more than 64 candidates), over a sparse background.  Bins are 2
-    units wide at these parameters (interaction radius 1.3)."""
+    bins of 95 cells (three consecutive rows above 256 entries together), two stacked bins of 70
+    cells (a two-row stage above 128 entries whose single rows fit) and a 3x3 block of bins with 9
+    cells each (cells with more than 64 candidates), over a sparse background.  Bins are 2 units
+    wide at these parameters (interaction radius 1.3)."""
     rng = np.random.default_rng(seed)
     params = default_params(motionType=motion, smoothingLength=0.65)
 
@@ -399,7 +400,8 @@ def crowded_bins(seed, motion=MOTION_COLLISION, size=(96, 96), background=400):
         p[:, 1] = 2.0 * by + rng.uniform(0.05, 1.95, n)
         return p
 
-    groups = [in_bin(10, 10, 140), in_bin(30, 20, 95), in_bin(30, 21, 95), in_bin(30, 22, 95)]
+    groups = [in_bin(10, 10, 140), in_bin(30, 20, 95), in_bin(30, 21, 95), in_bin(30, 22, 95),
+              in_bin(20, 34, 70), in_bin(20, 35, 70)]
     groups += [in_bin(5 + dx, 40 + dy, 9) for dx in range(3) for dy in range(3)]
     crowd_pos = np.concatenate(groups)
     crowd = np.zeros(len(crowd_pos), dtype=CELL_DTYPE)
