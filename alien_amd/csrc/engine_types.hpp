@@ -91,7 +91,7 @@ struct StepCounters {
     int staged;      // cells of the staged (bonded or barrier) list
     int candidates;  // fusion candidate pairs
     int errors;      // bit 0: candidate list overflow
-    int pad;
+    int liveParticles;  // energy particles alive after this step's particle sort, all worlds
     // neighbour-pass paths since the data was set (not reset per step): chunks that walked their
     // stencils in global memory, and cells of staged rows that walked theirs alone
     unsigned long long fallbackChunks;

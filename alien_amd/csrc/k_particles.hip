@@ -149,7 +149,7 @@ __global__ void k_part_absorb(Ctx c) {
 }
 
 void launch_particles(const Ctx& c, const LaunchShape& sh, hipStream_t s, Recorder* rec) {
-    if (!sh.partBlocks) return;
+    if (!sh.partBlocks || !sh.hasParticles) return;
     const dim3 g(sh.partGrid), b(sh.partThreads);
     if (rec) rec->mark("part_move");
     hipLaunchKernelGGL(k_part_move, g, b, 0, s, c);

@@ -96,6 +96,7 @@ __global__ void __launch_bounds__(1024) k_scan_top(uint32_t* blockSums, int numT
         if (sc) {
             sc->staged = 0;
             sc->candidates = 0;
+            sc->liveParticles = 0;
         }
     }
 }
@@ -220,7 +221,9 @@ __global__ void k_part_binsort(Ctx c) {
     if (blockIdx.x == 0) {
         for (int wi = threadIdx.x; wi < c.numWorlds; wi += blockDim.x) {
             const DevWorld& w = c.worlds[wi];
-            c.ctr[wi].numParticles = int(c.start[w.pbinOff + w.pnbx * w.pnby] - c.start[w.pbinOff]);
+            const int live = int(c.start[w.pbinOff + w.pnbx * w.pnby] - c.start[w.pbinOff]);
+            c.ctr[wi].numParticles = live;
+            if (live > 0) atomicAdd(&c.sc->liveParticles, live);
         }
     }
     int2 blk = c.pbinBlocks[blockIdx.x];
@@ -265,6 +268,7 @@ void launch_part_scatter(const Ctx& c, const LaunchShape& sh, hipStream_t s) {
 }
 
 void launch_part_sort(const Ctx& c, const LaunchShape& sh, hipStream_t s) {
+    if (!sh.hasParticles) return;
     if (sh.partBlocks) hipLaunchKernelGGL(k_part_scatter, dim3(sh.partGrid), dim3(sh.partThreads), 0, s, c);
     if (sh.pbinBlocks) hipLaunchKernelGGL(k_part_binsort, dim3(sh.pbinBlocks), dim3(sh.binThreads), 0, s, c);
 }

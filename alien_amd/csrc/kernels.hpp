@@ -19,6 +19,9 @@ struct LaunchShape {
     int candGrid = 16;     // grid-stride kernels over fusion candidates
     bool hasFluid = false, hasCollision = false;  // motion modes present in the set
     bool hasBondWork = true;  // some world has (or can get by fusion) a bonded or barrier cell
+    // some world has live energy particles (nothing on this path creates particles, so once all
+    // are absorbed or merged the particle kernels, which would find no work, are not launched)
+    bool hasParticles = true;
 };
 
 // Optional per-launch hook: mark(name) is called right before each kernel launch and

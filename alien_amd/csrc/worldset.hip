@@ -630,6 +630,7 @@ void WorldSet::build(const std::vector<Source>& sources) {
     chunk_shape(worlds_, params_);
     shape_.hasBondWork = false;
     for (uint8_t b : bondWork_) shape_.hasBondWork = shape_.hasBondWork || b != 0;
+    shape_.hasParticles = partOff > 0;
     parity_ = 0;
     int hintTotal = 0;
     for (int h : stagedHint_) hintTotal += h;
@@ -1012,6 +1013,8 @@ void WorldSet::step(uint64_t n) {
         densityPending_[w] = 0;
     }
     for (uint64_t s = 0; s < n; ++s) {
+        // while particles live, every 32 steps: once none is left, the particle kernels go
+        if (shape_.hasParticles && s > 0 && s % 32 == 0) dropParticlesIfNone();
         if (timing_) {
             launch_step(makeCtx(parity_), shape_, stream_, recorder_.get());
             check_hip(hipGetLastError(), "kernel launch");
@@ -1036,6 +1039,17 @@ void WorldSet::step(uint64_t n) {
     synchronize();
     if (timing_) recorder_->collect(kernelTimes_);
     checkErrors();
+    if (shape_.hasParticles) dropParticlesIfNone();
+}
+
+void WorldSet::dropParticlesIfNone() {
+    synchronize();
+    StepCounters sc{};
+    check_hip(hipMemcpy(&sc, dSc_, sizeof(sc), hipMemcpyDeviceToHost), "D2H");
+    if (sc.liveParticles == 0) {
+        shape_.hasParticles = false;
+        invalidateGraphs();
+    }
 }
 
 void WorldSet::extract(int wi, WorldImage& img) {

@@ -148,6 +148,7 @@ private:
     Ctx makeCtx(int parity) const;
     void ensureSorted();
     void checkErrors();
+    void dropParticlesIfNone();  // stops launching the particle kernels once no particle is left
 
     hipStream_t stream_ = nullptr;
     std::vector<HostWorld> hosts_;
