@@ -198,11 +198,7 @@ int alien_get_counts(alien_world* w, uint64_t* numCells, uint64_t* numParticles)
     return guarded([&] {
         requireHealthy(w);
         if (numCells) *numCells = uint64_t(w->set->numCells(0));
-        if (numParticles) {
-            WorldImage img;
-            w->set->extract(0, img);
-            *numParticles = uint64_t(img.numParticles);
-        }
+        if (numParticles) *numParticles = uint64_t(w->set->liveParticles(0));
     });
 }
 
@@ -422,7 +418,7 @@ int alien_batch_get_counts(alien_batch* b, int32_t world, uint64_t* numCells, ui
         b->flush();
         require(world >= 0 && world < b->resident, "world index out of range");
         if (numCells) *numCells = uint64_t(b->set->numCells(world));
-        if (numParticles) *numParticles = uint64_t(b->set->numParticles(world));
+        if (numParticles) *numParticles = uint64_t(b->set->liveParticles(world));
     });
 }
 

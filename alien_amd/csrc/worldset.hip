@@ -220,10 +220,12 @@ WorldImage WorldImage::fromTO(const alien_world_desc& desc, const alien_params& 
 void WorldImage::toTO(std::vector<alien_cell_to>& cells, std::vector<alien_particle_to>& particles) const {
     const HostWorld& h = host;
     const int n = h.numCells;
+    // ascending id (ties by index), sorted as compact (id, index) pairs
+    std::vector<std::pair<uint64_t, int>> keys(static_cast<size_t>(n));
+    for (int i = 0; i < n; ++i) keys[size_t(i)] = {h.cellId[slot[i]], i};
+    std::sort(keys.begin(), keys.end());
     std::vector<int> order(n);
-    std::iota(order.begin(), order.end(), 0);
-    std::stable_sort(order.begin(), order.end(),
-                     [&](int a, int b) { return h.cellId[slot[a]] < h.cellId[slot[b]]; });
+    for (int k = 0; k < n; ++k) order[k] = keys[size_t(k)].second;
     std::vector<int> posOf(n);
     for (int k = 0; k < n; ++k) posOf[order[k]] = k;
     cells.assign(n, alien_cell_to{});
@@ -280,10 +282,11 @@ void WorldImage::toTO(std::vector<alien_cell_to>& cells, std::vector<alien_parti
         c.genomeComplexity = h.genomeComplexity[sl];
     }
     const int np = numParticles;
+    std::vector<std::pair<uint64_t, int>> pkeys(static_cast<size_t>(np));
+    for (int i = 0; i < np; ++i) pkeys[size_t(i)] = {h.particleId[pslot[i]], i};
+    std::sort(pkeys.begin(), pkeys.end());
     std::vector<int> po(np);
-    std::iota(po.begin(), po.end(), 0);
-    std::stable_sort(po.begin(), po.end(),
-                     [&](int a, int b) { return h.particleId[pslot[a]] < h.particleId[pslot[b]]; });
+    for (int k = 0; k < np; ++k) po[k] = pkeys[size_t(k)].second;
     particles.assign(np, alien_particle_to{});
     for (int k = 0; k < np; ++k) {
         const int i = po[k];
@@ -942,6 +945,18 @@ int WorldSet::numParticles(int w) {
     DevCounters c{};
     check_hip(hipMemcpy(&c, dCtr_ + w, sizeof(c), hipMemcpyDeviceToHost), "D2H");
     return c.numParticles;
+}
+
+int WorldSet::liveParticles(int w) {
+    synchronize();
+    const int np = numParticles(w);
+    std::vector<uint8_t> f(static_cast<size_t>(std::max(np, 0)));
+    if (np > 0)
+        check_hip(hipMemcpy(f.data(), buf_[parity_].pflags + worlds_.at(w).partOff, size_t(np), hipMemcpyDeviceToHost),
+                  "D2H");
+    int live = 0;
+    for (uint8_t x : f) live += (x & kParticleDead) ? 0 : 1;
+    return live;
 }
 
 uint64_t WorldSet::timestep(int w) {

@@ -99,7 +99,8 @@ public:
     HostWorld& host(int w) { return hosts_.at(w); }
     const HostWorld& host(int w) const { return hosts_.at(w); }
     int numCells(int w) const { return worlds_.at(w).numCells; }
-    int numParticles(int w);
+    int numParticles(int w);   // live range, may still hold particles killed in the last step
+    int liveParticles(int w);  // the particles get_data returns
     uint64_t timestep(int w);
     void setTimestep(int w, uint64_t t);
     void setParams(int w, const alien_params& p);

@@ -222,6 +222,42 @@ def test_particles_only_and_cells_only():
     run_both(tiny_world(c, H.particles(0)), [1, 10])
 
 
+def test_particle_kernels_stop_when_none_is_left_and_return_with_new_data():
+    """Once every particle is absorbed the particle kernels are no longer launched; the run stays
+    identical to the oracle, and particles set again afterwards are simulated again."""
+    rng = np.random.default_rng(21)
+    c = H.cells(1500)
+    c["pos"] = rng.uniform(0, 64, size=(1500, 2)).astype(np.float32)
+    c["vel"] = rng.normal(0, 0.05, size=(1500, 2)).astype(np.float32)
+    c["maxConnections"] = 0
+    pa = H.particles(40)
+    pa["pos"] = c["pos"][:40]  # each particle starts on a cell
+    pa["energy"] = 2.0
+    w = tiny_world(c, pa, motion=MOTION_FLUID)
+    sim = engine.Simulation.from_world(w)
+    ref = from_world(w, threads=CPU_THREADS)
+    steps = 0
+    while sim.counts()[1] > 0 and steps < 300:
+        sim.step(10)
+        ref.step(10)
+        steps += 10
+    assert sim.counts()[1] == 0, "particles were expected to be absorbed"
+    sim.step(70)  # long enough for the in-call check (every 32 steps) as well
+    ref.step(70)
+    assert_identical(sim.get_data(), ref.get_data(), "after the last particle")
+    gc, _ = sim.get_data()
+    pb = H.particles(30, first_id=int(gc["id"].max()) + 1)
+    pb["pos"] = gc["pos"][:30]
+    pb["vel"] = rng.normal(0, 0.2, size=(30, 2)).astype(np.float32)
+    pb["energy"] = 3.0
+    sim.set_data(gc, pb)
+    ref.set_data(gc, pb)
+    sim.step(6)
+    ref.step(6)
+    assert_identical(sim.get_data(), ref.get_data(), "after new particles")
+    assert sim.counts()[1] < 30  # the new particles were absorbed or moved: they are simulated
+
+
 @pytest.mark.parametrize("motion", [MOTION_FLUID, MOTION_COLLISION])
 def test_seam_and_odd_sizes(motion):
     """Cells on and across the torus seam, a world whose side is not a multiple of the bin edge
