@@ -35,8 +35,8 @@ __global__ void k_part_bin(Ctx c) {
     });
 }
 
-// exclusive scan of count[0..n) into start[0..n], clearing count: tile sums, one-block scan of
-// the tile sums, tile scans
+// exclusive scan of the bin sizes into start[0..n]: tile totals, then tile scans that each add
+// the totals of the tiles before them
 constexpr int kScanThreads = 256;
 constexpr int kScanItems = 8;
 constexpr int kScanTile = kScanThreads * kScanItems;
@@ -63,37 +63,18 @@ __device__ __forceinline__ void load8(const uint32_t* count, const uint32_t* pri
     }
 }
 
+// tile totals; also opens the step: clears the per-step list counters
 __global__ void __launch_bounds__(kScanThreads) k_scan_reduce(const uint32_t* count, const uint32_t* priv, int nCell,
-                                                              uint32_t* blockSums, int n) {
+                                                              uint32_t* blockSums, int n, StepCounters* sc) {
     const size_t base = size_t(blockIdx.x) * kScanTile + size_t(threadIdx.x) * kScanItems;
     uint32_t v[8];
     load8(count, priv, nCell, base, n, v);
     uint32_t s = v[0] + v[1] + v[2] + v[3] + v[4] + v[5] + v[6] + v[7];
     uint32_t total;
     block_exclusive_scan(s, total);
-    if (threadIdx.x == 0) blockSums[blockIdx.x] = total;
-}
-
-// also opens the step: clears the per-step list counters
-__global__ void __launch_bounds__(1024) k_scan_top(uint32_t* blockSums, int numTiles, uint32_t* start, int n,
-                                                   StepCounters* sc) {
-    const int per = (numTiles + int(blockDim.x) - 1) / int(blockDim.x);
-    const int b0 = int(threadIdx.x) * per;
-    uint32_t s = 0;
-    for (int k = 0; k < per; ++k)
-        if (b0 + k < numTiles) s += blockSums[b0 + k];
-    uint32_t total;
-    uint32_t off = block_exclusive_scan(s, total);
-    for (int k = 0; k < per; ++k) {
-        if (b0 + k < numTiles) {
-            uint32_t t = blockSums[b0 + k];
-            blockSums[b0 + k] = off;
-            off += t;
-        }
-    }
     if (threadIdx.x == 0) {
-        start[n] = total;
-        if (sc) {
+        blockSums[blockIdx.x] = total;
+        if (sc && blockIdx.x == 0) {
             sc->staged = 0;
             sc->candidates = 0;
             sc->liveParticles = 0;
@@ -101,16 +82,22 @@ __global__ void __launch_bounds__(1024) k_scan_top(uint32_t* blockSums, int numT
     }
 }
 
-// writes start[]; `count` stays: the scatter counts it back down to zero, which leaves it cleared
-// for the step after next (the two histograms alternate)
+// writes start[]: every tile adds the totals of the tiles before it (a few thousand words, read
+// from L2) to its own exclusive scan.  `count` stays: the scatter counts it back down to zero,
+// which leaves it cleared for the step after next (the two histograms alternate)
 __global__ void __launch_bounds__(kScanThreads) k_scan_down(const uint32_t* count, const uint32_t* priv, int nCell,
                                                             const uint32_t* blockSums, uint32_t* start, int n) {
+    uint32_t before = 0;
+    for (int t = int(threadIdx.x); t < int(blockIdx.x); t += kScanThreads) before += blockSums[t];
+    uint32_t prefix;
+    block_exclusive_scan(before, prefix);
     const size_t base = size_t(blockIdx.x) * kScanTile + size_t(threadIdx.x) * kScanItems;
     uint32_t v[8];
     load8(count, priv, nCell, base, n, v);
     uint32_t s = v[0] + v[1] + v[2] + v[3] + v[4] + v[5] + v[6] + v[7];
     uint32_t total;
-    uint32_t off = block_exclusive_scan(s, total) + blockSums[blockIdx.x];
+    uint32_t off = block_exclusive_scan(s, total) + prefix;
+    if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) start[n] = prefix + total;
     uint32_t o[8];
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
@@ -253,8 +240,7 @@ void launch_seed_bins(const Ctx& c, const LaunchShape& sh, hipStream_t s) {
 void launch_scan(const Ctx& c, hipStream_t s, StepCounters* sc) {
     const int tiles = (c.numBins + kScanTile - 1) / kScanTile;
     hipLaunchKernelGGL(k_scan_reduce, dim3(tiles), dim3(kScanThreads), 0, s, c.count, c.priv, c.numCellBins,
-                       c.blockSums, c.numBins);
-    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, s, c.blockSums, tiles, c.start, c.numBins, sc);
+                       c.blockSums, c.numBins, sc);
     hipLaunchKernelGGL(k_scan_down, dim3(tiles), dim3(kScanThreads), 0, s, c.count, c.priv, c.numCellBins,
                        c.blockSums, c.start, c.numBins);
 }

@@ -22,7 +22,7 @@ void launch_step(const Ctx& c, const LaunchShape& sh, hipStream_t s, Recorder* r
     launch_scan(c, s, c.sc);
     if (rec) rec->mark("cell_scatter");
     launch_cell_scatter(c, sh, s);
-    if (rec) rec->mark("part_sort");
+    if (rec && sh.hasParticles) rec->mark("part_sort");
     launch_part_sort(c, sh, s);
     if (rec) rec->mark("neighbor");
     launch_neighbor(c, sh, s, sh.hasFluid, sh.hasCollision);
