@@ -60,10 +60,12 @@ __device__ __forceinline__ bool is_neighbor(const Nb& s, const Nb& o, float Wf, 
         return o.aidx != s.aidx && lensq(sub(o.pos, s.pos)) <= reachSq;
 }
 
-// The contribution of neighbour o to cell s, for a pair that passed is_neighbor.
-template <bool kFluid>
+// The contribution of neighbour o to cell s, for a pair that passed is_neighbor.  The logical
+// indices of the two cells are needed only for a fusion candidate: lidxOf() returns them as
+// (s, o) when asked.
+template <bool kFluid, class LidxOf>
 __device__ __forceinline__ uint32_t pair_term(const Ctx& c, const DevWorld& w, const DevParams& P, int wi, const Nb& s,
-                                              const Nb& o, bool bonded, float* t, bool seam = true) {
+                                              const Nb& o, bool bonded, float* t, bool seam, LidxOf&& lidxOf) {
     const float Wf = float(w.W), Hf = float(w.H);
     const float cmd = P.minDistance;
     const bool bari = flags_barrier(s.flags);
@@ -99,10 +101,13 @@ __device__ __forceinline__ uint32_t pair_term(const Ctx& c, const DevWorld& w, c
             f |= kPairForce;
         }
         if (flags_num_conn(s.flags) < flags_max_conn(s.flags) && flags_num_conn(o.flags) < flags_max_conn(o.flags) &&
-            !bari && s.lidx < o.lidx && lensq(vd) > P.fusionBelowSq) {
-            const float ei = c.energy[w.cellOff + c.A.slot[s.aidx]];
-            const float ej = c.energy[w.cellOff + c.A.slot[o.aidx]];
-            if (ei <= P.maxBindingEnergy && ej <= P.maxBindingEnergy) push_candidate(c, s.lidx, o.lidx, wi);
+            !bari && lensq(vd) > P.fusionBelowSq) {
+            const int2 L = lidxOf();
+            if (L.x < L.y) {
+                const float ei = c.energy[w.cellOff + c.A.slot[s.aidx]];
+                const float ej = c.energy[w.cellOff + c.A.slot[o.aidx]];
+                if (ei <= P.maxBindingEnergy && ej <= P.maxBindingEnergy) push_candidate(c, L.x, L.y, wi);
+            }
         }
     } else {
         // CellProcessor.cuh:297-361.  f(j,i) mirrors f(i,j): unit(-d) = -unit(d) and
@@ -134,10 +139,13 @@ __device__ __forceinline__ uint32_t pair_term(const Ctx& c, const DevWorld& w, c
         t[5] = nfji.y;
         f |= kPairForce;
         if (flags_num_conn(s.flags) < flags_max_conn(s.flags) && flags_num_conn(o.flags) < flags_max_conn(o.flags) &&
-            appr && !bari && !barj && s.lidx < o.lidx && lensq(vd) > P.fusionBelowSq) {
-            const float ei = c.energy[w.cellOff + c.A.slot[s.aidx]];
-            const float ej = c.energy[w.cellOff + c.A.slot[o.aidx]];
-            if (ei <= P.maxBindingEnergy && ej <= P.maxBindingEnergy) push_candidate(c, s.lidx, o.lidx, wi);
+            appr && !bari && !barj && lensq(vd) > P.fusionBelowSq) {
+            const int2 L = lidxOf();
+            if (L.x < L.y) {
+                const float ei = c.energy[w.cellOff + c.A.slot[s.aidx]];
+                const float ej = c.energy[w.cellOff + c.A.slot[o.aidx]];
+                if (ei <= P.maxBindingEnergy && ej <= P.maxBindingEnergy) push_candidate(c, L.x, L.y, wi);
+            }
         }
     }
     return f;
@@ -190,6 +198,37 @@ __device__ __forceinline__ void acc_add(Acc& a, uint32_t f, const float* t) {
             a.F = add(a.F, f2{t[2], t[3]});
             a.F = add(a.F, f2{t[4], t[5]});
         }
+    }
+}
+
+// acc_add without branches, for terms zeroed before pair_term wrote them (lanes with !on add
+// nothing).  Adding a zero term leaves a sum that started at +0 unchanged bit for bit (such a sum
+// is never -0), so the flags select only what is not a plain zero: a barrier neighbour's data, and
+// the collision position sum, which starts at the cell's position.
+template <bool kFluid>
+__device__ __forceinline__ void acc_add_sel(Acc& a, bool on, uint32_t f, const float* t) {
+    if constexpr (kFluid) {
+        const bool bar = on && (f & kPairBarrier) != 0u;
+        const bool sum = on && !bar;
+        const bool near = bar && a.numBarrier < 10 && (a.cb < 0 || t[1] < a.cbDist);
+        a.cb = near ? __float_as_int(t[0]) : a.cb;
+        a.cbDist = near ? t[1] : a.cbDist;
+        a.numBarrier += bar ? 1 : 0;
+        a.rho = sum ? a.rho + t[0] : a.rho;
+        a.D.x = sum ? a.D.x + t[1] : a.D.x;
+        a.D.y = sum ? a.D.y + t[2] : a.D.y;
+        a.Fp.x = on ? a.Fp.x + t[3] : a.Fp.x;
+        a.Fp.y = on ? a.Fp.y + t[4] : a.Fp.y;
+        a.Fv.x = on ? a.Fv.x + t[5] : a.Fv.x;
+        a.Fv.y = on ? a.Fv.y + t[6] : a.Fv.y;
+    } else {
+        const bool ov = on && (f & kPairOverlap) != 0u;
+        a.pc.x = ov ? a.pc.x + t[0] : a.pc.x;
+        a.pc.y = ov ? a.pc.y + t[1] : a.pc.y;
+        f2 F = add(a.F, f2{t[2], t[3]});
+        F = add(F, f2{t[4], t[5]});
+        a.F.x = on ? F.x : a.F.x;
+        a.F.y = on ? F.y : a.F.y;
     }
 }
 
@@ -337,7 +376,8 @@ __device__ __forceinline__ void cell_direct(const Ctx& c, const DevWorld& w, con
     forEachNb([&](const Nb& o) {
         if (!is_neighbor<kFluid>(s, o, Wf, Hf, P.reachSq)) return;
         float t[kPairTerms];
-        const uint32_t f = pair_term<kFluid>(c, w, P, wi, s, o, is_bonded(bt, o.aidx - w.cellOff), t);
+        const uint32_t f = pair_term<kFluid>(c, w, P, wi, s, o, is_bonded(bt, o.aidx - w.cellOff), t, true,
+                                             [&] { return make_int2(s.lidx, o.lidx); });
         acc_add<kFluid>(a, f, t);
     });
     cell_finish<kFluid>(c, w, P, wi, s, c.A.slot[s.aidx], bt, a, R);
@@ -429,8 +469,17 @@ __device__ __forceinline__ int seg_first(int seg, int HX) { return seg == 0 ? 0 
 template <int kSub>
 __device__ __forceinline__ int slot_row(int j, int HX) { return kSub == 2 && j >= HX ? 1 : 0; }
 
-// the staged entry at pool position pp (already reduced modulo kPool) of ring row r
+// logical index of the staged entry at pool position pp (already reduced modulo kPool) of ring row r
 template <int kSub, class RS>
+__device__ __forceinline__ int row_lidx(const RS& S, const StripGeom& g, int r, int pp) {
+    const int j = S.sbin[pp];
+    const int t = slot_row<kSub>(j, g.HX);
+    const int seg = seg_of(j - t * g.HX, g.HX);
+    return S.rm[r][t][seg] + ((pp - S.rs[r][t * g.HX + seg_first(seg, g.HX)]) & kPoolMask);
+}
+
+// the staged entry at pool position pp of ring row r; without its logical index if !kLidx
+template <int kSub, class RS, bool kLidx = true>
 __device__ __forceinline__ Nb row_nb(const RS& S, const StripGeom& g, int r, int pp) {
     Nb o;
     o.pos = to_f2(S.pos[pp]);
@@ -438,10 +487,7 @@ __device__ __forceinline__ Nb row_nb(const RS& S, const StripGeom& g, int r, int
     o.dens = S.dens[pp];
     o.flags = S.flags[pp];
     o.aidx = S.aidx[pp];
-    const int j = S.sbin[pp];
-    const int t = slot_row<kSub>(j, g.HX);
-    const int seg = seg_of(j - t * g.HX, g.HX);
-    o.lidx = S.rm[r][t][seg] + ((pp - S.rs[r][t * g.HX + seg_first(seg, g.HX)]) & kPoolMask);
+    o.lidx = kLidx ? row_lidx<kSub>(S, g, r, pp) : -1;
     return o;
 }
 
@@ -719,9 +765,12 @@ __device__ __forceinline__ void row_compute(const Ctx& c, const DevWorld& w, con
                 uint32_t f = 0u;
                 if (p < wend) {
                     const uint32_t rec = S.u.pr.pnb[p - W0];
-                    const Nb s = row_nb<kSub>(S, g, rc, S.u.pr.pself[p - W0]);
-                    const Nb o = row_nb<kSub>(S, g, int((rec >> 8) & 3u), int(rec & 0xFFu));
-                    f = pair_term<kFluid>(c, w, P, wi, s, o, (rec >> 15) != 0u, tm, g.seam);
+                    const int ps = S.u.pr.pself[p - W0], ro = int((rec >> 8) & 3u), po = int(rec & 0xFFu);
+                    const Nb s = row_nb<kSub, RS, false>(S, g, rc, ps);
+                    const Nb o = row_nb<kSub, RS, false>(S, g, ro, po);
+                    f = pair_term<kFluid>(c, w, P, wi, s, o, (rec >> 15) != 0u, tm, g.seam, [&] {
+                        return make_int2(row_lidx<kSub>(S, g, rc, ps), row_lidx<kSub>(S, g, ro, po));
+                    });
                 }
                 // the owner adds its pairs of this round in visiting order, reading each from the
                 // lane that evaluated it (every lane takes part in every exchange)
@@ -733,10 +782,9 @@ __device__ __forceinline__ void row_compute(const Ctx& c, const DevWorld& w, con
 #pragma unroll
                     for (int v = 0; v < kPairTerms; ++v) tt[v] = __shfl(tm[v], src, kRowThreads);
                     const uint32_t ff = uint32_t(__shfl(int(f), src, kRowThreads));
-                    if (j < n) {
-                        if (lo + j - pbase == selfIdx) a.rho = a.rho + w0;
-                        acc_add<kFluid>(a, ff, tt);
-                    }
+                    const bool on = j < n;
+                    if constexpr (kFluid) a.rho = on && lo + j - pbase == selfIdx ? a.rho + w0 : a.rho;
+                    acc_add_sel<kFluid>(a, on, ff, tt);
                 }
             }
             __syncthreads();
@@ -890,8 +938,10 @@ __global__ void __launch_bounds__(kRowThreads) k_neighbor_rows(Ctx c) {
     const int2 tb = c.tileBlocks[blockIdx.x];
     if (tb.x < 0) return;
     const int wi = tb.x;
-    const DevWorld& w = c.worlds[wi];
-    const DevParams& P = c.params[wi];
+    // the world and its parameters are read once, before any store: read through references,
+    // every use after a global store would load them again
+    const DevWorld w = c.worlds[wi];
+    const DevParams P = c.params[wi];
     if ((P.motion == 0) != kFluid) return;
     const int strip = tb.y % w.ntx, chunk = tb.y / w.ntx;
     StripGeom g;
