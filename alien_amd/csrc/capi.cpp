@@ -337,12 +337,16 @@ struct alien_batch {
     std::unique_ptr<WorldSet> set;
     int resident = 0;
     int blockSize = 256;
+    int stripCap = kStripCapMax;  // launch shape chosen by the autotuner, kept when worlds are added
+    int chunkRows = 0;
     bool useGraph = true;
     bool timing = false;
     void flush() {
         if (pending.empty()) return;
         auto fresh = std::make_unique<WorldSet>();
         fresh->setBlockSize(blockSize);
+        fresh->setStripCap(stripCap);
+        fresh->setChunkRows(chunkRows);
         fresh->setUseGraph(useGraph);
         fresh->setKernelTiming(timing);
         std::vector<WorldSet::Source> src;
@@ -470,7 +474,7 @@ int alien_batch_get_kernel_timing(alien_batch* b, alien_kernel_time* out, int32_
 }
 
 int alien_batch_autotune(alien_batch* b, uint32_t trialSteps, int32_t* blockSize, int32_t* stripCap,
-                         double* msPerStep) {
+                         int32_t* chunkRows, double* msPerStep) {
     return guarded([&] {
         require(b != nullptr, "null batch");
         require(trialSteps > 0, "trialSteps must be positive");
@@ -486,26 +490,45 @@ int alien_batch_autotune(alien_batch* b, uint32_t trialSteps, int32_t* blockSize
         trial.setUseGraph(b->useGraph);
         trial.build(src);
         double best = 1e300;
-        int bestBlock = b->blockSize, bestStrip = b->set->stripCap();
+        int bestBlock = b->blockSize, bestStrip = b->set->stripCap(), bestRows = 0;
+        auto time = [&]() {
+            trial.step(1);  // captures the graphs of this shape
+            const double t0 = nowSeconds();
+            trial.step(trialSteps);
+            return (nowSeconds() - t0) / double(trialSteps);
+        };
+        // block size x strip cap with the default chunk rows, then the chunk rows for the best
+        trial.setChunkRows(0);
         for (int blk : {64, 128, 256})
             for (int sc : {kStripCapMax, 32, 16}) {
                 trial.setBlockSize(blk);
                 trial.setStripCap(sc);
-                trial.step(1);  // captures the graphs of this shape
-                const double t0 = nowSeconds();
-                trial.step(trialSteps);
-                const double dt = (nowSeconds() - t0) / double(trialSteps);
+                const double dt = time();
                 if (dt < best) {
                     best = dt;
                     bestBlock = blk;
                     bestStrip = sc;
                 }
             }
+        trial.setBlockSize(bestBlock);
+        trial.setStripCap(bestStrip);
+        for (int rows : {2, 4, 8, 16}) {
+            trial.setChunkRows(rows);
+            const double dt = time();
+            if (dt < best) {
+                best = dt;
+                bestRows = rows;
+            }
+        }
         b->blockSize = bestBlock;
+        b->stripCap = bestStrip;
+        b->chunkRows = bestRows;
         b->set->setBlockSize(bestBlock);
         b->set->setStripCap(bestStrip);
+        b->set->setChunkRows(bestRows);
         if (blockSize) *blockSize = bestBlock;
         if (stripCap) *stripCap = bestStrip;
+        if (chunkRows) *chunkRows = bestRows;
         if (msPerStep) *msPerStep = best * 1000.0;
     });
 }

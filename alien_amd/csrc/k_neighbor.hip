@@ -277,10 +277,12 @@ __device__ __forceinline__ uint32_t hist_count(const uint32_t* hist, int b) {
     return (hist[b >> 1] >> (uint32_t(b & 1) << 4)) & 0xFFFFu;
 }
 
-// Barrier reflection, applyForces, and the end-of-step writes of one cell.
+// Barrier reflection, applyForces, and the end-of-step writes of one cell.  A bonded or barrier
+// cell goes to the staged list at stagedPos, or at a place taken here if stagedPos < 0.
 template <bool kFluid>
 __device__ __forceinline__ void cell_finish(const Ctx& c, const DevWorld& w, const DevParams& P, int wi, const Nb& s,
-                                            int slot, const int bt[kMaxBonds], Acc& a, const Region& R) {
+                                            int slot, const int bt[kMaxBonds], Acc& a, const Region& R,
+                                            int stagedPos = -1) {
     const int cap = c.cellCap;
     const float Wf = float(w.W), Hf = float(w.H);
     const int i = s.aidx, k = s.lidx;
@@ -330,15 +332,33 @@ __device__ __forceinline__ void cell_finish(const Ctx& c, const DevWorld& w, con
     c.B.flags[k] = fi;
     c.B.slot[k] = slot;
     c.B.dens[k] = kFluid ? a.rho : s.dens;
-    for (int b = 0; b < nci; ++b) {
-        c.B.btgt[b * cap + k] = bt[b];
-        c.B.brest[b * cap + k] = c.A.brest[b * cap + i];
-        c.B.bang[b * cap + k] = c.A.bang[b * cap + i];
+    if constexpr (kFluid) {
+        for (int b = 0; b < nci; ++b) {
+            c.B.btgt[b * cap + k] = bt[b];
+            c.B.brest[b * cap + k] = c.A.brest[b * cap + i];
+            c.B.bang[b * cap + k] = c.A.bang[b * cap + i];
+        }
+        if (nci > 0) c.bstiff[k] = c.stiff[w.cellOff + slot];
+    } else {
+        // (collision worlds are mostly bonded cells: their bond data is loaded before any of it
+        // is stored, so that the loads are in flight together)
+        float brest[kMaxBonds], bang[kMaxBonds];
+#pragma unroll
+        for (int b = 0; b < kMaxBonds; ++b) {
+            brest[b] = b < nci ? c.A.brest[b * cap + i] : 0.0f;
+            bang[b] = b < nci ? c.A.bang[b * cap + i] : 0.0f;
+        }
+        const float stiff = nci > 0 ? c.stiff[w.cellOff + slot] : 0.0f;
+#pragma unroll
+        for (int b = 0; b < kMaxBonds; ++b)
+            if (b < nci) {
+                c.B.btgt[b * cap + k] = bt[b];
+                c.B.brest[b * cap + k] = brest[b];
+                c.B.bang[b * cap + k] = bang[b];
+            }
+        if (nci > 0) c.bstiff[k] = stiff;
     }
-    if (nci > 0) {
-        c.newidx[i] = k;
-        c.bstiff[k] = c.stiff[w.cellOff + slot];
-    }
+    if (nci > 0) c.newidx[i] = k;
     f2 v = vi;
     if (!bari) {  // applyForces
         v = add(v, F);
@@ -359,7 +379,7 @@ __device__ __forceinline__ void cell_finish(const Ctx& c, const DevWorld& w, con
     } else {
         c.B.pos[k] = to_float2(pc);
         c.B.vel[k] = to_float2(v);
-        const int q = atomicAdd(&c.sc->staged, 1);
+        const int q = stagedPos >= 0 ? stagedPos : atomicAdd(&c.sc->staged, 1);
         c.staged[q] = make_int2(k, wi);
     }
 }
@@ -367,7 +387,7 @@ __device__ __forceinline__ void cell_finish(const Ctx& c, const DevWorld& w, con
 // One cell, neighbours visited one after another (forEachNb calls visit(o) in stencil order).
 template <bool kFluid, class ForEach>
 __device__ __forceinline__ void cell_direct(const Ctx& c, const DevWorld& w, const DevParams& P, int wi, const Nb& s,
-                                            const Region& R, ForEach&& forEachNb) {
+                                            const Region& R, ForEach&& forEachNb, int stagedPos = -1) {
     const float Wf = float(w.W), Hf = float(w.H);
     int bt[kMaxBonds];
     load_bonds(c, s, bt);
@@ -380,7 +400,7 @@ __device__ __forceinline__ void cell_direct(const Ctx& c, const DevWorld& w, con
                                              [&] { return make_int2(s.lidx, o.lidx); });
         acc_add<kFluid>(a, f, t);
     });
-    cell_finish<kFluid>(c, w, P, wi, s, c.A.slot[s.aidx], bt, a, R);
+    cell_finish<kFluid>(c, w, P, wi, s, c.A.slot[s.aidx], bt, a, R, stagedPos);
 }
 
 
@@ -631,6 +651,9 @@ __device__ __forceinline__ void row_compute(const Ctx& c, const DevWorld& w, con
         unsigned long long mask = 0ull, bmask = 0ull;
         bool direct = false;
         int selfIdx = -1;  // fluid: number of the cell's pairs that come before its own density term
+        // the staged (bonded or barrier) cells of this batch take their list places up front, so
+        // that the atomic's round trip overlaps the pair work
+        int stBase = 0;
         const int t = kSub == 2 && q >= n0 ? 1 : 0;    // the cell's bin row in the staged row
         const int su = t == 0 ? cs0 + q : cs1 + (q - n0);  // pool position, not wrapped
         const int sp = su & kPoolMask;
@@ -639,6 +662,9 @@ __device__ __forceinline__ void row_compute(const Ctx& c, const DevWorld& w, con
             self = row_nb<kSub>(S, g, rc, sp);
             slot = c.A.slot[self.aidx];  // needed only at the end: loaded early
             load_bonds(c, self, bt);
+            // (lane 0 is active whenever any lane is: the active lanes are the lowest ones)
+            const unsigned long long stBal = __ballot(flags_barrier(self.flags) || flags_num_conn(self.flags) > 0);
+            if (lane == 0 && stBal != 0ull) stBase = atomicAdd(&c.sc->staged, int(__popcll(stBal)));
             const int hx = S.sbin[sp] - t * g.HX;
             // stencil bin rows (ring row, bin row): above, own, below
             int subOf[3] = {0, 0, 0};
@@ -790,17 +816,22 @@ __device__ __forceinline__ void row_compute(const Ctx& c, const DevWorld& w, con
             __syncthreads();
         }
         if (active && selfIdx == count) a.rho = a.rho + w0;  // the own term after all pairs
+        const unsigned long long stBal = __ballot(active && (flags_barrier(self.flags) || flags_num_conn(self.flags) > 0));
+        const int stPos = __shfl(stBase, 0, kRowThreads) + int(__popcll(stBal & ((1ull << lane) - 1ull)));
         if (active) {
             if (direct) {
                 atomicAdd(&c.sc->directCells, 1ull);
-                cell_direct<kFluid>(c, w, P, wi, self, R, [&](auto&& visit) {
+                cell_direct<kFluid>(
+                    c, w, P, wi, self, R,
+                    [&](auto&& visit) {
 #pragma unroll
-                    for (int r = 0; r < 3; ++r)
-                        for (int u = a0s[r]; u < a0s[r] + ns[r]; ++u)
-                            visit(row_nb<kSub>(S, g, ringOf[r], u & kPoolMask));
-                });
+                        for (int r = 0; r < 3; ++r)
+                            for (int u = a0s[r]; u < a0s[r] + ns[r]; ++u)
+                                visit(row_nb<kSub>(S, g, ringOf[r], u & kPoolMask));
+                    },
+                    stPos);
             } else {
-                cell_finish<kFluid>(c, w, P, wi, self, slot, bt, a, R);
+                cell_finish<kFluid>(c, w, P, wi, self, slot, bt, a, R, stPos);
             }
         }
     }
@@ -809,7 +840,7 @@ __device__ __forceinline__ void row_compute(const Ctx& c, const DevWorld& w, con
 // A chunk with a row too dense for LDS: every cell walks its stencil in global memory, entries of
 // a bin by increasing previous index (selection over the bin), same order and results.
 template <bool kFluid>
-__device__ void rows_direct(const Ctx& c, const DevWorld& w, const DevParams& P, int wi, const StripGeom& g, int y0,
+__device__ __forceinline__ void rows_direct(const Ctx& c, const DevWorld& w, const DevParams& P, int wi, const StripGeom& g, int y0,
                             int IY) {
     const uint32_t base = g.base;
     const Region R{nullptr, 0, 0, 0, 0};  // every cell in the shared histogram

@@ -630,7 +630,7 @@ void WorldSet::build(const std::vector<Source>& sources) {
     active_.assign(worlds_.size(), 1);
     stripChoice_ = std::move(stripChoice);
     stageChoice_ = std::move(stageChoice);
-    chunk_shape(worlds_, params_);
+    chunk_shape(worlds_, params_, chunkRows_);
     shape_.hasBondWork = false;
     for (uint8_t b : bondWork_) shape_.hasBondWork = shape_.hasBondWork || b != 0;
     shape_.hasParticles = partOff > 0;
@@ -726,12 +726,13 @@ void strip_shape(DevWorld& w, int strip, int rowsPerStage) {
     w.nty = w.nby / rowsPerStage;
 }
 
-// Rows per neighbour-pass chunk, one value for all worlds of a set.  A chunk stages two halo rows
-// besides its own (about kHaloCost computed rows' worth of work); its workgroups run in rounds of
-// as many as the chip keeps resident, and a partly filled last round costs a full one.  The rows
-// minimise rounds x (rows + kHaloCost).
-void chunk_shape(std::vector<DevWorld>& worlds, const std::vector<DevParams>& params) {
-    constexpr double kHaloCost = 0.6;
+// Rows per neighbour-pass chunk, one value for all worlds of a set (forcedRows if nonzero).  Tall
+// chunks stage fewer halo rows per computed row; short ones even out the work of the workgroups,
+// whose cell counts vary.  Measured on MI355X, the pass ran fastest with the tallest chunks that
+// still give at least kMinRounds workgroups per resident slot (4M-cell fluid world: 6 rows,
+// 3.6 rounds, 3 % faster than at 2 rounds); sets too small for that take the shortest chunks.
+void chunk_shape(std::vector<DevWorld>& worlds, const std::vector<DevParams>& params, int forcedRows) {
+    constexpr double kMinRounds = 3.5;
     static const int numCUs = [] {
         int dev = 0, n = 0;
         if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
@@ -746,7 +747,6 @@ void chunk_shape(std::vector<DevWorld>& worlds, const std::vector<DevParams>& pa
     if (!fluid && !collision) perCU = 1;
     const int64_t slots = int64_t(perCU) * numCUs;
     int best = 2;
-    double bestCost = 1e300;
     // staged rows per chunk of a world for `rows` bin rows (a chunk covers at most kChunkMax bin
     // rows: the LDS histogram of its region)
     auto stagedRows = [](const DevWorld& w, int rows) {
@@ -759,12 +759,9 @@ void chunk_shape(std::vector<DevWorld>& worlds, const std::vector<DevParams>& pa
             const int ty = stagedRows(w, rows), nyS = w.nby / w.rowsPerStage;
             tiles += int64_t(w.ntx) * ((nyS + ty - 1) / ty);
         }
-        const double cost = double((tiles + slots - 1) / slots) * (rows + kHaloCost);
-        if (cost < bestCost) {
-            bestCost = cost;
-            best = rows;
-        }
+        if (double(tiles) >= kMinRounds * double(slots)) best = rows;
     }
+    if (forcedRows > 0) best = forcedRows;
     for (DevWorld& w : worlds) {
         const int nyS = w.nby / w.rowsPerStage;
         w.tileY = stagedRows(w, best);
@@ -853,7 +850,16 @@ void WorldSet::setStripCap(int cap) {
     if (worlds_.empty()) return;
     for (size_t i = 0; i < worlds_.size(); ++i)
         strip_shape(worlds_[i], std::min(stripChoice_[i], cap), stageChoice_[i]);
-    chunk_shape(worlds_, params_);
+    chunk_shape(worlds_, params_, chunkRows_);
+    check_hip(hipMemcpy(dWorlds_, worlds_.data(), sizeof(DevWorld) * worlds_.size(), hipMemcpyHostToDevice), "H2D");
+    buildTables();
+}
+
+void WorldSet::setChunkRows(int rows) {
+    if (rows != 0 && (rows < 2 || rows > kChunkMax)) throw Error(ALIEN_E_ARG, "chunk rows must be 0 or 2..16");
+    chunkRows_ = rows;
+    if (worlds_.empty()) return;
+    chunk_shape(worlds_, params_, chunkRows_);
     check_hip(hipMemcpy(dWorlds_, worlds_.data(), sizeof(DevWorld) * worlds_.size(), hipMemcpyHostToDevice), "H2D");
     buildTables();
 }

@@ -35,7 +35,7 @@ void validate_world(const alien_world_desc& d, const alien_params& p);
 DevParams to_dev_params(const alien_params& p);
 int choose_strip(const std::vector<float2>& pos, int W, int H, int binSize, bool twoRowStages, int& rowsPerStage);
 void strip_shape(DevWorld& w, int strip, int rowsPerStage);
-void chunk_shape(std::vector<DevWorld>& worlds, const std::vector<DevParams>& params);
+void chunk_shape(std::vector<DevWorld>& worlds, const std::vector<DevParams>& params, int forcedRows = 0);
 constexpr int kXcds = 8;  // accelerator complex dies of an MI355X
 
 // host-side identity and pass-through data of one world (fields the physics path never touches)
@@ -122,6 +122,9 @@ public:
     // all worlds are active after build.
     void setActive(int w, bool on);
     int stripCap() const { return stripCap_; }
+    // bin rows per neighbour-pass chunk (2..16), or 0 for the occupancy rule; launch shape only
+    void setChunkRows(int rows);
+    int chunkRows() const { return chunkRows_; }
     void setUseGraph(bool on);
     // kernel timing mode: no graphs, every launch bracketed by events (see alien_set_kernel_timing)
     void setKernelTiming(bool on);
@@ -198,6 +201,7 @@ private:
     std::vector<int> stripChoice_;   // per world: strip width chosen from its cell density
     std::vector<int> stageChoice_;   // per world: bin rows per staged row (1 or 2)
     int stripCap_ = kStripMax;
+    int chunkRows_ = 0;
     void adaptGrids();
     std::vector<Arr> tableAllocs_;
 

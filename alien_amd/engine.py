@@ -83,7 +83,7 @@ def lib():
         L.alien_batch_set_block_size.argtypes = [vp, i32]
         L.alien_batch_set_use_graph.argtypes = [vp, i32]
         L.alien_batch_autotune.argtypes = [vp, ctypes.c_uint32, ctypes.POINTER(i32), ctypes.POINTER(i32),
-                                           ctypes.POINTER(ctypes.c_double)]
+                                           ctypes.POINTER(i32), ctypes.POINTER(ctypes.c_double)]
         L.alien_set_kernel_timing.argtypes = [vp, i32]
         L.alien_get_kernel_timing.argtypes = [vp, ctypes.POINTER(KernelTime), i32, ctypes.POINTER(i32)]
         L.alien_get_path_counts.argtypes = [vp, pu64, pu64]
@@ -323,11 +323,12 @@ class Batch:
         check(lib().alien_batch_set_kernel_timing(self._h, 1 if on else 0))
 
     def autotune(self, trial_steps=20):
-        """Picks the fastest launch shape on a device copy; returns (block, strip cap, ms/step)."""
-        blk, strip, ms = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_double()
+        """Picks the fastest launch shape on a device copy; returns (block, strip cap, chunk rows
+        (0: the default rule), ms/step)."""
+        blk, strip, rows, ms = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32(), ctypes.c_double()
         check(lib().alien_batch_autotune(self._h, int(trial_steps), ctypes.byref(blk), ctypes.byref(strip),
-                                         ctypes.byref(ms)))
-        return blk.value, strip.value, ms.value
+                                         ctypes.byref(rows), ctypes.byref(ms)))
+        return blk.value, strip.value, rows.value, ms.value
 
     def kernel_times(self):
         return _kernel_times(lib().alien_batch_get_kernel_timing, self._h)

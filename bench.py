@@ -132,7 +132,8 @@ def bench_batch(args):
     b.step(args.steps)
     batched_s = time.perf_counter() - t0
     b.close()
-    progress(f"batched: {args.steps} steps in {batched_s:.2f} s (autotune block {tune[0]}, strip cap {tune[1]})")
+    progress(f"batched: {args.steps} steps in {batched_s:.2f} s (autotune block {tune[0]}, strip cap {tune[1]}, "
+             f"chunk rows {tune[2]})")
     # per-world sequential: each world alone, same steps, same library
     seq_s = 0.0
     for w in ws:
@@ -156,7 +157,7 @@ def bench_batch(args):
                    "seed": worlds.SEED_BASE + 3, "device": engine.gpu_name()},
         "per_world_sequential": {"value": seq_value, "seconds": seq_s},
         "batched_over_sequential": value / seq_value,
-        "autotune": {"block": tune[0], "strip_cap": tune[1], "trial_ms_per_step": tune[2]},
+        "autotune": {"block": tune[0], "strip_cap": tune[1], "chunk_rows": tune[2], "trial_ms_per_step": tune[3]},
         "generation_s": gen_s,
     }), flush=True)
 

@@ -223,11 +223,12 @@ uint64_t alien_batch_get_timestep(const alien_batch* b, int32_t world);
 int alien_batch_set_block_size(alien_batch* b, int32_t threads);
 int alien_batch_set_use_graph(alien_batch* b, int32_t enable);
 /* Autotuner (no counterpart in ALIEN): times trialSteps steps of a device copy of the batch for
- * every candidate launch shape (cell-kernel block size 64/128/256 x neighbour-pass strip cap
- * 64/32/16 bins) and keeps the fastest.  The batch's own worlds do not advance and no result
- * depends on the choice.  Outputs may be NULL. */
+ * candidate launch shapes (cell-kernel block size 64/128/256 x neighbour-pass strip cap 64/32/16
+ * bins, then bin rows per neighbour-pass chunk 2/4/8/16 against the default rule, reported as 0)
+ * and keeps the fastest, also for worlds added later.  The batch's own worlds do not advance and
+ * no result depends on the choice.  Outputs may be NULL. */
 int alien_batch_autotune(alien_batch* b, uint32_t trialSteps, int32_t* blockSize, int32_t* stripCap,
-                         double* msPerStep);
+                         int32_t* chunkRows, double* msPerStep);
 int alien_batch_set_kernel_timing(alien_batch* b, int32_t enable);
 int alien_batch_get_kernel_timing(alien_batch* b, alien_kernel_time* out, int32_t cap, int32_t* count);
 /* alien_get_path_counts for all worlds of a batch together (no counterpart in ALIEN). */
