@@ -477,12 +477,9 @@ void launch_bonds(const Ctx& c, const LaunchShape& sh, hipStream_t s, Recorder* 
         if (rec) rec->mark(name);                               \
         hipLaunchKernelGGL(kernel, grid, b, 0, s, __VA_ARGS__); \
     } while (0)
-    if (!sh.hasBondWork) {
-        // no world can have a bonded or barrier cell this step (none has one and none can fuse):
-        // only the end-of-step bookkeeping runs
-        L("clear_flags", k_clear_flags, dim3(1), c);
-        return;
-    }
+    // no world can have a bonded or barrier cell this step (none has one and none can fuse): the
+    // staged list is empty, and the scan at the start of the step advanced the timesteps
+    if (!sh.hasBondWork) return;
     L("remap", k_remap, sg, c);
     L("conn_own", k_conn_own, sg, c, 0);
     L("conn_verlet_pos", k_conn_verlet_pos, sg, c);

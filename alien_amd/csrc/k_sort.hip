@@ -63,9 +63,11 @@ __device__ __forceinline__ void load8(const uint32_t* count, const uint32_t* pri
     }
 }
 
-// tile totals; also opens the step: clears the per-step list counters
+// tile totals; also opens the step: clears the per-step list counters, and advances the worlds'
+// timesteps if `advance` is set (a step without bond work has no later reader of them)
 __global__ void __launch_bounds__(kScanThreads) k_scan_reduce(const uint32_t* count, const uint32_t* priv, int nCell,
-                                                              uint32_t* blockSums, int n, StepCounters* sc) {
+                                                              uint32_t* blockSums, int n, StepCounters* sc,
+                                                              DevCounters* advance, int numWorlds) {
     const size_t base = size_t(blockIdx.x) * kScanTile + size_t(threadIdx.x) * kScanItems;
     uint32_t v[8];
     load8(count, priv, nCell, base, n, v);
@@ -80,6 +82,8 @@ __global__ void __launch_bounds__(kScanThreads) k_scan_reduce(const uint32_t* co
             sc->liveParticles = 0;
         }
     }
+    if (advance && blockIdx.x == 0)
+        for (int wi = threadIdx.x; wi < numWorlds; wi += kScanThreads) advance[wi].timestep += 1ull;
 }
 
 // writes start[]: every tile adds the totals of the tiles before it (a few thousand words, read
@@ -237,10 +241,10 @@ void launch_seed_bins(const Ctx& c, const LaunchShape& sh, hipStream_t s) {
     if (sh.partBlocks) hipLaunchKernelGGL(k_part_bin, dim3(sh.partGrid), dim3(sh.partThreads), 0, s, c);
 }
 
-void launch_scan(const Ctx& c, hipStream_t s, StepCounters* sc) {
+void launch_scan(const Ctx& c, hipStream_t s, StepCounters* sc, bool advanceTimestep) {
     const int tiles = (c.numBins + kScanTile - 1) / kScanTile;
     hipLaunchKernelGGL(k_scan_reduce, dim3(tiles), dim3(kScanThreads), 0, s, c.count, c.priv, c.numCellBins,
-                       c.blockSums, c.numBins, sc);
+                       c.blockSums, c.numBins, sc, advanceTimestep ? c.ctr : nullptr, c.numWorlds);
     hipLaunchKernelGGL(k_scan_down, dim3(tiles), dim3(kScanThreads), 0, s, c.count, c.priv, c.numCellBins,
                        c.blockSums, c.start, c.numBins);
 }

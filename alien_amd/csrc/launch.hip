@@ -8,7 +8,7 @@
 
 namespace alien {
 
-void launch_scan(const Ctx& c, hipStream_t s, StepCounters* sc);
+void launch_scan(const Ctx& c, hipStream_t s, StepCounters* sc, bool advanceTimestep);
 void launch_cell_scatter(const Ctx& c, const LaunchShape& sh, hipStream_t s);
 void launch_part_sort(const Ctx& c, const LaunchShape& sh, hipStream_t s);
 void launch_part_scatter(const Ctx& c, const LaunchShape& sh, hipStream_t s);
@@ -19,7 +19,8 @@ void launch_bonds(const Ctx& c, const LaunchShape& sh, hipStream_t s, Recorder* 
 
 void launch_step(const Ctx& c, const LaunchShape& sh, hipStream_t s, Recorder* rec) {
     if (rec) rec->mark("scan");
-    launch_scan(c, s, c.sc);
+    // without bond work the end-of-step kernel would only advance the timesteps: the scan does it
+    launch_scan(c, s, c.sc, !sh.hasBondWork);
     if (rec) rec->mark("cell_scatter");
     launch_cell_scatter(c, sh, s);
     if (rec && sh.hasParticles) rec->mark("part_sort");
@@ -34,7 +35,7 @@ void launch_step(const Ctx& c, const LaunchShape& sh, hipStream_t s, Recorder* r
 // The scatters count the histogram down to zero; seeding the bins again restores it for the next
 // step.
 void launch_debug_map(const Ctx& c, const LaunchShape& sh, hipStream_t s) {
-    launch_scan(c, s, nullptr);
+    launch_scan(c, s, nullptr, false);
     launch_cell_scatter(c, sh, s);
     launch_part_scatter(c, sh, s);
     launch_cell_binsort(c, sh, s);
