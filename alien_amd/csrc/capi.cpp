@@ -512,7 +512,7 @@ int alien_batch_autotune(alien_batch* b, uint32_t trialSteps, int32_t* blockSize
             }
         trial.setBlockSize(bestBlock);
         trial.setStripCap(bestStrip);
-        for (int rows : {2, 4, 8, 16}) {
+        for (int rows : {2, 4, alien::kChunkMax}) {
             trial.setChunkRows(rows);
             const double dt = time();
             if (dt < best) {

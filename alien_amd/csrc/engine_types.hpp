@@ -23,7 +23,7 @@ constexpr int kRowThreads = 64;
 constexpr int kRowCap = 128;
 constexpr int kRowPool = 256;  // entries of three consecutive staged rows
 constexpr int kStripMax = 64;
-constexpr int kChunkMax = 16;  // bin rows per neighbour-pass chunk
+constexpr int kChunkMax = 8;  // bin rows per neighbour-pass chunk
 constexpr int kClaimFree = 0x7fffffff;
 
 // cell flags word

@@ -856,7 +856,7 @@ void WorldSet::setStripCap(int cap) {
 }
 
 void WorldSet::setChunkRows(int rows) {
-    if (rows != 0 && (rows < 2 || rows > kChunkMax)) throw Error(ALIEN_E_ARG, "chunk rows must be 0 or 2..16");
+    if (rows != 0 && (rows < 2 || rows > kChunkMax)) throw Error(ALIEN_E_ARG, "chunk rows must be 0 or 2..8");
     chunkRows_ = rows;
     if (worlds_.empty()) return;
     chunk_shape(worlds_, params_, chunkRows_);

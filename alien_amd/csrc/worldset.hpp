@@ -122,7 +122,7 @@ public:
     // all worlds are active after build.
     void setActive(int w, bool on);
     int stripCap() const { return stripCap_; }
-    // bin rows per neighbour-pass chunk (2..16), or 0 for the occupancy rule; launch shape only
+    // bin rows per neighbour-pass chunk (2..kChunkMax), or 0 for the occupancy rule; launch shape only
     void setChunkRows(int rows);
     int chunkRows() const { return chunkRows_; }
     void setUseGraph(bool on);

@@ -224,7 +224,7 @@ int alien_batch_set_block_size(alien_batch* b, int32_t threads);
 int alien_batch_set_use_graph(alien_batch* b, int32_t enable);
 /* Autotuner (no counterpart in ALIEN): times trialSteps steps of a device copy of the batch for
  * candidate launch shapes (cell-kernel block size 64/128/256 x neighbour-pass strip cap 64/32/16
- * bins, then bin rows per neighbour-pass chunk 2/4/8/16 against the default rule, reported as 0)
+ * bins, then bin rows per neighbour-pass chunk 2/4/8 against the default rule, reported as 0)
  * and keeps the fastest, also for worlds added later.  The batch's own worlds do not advance and
  * no result depends on the choice.  Outputs may be NULL. */
 int alien_batch_autotune(alien_batch* b, uint32_t trialSteps, int32_t* blockSize, int32_t* stripCap,
