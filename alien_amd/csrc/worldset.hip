@@ -729,10 +729,11 @@ void strip_shape(DevWorld& w, int strip, int rowsPerStage) {
 // Rows per neighbour-pass chunk, one value for all worlds of a set (forcedRows if nonzero).  Tall
 // chunks stage fewer halo rows per computed row; short ones even out the work of the workgroups,
 // whose cell counts vary.  Measured on MI355X, the pass ran fastest with the tallest chunks that
-// still give at least kMinRounds workgroups per resident slot (4M-cell fluid world: 6 rows,
-// 3.6 rounds, 3 % faster than at 2 rounds); sets too small for that take the shortest chunks.
+// still give at least minRounds workgroups per resident slot: 3.5 for the 4M-cell fluid world
+// (6 rows, 3 % faster than at 2 rounds), 5 for the 3.4M-cell soft-body world, whose bodies make
+// the workgroups' loads unequal (2 % faster than at 3.5); sets too small for that take the
+// shortest chunks.
 void chunk_shape(std::vector<DevWorld>& worlds, const std::vector<DevParams>& params, int forcedRows) {
-    constexpr double kMinRounds = 3.5;
     static const int numCUs = [] {
         int dev = 0, n = 0;
         if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
@@ -746,6 +747,7 @@ void chunk_shape(std::vector<DevWorld>& worlds, const std::vector<DevParams>& pa
     if (collision) perCU = std::min(perCU, neighbor_blocks_per_cu(false));
     if (!fluid && !collision) perCU = 1;
     const int64_t slots = int64_t(perCU) * numCUs;
+    const double minRounds = collision ? 5.0 : 3.5;
     int best = 2;
     // staged rows per chunk of a world for `rows` bin rows (a chunk covers at most kChunkMax bin
     // rows: the LDS histogram of its region)
@@ -759,7 +761,7 @@ void chunk_shape(std::vector<DevWorld>& worlds, const std::vector<DevParams>& pa
             const int ty = stagedRows(w, rows), nyS = w.nby / w.rowsPerStage;
             tiles += int64_t(w.ntx) * ((nyS + ty - 1) / ty);
         }
-        if (double(tiles) >= kMinRounds * double(slots)) best = rows;
+        if (double(tiles) >= minRounds * double(slots)) best = rows;
     }
     if (forcedRows > 0) best = forcedRows;
     for (DevWorld& w : worlds) {
