@@ -831,7 +831,11 @@ __device__ __forceinline__ void row_compute(const Ctx& c, const DevWorld& w, con
                     },
                     stPos);
             } else {
-                cell_finish<kFluid>(c, w, P, wi, self, slot, bt, a, R, stPos);
+                // the bond targets are loaded again rather than kept in registers through the pair
+                // work (few cells have bonds in a fluid world; the loads hit the cache)
+                int bt2[kMaxBonds];
+                load_bonds(c, self, bt2);
+                cell_finish<kFluid>(c, w, P, wi, self, slot, bt2, a, R, stPos);
             }
         }
     }
