@@ -441,7 +441,7 @@ struct RowShared {
     float2 pos[kPool + kPosMirror];
     float2 vel[kPool];
     float dens[kPool];  // fluid: 1 / density; collision: density
-    uint32_t flags[kPool];
+    uint16_t flags[kPool];  // the bits the pass reads of an entry's flags (flags16)
     int aidx[kPool];
     uint8_t sbin[kPool];  // bin slot: bin row * HX + halo column
     // the row in flight
@@ -489,6 +489,11 @@ __device__ __forceinline__ int seg_first(int seg, int HX) { return seg == 0 ? 0 
 template <int kSub>
 __device__ __forceinline__ int slot_row(int j, int HX) { return kSub == 2 && j >= HX ? 1 : 0; }
 
+// An entry's flags in LDS: bond counts (bits 0..7) and the barrier bit (moved to bit 8), the only
+// bits the pass reads of a neighbour; a cell's own full flags are read again before its writes.
+__device__ __forceinline__ uint16_t flags16(uint32_t f) { return uint16_t((f & 0xFFu) | ((f & kBarrierBit) >> 8)); }
+__device__ __forceinline__ uint32_t flags32(uint16_t p) { return (uint32_t(p) & 0xFFu) | ((uint32_t(p) & 0x100u) << 8); }
+
 // logical index of the staged entry at pool position pp (already reduced modulo kPool) of ring row r
 template <int kSub, class RS>
 __device__ __forceinline__ int row_lidx(const RS& S, const StripGeom& g, int r, int pp) {
@@ -505,7 +510,7 @@ __device__ __forceinline__ Nb row_nb(const RS& S, const StripGeom& g, int r, int
     o.pos = to_f2(S.pos[pp]);
     o.vel = to_f2(S.vel[pp]);
     o.dens = S.dens[pp];
-    o.flags = S.flags[pp];
+    o.flags = flags32(S.flags[pp]);
     o.aidx = S.aidx[pp];
     o.lidx = kLidx ? row_lidx<kSub>(S, g, r, pp) : -1;
     return o;
@@ -617,7 +622,7 @@ __device__ __forceinline__ void row_commit(const StripGeom& g, RS& S, const RowF
             if (pp < kPosMirror) S.pos[kPool + pp] = F.pos[q];
             S.vel[pp] = F.vel[q];
             S.dens[pp] = F.dens[q];
-            S.flags[pp] = F.flags[q];
+            S.flags[pp] = flags16(F.flags[q]);
             S.aidx[pp] = a;
             S.sbin[pp] = uint8_t(j);
         }
@@ -819,6 +824,9 @@ __device__ __forceinline__ void row_compute(const Ctx& c, const DevWorld& w, con
         const unsigned long long stBal = __ballot(active && (flags_barrier(self.flags) || flags_num_conn(self.flags) > 0));
         const int stPos = __shfl(stBase, 0, kRowThreads) + int(__popcll(stBal & ((1ull << lane) - 1ull)));
         if (active) {
+            // the cell's full flags, from the start-of-step arrays (the map copy holds a bin's
+            // entries in arrival order, not at their logical indices)
+            self.flags = c.A.flags[self.aidx];
             if (direct) {
                 atomicAdd(&c.sc->directCells, 1ull);
                 cell_direct<kFluid>(
