@@ -975,9 +975,9 @@ __device__ __forceinline__ bool chunk_rows(const Ctx& c, const DevWorld& w, cons
 
 // A collision world staged two bin rows at a time (sparse rows) runs a chunk whose two-row stages
 // do not fit as single bin rows over the same bins.
-// The fluid kernel is held to 128 VGPRs, four waves per SIMD (the LDS allows 15 workgroups per
-// CU): a few registers spill, and the pass ran 4 % faster than at 135 VGPRs and three waves.  The
-// collision kernel needs ~150 and would spill more than it gains.
+// The fluid kernel is held to 128 VGPRs, four waves per SIMD (its 10.2 KB of LDS allow 16
+// workgroups per CU); it ran 4 % faster than at 135 VGPRs and three waves.  The collision kernel
+// needs ~146 and would spill more than it gains.
 template <bool kFluid>
 __global__ void __launch_bounds__(kRowThreads) __attribute__((amdgpu_waves_per_eu(kFluid ? 4 : 1)))
 k_neighbor_rows(Ctx c) {
