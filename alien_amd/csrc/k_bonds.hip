@@ -49,7 +49,8 @@ __device__ __forceinline__ f2 bond_vector(const Ctx& c, int o, f2 px, f2 sx, int
 // pass 1 also runs checkConnections (CellProcessor.cuh:493-523) on the moved positions.  The
 // partners' data of all six bond slots is loaded up front (slots beyond the cell's bonds read the
 // cell itself and are not used), so the loads are in flight together.
-__global__ void k_conn_own(Ctx c, int pass) {
+// (at most 80 VGPRs, six waves per SIMD: the partner loads are latency-bound)
+__global__ void __attribute__((amdgpu_waves_per_eu(6))) k_conn_own(Ctx c, int pass) {
     const int n = c.sc->staged;
     const int cap = c.cellCap;
     for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {

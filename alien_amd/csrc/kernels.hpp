@@ -7,6 +7,11 @@
 
 namespace alien {
 
+// Largest grid of the kernels over the staged list (256 threads a block).  These kernels chase
+// bond partners through memory; one cell per thread up to ~67M cells keeps the most loads in
+// flight (a 2048-block grid-stride loop ran the soft-body batch's connection forces 25 % slower).
+constexpr int kStagedGridMax = 262144;
+
 // Launch shape of one world set: sizes of the per-world block tables and grid sizes of the
 // list-driven kernels.  None of it changes results: no arithmetic depends on the launch shape.
 struct LaunchShape {

@@ -637,9 +637,10 @@ void WorldSet::build(const std::vector<Source>& sources) {
     parity_ = 0;
     int hintTotal = 0;
     for (int h : stagedHint_) hintTotal += h;
-    shape_.stagedGrid = std::max(16, std::min(2048, (hintTotal + 255) / 256));
+    shape_.stagedGrid = std::max(16, std::min(kStagedGridMax, (hintTotal + 255) / 256));
     shape_.candGrid = 16;
-    shape_.partGrid = std::max(1, std::min(1024, (partOff + shape_.partThreads - 1) / shape_.partThreads));
+    // one particle block per workgroup (blocks past a world's live particles exit at once)
+    shape_.partGrid = std::max(1, std::min(65536, (partOff + shape_.partThreads - 1) / shape_.partThreads));
 
     std::vector<DevCounters> ctr(worlds_.size());
     for (size_t i = 0; i < worlds_.size(); ++i) {
@@ -1016,7 +1017,7 @@ void WorldSet::checkErrors() {
         check_hip(hipMemcpy(dSc_, &sc, sizeof(sc), hipMemcpyHostToDevice), "H2D");
         throw Error(ALIEN_E_CAPACITY, "fusion candidate list overflow");
     }
-    const int wantStaged = std::min(2048, (sc.staged + 255) / 256);
+    const int wantStaged = std::min(kStagedGridMax, (sc.staged + 255) / 256);
     const int wantCand = std::min(2048, (std::min(sc.candidates, candCap_) + 255) / 256);
     if (wantStaged > 2 * shape_.stagedGrid || wantCand > 2 * shape_.candGrid) {
         shape_.stagedGrid = std::max(shape_.stagedGrid, wantStaged);
