@@ -134,7 +134,7 @@ struct Ctx {
     // Bin sizes.  A cell bin's size is priv[b] + count[b]: priv holds the cells counted by the
     // neighbour-pass wave that owns the bin (plain stores, every cell bin written every step),
     // count the cells counted with global atomics (those that left their wave's region, bonded
-    // and barrier cells, seeding, and all particle bins).  The scatter counts `count` back down.
+    // and barrier cells, seeding, and all particle bins).  The scatters bring `count` back to zero.
     uint32_t* count;      // [numBins] start-of-step state
     uint32_t* countNext;  // [numBins] end-of-step state (next step's `count`)
     uint32_t* priv;       // [numCellBins] start-of-step state

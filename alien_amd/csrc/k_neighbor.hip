@@ -268,8 +268,7 @@ __device__ __forceinline__ void bin_next(const Ctx& c, const DevWorld& w, int k,
         const uint32_t sh = uint32_t(b & 1) << 4;
         c.arrival[k] = int((atomicAdd(&R.hist[b >> 1], 1u << sh) >> sh) & 0xFFFFu);
     } else {
-        atomicAdd(&c.countNext[gb], 1u);
-        c.arrival[k] = kArrivalShared;
+        c.arrival[k] = kArrivalShared | int(atomicAdd(&c.countNext[gb], 1u));
     }
 }
 

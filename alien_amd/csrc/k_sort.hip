@@ -87,7 +87,7 @@ __global__ void __launch_bounds__(kScanThreads) k_scan_reduce(const uint32_t* co
 }
 
 // writes start[]: every tile adds the totals of the tiles before it (a few thousand words, read
-// from L2) to its own exclusive scan.  `count` stays: the scatter counts it back down to zero,
+// from L2) to its own exclusive scan.  `count` stays: the scatters clear it (cells) or count it down (particles) to zero,
 // which leaves it cleared for the step after next (the two histograms alternate)
 __global__ void __launch_bounds__(kScanThreads) k_scan_down(const uint32_t* count, const uint32_t* priv, int nCell,
                                                             const uint32_t* blockSums, uint32_t* start, int n) {
@@ -120,8 +120,8 @@ __global__ void __launch_bounds__(kScanThreads) k_scan_down(const uint32_t* coun
 
 // entries land in their bin (the in-bin order is fixed later by ranking), with a copy of the
 // state the neighbour pass reads: owner-counted cells at the rank they got when counted, the
-// shared ones after them, each at the place its count-down of the shared histogram returns
-// (which also clears the histogram)
+// shared ones after them at the place their count returned.  A shared cell also clears its bin's
+// shared counter (plain stores of zero), which leaves the histogram empty for the step after next.
 __global__ void k_cell_scatter(Ctx c) {
     int2 blk = c.cellBlocks[blockIdx.x];
     const DevWorld w = c.worlds[blk.x];
@@ -134,7 +134,10 @@ __global__ void k_cell_scatter(Ctx c) {
     const int gb = c.binOf[i];
     const int arr = c.arrival[i];
     int rank = arr;
-    if (arr & kArrivalShared) rank = int(c.priv[gb]) + int(atomicSub(&c.count[gb], 1u)) - 1;
+    if (arr & kArrivalShared) {
+        rank = int(c.priv[gb]) + (arr & ~kArrivalShared);
+        c.count[gb] = 0u;
+    }
     const int dest = w.cellOff + int(c.start[gb] - c.start[w.binOff]) + rank;
     c.sorted[dest] = i;
     c.mpos[dest] = p;

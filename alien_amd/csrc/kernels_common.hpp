@@ -50,8 +50,8 @@ __device__ __forceinline__ int rank_in_bin(const int* sorted, int s, int e, int 
     return r;
 }
 
-// counting-sort bookkeeping: the arrival mark of a cell counted in the shared (atomic) histogram
-// (its place among the bin's shared arrivals is taken by the scatter)
+// counting-sort bookkeeping: a cell counted in the shared (atomic) histogram records its place
+// among the bin's shared arrivals (the count before its own) with this mark
 constexpr int kArrivalShared = 0x40000000;
 
 // cell bin of a position (global bin index)
@@ -62,12 +62,11 @@ __device__ __forceinline__ int cell_bin_of(const DevWorld& w, f2 p) {
 }
 
 // counting-sort bookkeeping for the next step: bin of the end-of-step position, counted into the
-// shared histogram `hist` (no result is waited for)
+// shared histogram `hist`, and the cell's place among the bin's shared arrivals
 __device__ __forceinline__ void bin_for_next_step(const Ctx& c, const DevWorld& w, int k, f2 p, uint32_t* hist) {
     const int gb = cell_bin_of(w, p);
     c.binOf[k] = gb;
-    atomicAdd(&hist[gb], 1u);
-    c.arrival[k] = kArrivalShared;
+    c.arrival[k] = kArrivalShared | int(atomicAdd(&hist[gb], 1u));
 }
 __device__ __forceinline__ void pbin_for_next_step(const Ctx& c, const DevWorld& w, int k, f2 p, uint32_t* hist) {
     int sx, sy;

@@ -32,7 +32,7 @@ void launch_step(const Ctx& c, const LaunchShape& sh, hipStream_t s, Recorder* r
     if (rec) rec->mark(nullptr);
 }
 
-// The scatters count the histogram down to zero; seeding the bins again restores it for the next
+// The scatters leave the histogram at zero; seeding the bins again restores it for the next
 // step.
 void launch_debug_map(const Ctx& c, const LaunchShape& sh, hipStream_t s) {
     launch_scan(c, s, nullptr, false);
