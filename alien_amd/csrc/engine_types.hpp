@@ -77,6 +77,7 @@ struct DevWorld {
     int tileX, tileY;       // neighbour pass: bins per strip, staged rows per chunk (launch shape only)
     int ntx, nty;           // strips, chunks
     int rowsPerStage;       // bin rows per staged row of the neighbour pass (1, or 2 for sparse rows)
+    int bondWork;           // 1 if a bonded or barrier cell exists or can appear by fusion
 };
 
 // per-world device counters
@@ -88,7 +89,6 @@ struct DevCounters {
 
 // global counters of one step (reset at the start of every step)
 struct StepCounters {
-    int staged;      // cells of the staged (bonded or barrier) list
     int candidates;  // fusion candidate pairs
     int errors;      // bit 0: candidate list overflow
     int liveParticles;  // energy particles alive after this step's particle sort, all worlds
@@ -152,7 +152,8 @@ struct Ctx {
     uint32_t* mflags;
     int* newidx;   // [cellCap] A index -> new index
     int* psorted;  // [partCap]
-    int2* staged;  // [cellCap] (new index, world)
+    uint8_t* stagedMark;  // [cellCap] end-of-step cell is staged (bonded or barrier at the neighbour pass);
+                          // written for the cells of bond-work worlds
     float2* own;    // [cellCap]
     float2* shift;  // [cellCap] Verlet displacement of a staged cell, applied after the second force pass
     float* bstiff;  // [cellCap] stiffness of a bonded cell in end-of-step order (written by the neighbour pass)
@@ -170,6 +171,7 @@ struct Ctx {
     int* absorbCell;  // [partCap]
     int* cclaim;      // [cellCap]
     const int2* cellBlocks;  // per block: (world, first global cell index)
+    const int2* bondBlocks;  // the same over the worlds with bond work
     const int2* binBlocks;   // per block: (world, first global bin index) over cell bins
     const int2* pbinBlocks;  // per block: (world, first global bin index) over particle bins
     const int2* partBlocks;  // per block: (world, first global particle index)

@@ -13,16 +13,25 @@
 
 namespace alien {
 
-#define FOR_STAGED(q) for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x)
+// The kernels below run one thread per cell of the worlds that can have bonded or barrier cells
+// (bond block table) and act on the staged cells: those the neighbour pass found bonded or barrier
+// (stagedMark), which is the set CellProcessor's bonded-cell passes cover.  Cells bonded by this
+// step's fusion are not among them.
+__device__ __forceinline__ bool staged_cell(const Ctx& c, int& x, int& wi) {
+    const int2 blk = c.bondBlocks[blockIdx.x];
+    wi = blk.x;
+    x = blk.y + int(threadIdx.x);
+    const DevWorld& w = c.worlds[wi];
+    return x < w.cellOff + w.numCells && c.stagedMark[x] != 0;
+}
 
 // bond targets of the staged cells from start-of-step to end-of-step indices
 __global__ void k_remap(Ctx c) {
-    const int n = c.sc->staged;
     const int cap = c.cellCap;
-    FOR_STAGED(q) {
-        const int2 ent = c.staged[q];
-        const int x = ent.x;
-        const DevWorld& w = c.worlds[ent.y];
+    int x, wi;
+    if (!staged_cell(c, x, wi)) return;
+    {
+        const DevWorld& w = c.worlds[wi];
         const int nc = flags_num_conn(c.B.flags[x]);
         // all targets first, then all lookups, then the stores (so that the loads overlap)
         int t[kMaxBonds], ni[kMaxBonds];
@@ -51,18 +60,17 @@ __device__ __forceinline__ f2 bond_vector(const Ctx& c, int o, f2 px, f2 sx, int
 // cell itself and are not used), so the loads are in flight together.
 // (at most 80 VGPRs, six waves per SIMD: the partner loads are latency-bound)
 __global__ void __attribute__((amdgpu_waves_per_eu(6))) k_conn_own(Ctx c, int pass) {
-    const int n = c.sc->staged;
     const int cap = c.cellCap;
-    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
-        const int2 ent = c.staged[q];
-        const int x = ent.x, wi = ent.y;
+    int x, wi;
+    if (!staged_cell(c, x, wi)) return;
+    {
         const DevWorld& w = c.worlds[wi];
         const DevParams& P = c.params[wi];
         const uint32_t fx = c.B.flags[x];
         const int nc = flags_num_conn(fx);
         if (nc == 0 || flags_barrier(fx)) {
             c.valid[x] = 0u;
-            continue;
+            return;
         }
         const float Wf = float(w.W), Hf = float(w.H);
         const bool considerAngles = (c.ctr[wi].timestep % 3ull) == 0ull;
@@ -189,10 +197,9 @@ __device__ __forceinline__ f2 gather_conn(const Ctx& c, const DevWorld& w, int x
 // Verlet position step (CellProcessor.cuh:525-544): the displacement is stored and applied after
 // the second force pass
 __global__ void k_conn_verlet_pos(Ctx c) {
-    const int n = c.sc->staged;
-    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
-        const int2 ent = c.staged[q];
-        const int x = ent.x, wi = ent.y;
+    int x, wi;
+    if (!staged_cell(c, x, wi)) return;
+    {
         const DevWorld& w = c.worlds[wi];
         const DevParams& P = c.params[wi];
         const uint32_t fx = c.B.flags[x];
@@ -200,7 +207,7 @@ __global__ void k_conn_verlet_pos(Ctx c) {
         const f2 v = to_f2(c.B.vel[x]);
         if (flags_barrier(fx)) {
             c.shift[x] = to_float2(mul(v, dt));
-            continue;
+            return;
         }
         const f2 s1 = gather_conn(c, w, x, flags_num_conn(fx));
         c.shift[x] = to_float2(add(mul(v, dt), mul(mul(mul(s1, dt), dt), 0.5f)));  // x/2 == x*0.5 exactly
@@ -211,17 +218,16 @@ __global__ void k_conn_verlet_pos(Ctx c) {
 // Verlet velocity step, friction and decay flag (CellProcessor.cuh:546-559, 707-723, 786-790); the
 // moved position is written here, and counted into the histogram of the next step
 __global__ void k_conn_verlet_vel(Ctx c) {
-    const int n = c.sc->staged;
-    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
-        const int2 ent = c.staged[q];
-        const int x = ent.x, wi = ent.y;
+    int x, wi;
+    if (!staged_cell(c, x, wi)) return;
+    {
         const DevWorld& w = c.worlds[wi];
         const DevParams& P = c.params[wi];
         const uint32_t fx = c.B.flags[x];
         const f2 p2 = torus_pos(add(to_f2(c.B.pos[x]), to_f2(c.shift[x])), w.W, w.H);
         c.B.pos[x] = to_float2(p2);
         bin_for_next_step(c, w, x, p2, c.countNext);
-        if (flags_barrier(fx)) continue;
+        if (flags_barrier(fx)) return;
         const f2 s1 = gather_conn(c, w, x, flags_num_conn(fx));
         const f2 acc = mul(add(s1, to_f2(c.s2[x])), 0.5f);
         f2 v = to_f2(c.B.vel[x]);
@@ -236,12 +242,11 @@ __global__ void k_conn_verlet_vel(Ctx c) {
 // structural operations (CellConnectionProcessor.cuh), R5/R6
 // ---------------------------------------------------------------------------------------------
 __global__ void k_mark(Ctx c) {
-    const int n = c.sc->staged;
     const int cap = c.cellCap;
-    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
-        const int2 ent = c.staged[q];
-        const int x = ent.x;
-        const DevWorld& w = c.worlds[ent.y];
+    int x, wi;
+    if (!staged_cell(c, x, wi)) return;
+    {
+        const DevWorld& w = c.worlds[wi];
         uint8_t m = c.delFlag[x];
         const int nc = flags_num_conn(c.B.flags[x]);
         for (int b = 0; b < nc && !m; ++b)
@@ -442,13 +447,12 @@ __device__ void delete_one_way(const Ctx& c, int c1, int targetLocal) {
 }
 
 __global__ void k_delete(Ctx c) {
-    const int n = c.sc->staged;
     const int cap = c.cellCap;
-    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
-        const int2 ent = c.staged[q];
-        const int x = ent.x;
-        if (!c.mark[x]) continue;
-        const DevWorld& w = c.worlds[ent.y];
+    int x, wi;
+    if (!staged_cell(c, x, wi)) return;
+    {
+        if (!c.mark[x]) return;
+        const DevWorld& w = c.worlds[wi];
         const int nc = flags_num_conn(c.B.flags[x]);
         int snap[kMaxBonds];
         for (int b = 0; b < nc; ++b) snap[b] = c.B.btgt[b * cap + x];
@@ -461,37 +465,36 @@ __global__ void k_delete(Ctx c) {
 // last kernel of the step: clears the flags of the staged cells and advances every world's
 // timestep (SimulationCudaFacade.cu:140-143)
 __global__ void k_clear_flags(Ctx c) {
-    const int n = c.sc->staged;
-    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n; q += gridDim.x * blockDim.x) {
-        const int x = c.staged[q].x;
-        c.mark[x] = 0;
-        c.delFlag[x] = 0;
-    }
     if (blockIdx.x == 0)
         for (int wi = threadIdx.x; wi < c.numWorlds; wi += blockDim.x) c.ctr[wi].timestep += 1ull;
+    int x, wi;
+    if (!staged_cell(c, x, wi)) return;
+    c.mark[x] = 0;
+    c.delFlag[x] = 0;
 }
 
 void launch_bonds(const Ctx& c, const LaunchShape& sh, hipStream_t s, Recorder* rec) {
-    const dim3 sg(sh.stagedGrid), cg(sh.candGrid), b(256);
-#define L(name, kernel, grid, ...)                              \
-    do {                                                        \
-        if (rec) rec->mark(name);                               \
-        hipLaunchKernelGGL(kernel, grid, b, 0, s, __VA_ARGS__); \
+    const dim3 sg(sh.bondBlocks), cg(sh.candGrid);
+#define L(name, kernel, grid, threads, ...)                           \
+    do {                                                              \
+        if (rec) rec->mark(name);                                     \
+        hipLaunchKernelGGL(kernel, grid, threads, 0, s, __VA_ARGS__); \
     } while (0)
-    // no world can have a bonded or barrier cell this step (none has one and none can fuse): the
-    // staged list is empty, and the scan at the start of the step advanced the timesteps
-    if (!sh.hasBondWork) return;
-    L("remap", k_remap, sg, c);
-    L("conn_own", k_conn_own, sg, c, 0);
-    L("conn_verlet_pos", k_conn_verlet_pos, sg, c);
-    L("conn_own", k_conn_own, sg, c, 1);
-    L("conn_verlet_vel", k_conn_verlet_vel, sg, c);
-    L("mark", k_mark, sg, c);
-    L("fuse_claim", k_fuse_claim, cg, c);
-    L("fuse_apply", k_fuse_apply, cg, c);
-    L("fuse_reset", k_fuse_reset, cg, c);
-    L("delete", k_delete, sg, c);
-    L("clear_flags", k_clear_flags, sg, c);
+    // no world can have a bonded or barrier cell this step (none has one and none can fuse): no
+    // cell is staged, and the scan at the start of the step advanced the timesteps
+    if (!sh.hasBondWork || sh.bondBlocks == 0) return;
+    const dim3 b(sh.cellThreads), cb(256);
+    L("remap", k_remap, sg, b, c);
+    L("conn_own", k_conn_own, sg, b, c, 0);
+    L("conn_verlet_pos", k_conn_verlet_pos, sg, b, c);
+    L("conn_own", k_conn_own, sg, b, c, 1);
+    L("conn_verlet_vel", k_conn_verlet_vel, sg, b, c);
+    L("mark", k_mark, sg, b, c);
+    L("fuse_claim", k_fuse_claim, cg, cb, c);
+    L("fuse_apply", k_fuse_apply, cg, cb, c);
+    L("fuse_reset", k_fuse_reset, cg, cb, c);
+    L("delete", k_delete, sg, b, c);
+    L("clear_flags", k_clear_flags, sg, b, c);
 #undef L
 }
 

@@ -277,11 +277,10 @@ __device__ __forceinline__ uint32_t hist_count(const uint32_t* hist, int b) {
 }
 
 // Barrier reflection, applyForces, and the end-of-step writes of one cell.  A bonded or barrier
-// cell goes to the staged list at stagedPos, or at a place taken here if stagedPos < 0.
+// cell is marked staged for the bond kernels of the step.
 template <bool kFluid>
 __device__ __forceinline__ void cell_finish(const Ctx& c, const DevWorld& w, const DevParams& P, int wi, const Nb& s,
-                                            int slot, const int bt[kMaxBonds], Acc& a, const Region& R,
-                                            int stagedPos = -1) {
+                                            int slot, const int bt[kMaxBonds], Acc& a, const Region& R) {
     const int cap = c.cellCap;
     const float Wf = float(w.W), Hf = float(w.H);
     const int i = s.aidx, k = s.lidx;
@@ -378,15 +377,14 @@ __device__ __forceinline__ void cell_finish(const Ctx& c, const DevWorld& w, con
     } else {
         c.B.pos[k] = to_float2(pc);
         c.B.vel[k] = to_float2(v);
-        const int q = stagedPos >= 0 ? stagedPos : atomicAdd(&c.sc->staged, 1);
-        c.staged[q] = make_int2(k, wi);
     }
+    if (w.bondWork) c.stagedMark[k] = uint8_t(bari || nci != 0);
 }
 
 // One cell, neighbours visited one after another (forEachNb calls visit(o) in stencil order).
 template <bool kFluid, class ForEach>
 __device__ __forceinline__ void cell_direct(const Ctx& c, const DevWorld& w, const DevParams& P, int wi, const Nb& s,
-                                            const Region& R, ForEach&& forEachNb, int stagedPos = -1) {
+                                            const Region& R, ForEach&& forEachNb) {
     const float Wf = float(w.W), Hf = float(w.H);
     int bt[kMaxBonds];
     load_bonds(c, s, bt);
@@ -399,7 +397,7 @@ __device__ __forceinline__ void cell_direct(const Ctx& c, const DevWorld& w, con
                                              [&] { return make_int2(s.lidx, o.lidx); });
         acc_add<kFluid>(a, f, t);
     });
-    cell_finish<kFluid>(c, w, P, wi, s, c.A.slot[s.aidx], bt, a, R, stagedPos);
+    cell_finish<kFluid>(c, w, P, wi, s, c.A.slot[s.aidx], bt, a, R);
 }
 
 
@@ -655,9 +653,6 @@ __device__ __forceinline__ void row_compute(const Ctx& c, const DevWorld& w, con
         unsigned long long mask = 0ull, bmask = 0ull;
         bool direct = false;
         int selfIdx = -1;  // fluid: number of the cell's pairs that come before its own density term
-        // the staged (bonded or barrier) cells of this batch take their list places up front, so
-        // that the atomic's round trip overlaps the pair work
-        int stBase = 0;
         const int t = kSub == 2 && q >= n0 ? 1 : 0;    // the cell's bin row in the staged row
         const int su = t == 0 ? cs0 + q : cs1 + (q - n0);  // pool position, not wrapped
         const int sp = su & kPoolMask;
@@ -666,9 +661,6 @@ __device__ __forceinline__ void row_compute(const Ctx& c, const DevWorld& w, con
             self = row_nb<kSub>(S, g, rc, sp);
             slot = c.A.slot[self.aidx];  // needed only at the end: loaded early
             load_bonds(c, self, bt);
-            // (lane 0 is active whenever any lane is: the active lanes are the lowest ones)
-            const unsigned long long stBal = __ballot(flags_barrier(self.flags) || flags_num_conn(self.flags) > 0);
-            if (lane == 0 && stBal != 0ull) stBase = atomicAdd(&c.sc->staged, int(__popcll(stBal)));
             const int hx = S.sbin[sp] - t * g.HX;
             // stencil bin rows (ring row, bin row): above, own, below
             int subOf[3] = {0, 0, 0};
@@ -820,29 +812,24 @@ __device__ __forceinline__ void row_compute(const Ctx& c, const DevWorld& w, con
             __syncthreads();
         }
         if (active && selfIdx == count) a.rho = a.rho + w0;  // the own term after all pairs
-        const unsigned long long stBal = __ballot(active && (flags_barrier(self.flags) || flags_num_conn(self.flags) > 0));
-        const int stPos = __shfl(stBase, 0, kRowThreads) + int(__popcll(stBal & ((1ull << lane) - 1ull)));
         if (active) {
             // the cell's full flags, from the start-of-step arrays (the map copy holds a bin's
             // entries in arrival order, not at their logical indices)
             self.flags = c.A.flags[self.aidx];
             if (direct) {
                 atomicAdd(&c.sc->directCells, 1ull);
-                cell_direct<kFluid>(
-                    c, w, P, wi, self, R,
-                    [&](auto&& visit) {
+                cell_direct<kFluid>(c, w, P, wi, self, R, [&](auto&& visit) {
 #pragma unroll
-                        for (int r = 0; r < 3; ++r)
-                            for (int u = a0s[r]; u < a0s[r] + ns[r]; ++u)
-                                visit(row_nb<kSub>(S, g, ringOf[r], u & kPoolMask));
-                    },
-                    stPos);
+                    for (int r = 0; r < 3; ++r)
+                        for (int u = a0s[r]; u < a0s[r] + ns[r]; ++u)
+                            visit(row_nb<kSub>(S, g, ringOf[r], u & kPoolMask));
+                });
             } else {
                 // the bond targets are loaded again rather than kept in registers through the pair
                 // work (few cells have bonds in a fluid world; the loads hit the cache)
                 int bt2[kMaxBonds];
                 load_bonds(c, self, bt2);
-                cell_finish<kFluid>(c, w, P, wi, self, slot, bt2, a, R, stPos);
+                cell_finish<kFluid>(c, w, P, wi, self, slot, bt2, a, R);
             }
         }
     }

@@ -77,7 +77,6 @@ __global__ void __launch_bounds__(kScanThreads) k_scan_reduce(const uint32_t* co
     if (threadIdx.x == 0) {
         blockSums[blockIdx.x] = total;
         if (sc && blockIdx.x == 0) {
-            sc->staged = 0;
             sc->candidates = 0;
             sc->liveParticles = 0;
         }

@@ -7,11 +7,6 @@
 
 namespace alien {
 
-// Largest grid of the kernels over the staged list (256 threads a block).  These kernels chase
-// bond partners through memory; one cell per thread up to ~67M cells keeps the most loads in
-// flight (a 2048-block grid-stride loop ran the soft-body batch's connection forces 25 % slower).
-constexpr int kStagedGridMax = 262144;
-
 // Launch shape of one world set: sizes of the per-world block tables and grid sizes of the
 // list-driven kernels.  None of it changes results: no arithmetic depends on the launch shape.
 struct LaunchShape {
@@ -20,10 +15,10 @@ struct LaunchShape {
     int binBlocks = 0, pbinBlocks = 0, binThreads = 256;
     int tileBlocks = 0;    // neighbour-pass workgroups (including XCD padding)
     int partGrid = 64;     // grid-stride particle kernels
-    int stagedGrid = 16;   // grid-stride kernels over the staged (bonded/barrier) cells
+    int bondBlocks = 0;    // bond kernels: cell blocks of the worlds that can have bonded cells
     int candGrid = 16;     // grid-stride kernels over fusion candidates
     bool hasFluid = false, hasCollision = false;  // motion modes present in the set
-    bool hasBondWork = true;  // some world has (or can get by fusion) a bonded or barrier cell
+    bool hasBondWork = true;  // some active world has (or can get by fusion) a bonded or barrier cell
     // some world has live energy particles (nothing on this path creates particles, so once all
     // are absorbed or merged the particle kernels, which would find no work, are not launched)
     bool hasParticles = true;

@@ -408,7 +408,6 @@ void WorldSet::build(const std::vector<Source>& sources) {
     std::vector<uint64_t> timesteps;
     std::vector<int> liveParticles;
     std::vector<uint8_t> pending;
-    std::vector<int> stagedHint;
     std::vector<uint8_t> bondWork;
     std::vector<int> stripChoice, stageChoice;
     int cellOff = 0, partOff = 0, bins = 0;
@@ -417,7 +416,7 @@ void WorldSet::build(const std::vector<Source>& sources) {
         int live = 0;
         uint64_t ts = 0;
         bool pend = false;
-        int strip = kStripMax, stage = 1, hint = 0;
+        int strip = kStripMax, stage = 1;
         bool work = false;
         if (src.image) {
             if (src.from == this) throw Error(ALIEN_E_ARG, "a world set cannot be rebuilt from itself");
@@ -426,8 +425,7 @@ void WorldSet::build(const std::vector<Source>& sources) {
             ts = src.image->timestep;
             pend = src.image->densityResetPending;
             const WorldImage& im = *src.image;
-            for (int i = 0; i < h.numCells; ++i) {
-                if (flags_num_conn(im.flags[i]) > 0 || flags_barrier(im.flags[i])) ++hint;
+            for (int i = 0; i < h.numCells && !work; ++i) {
                 // bonds can appear only by fusion, which needs a free bond slot
                 if (flags_num_conn(im.flags[i]) > 0 || flags_barrier(im.flags[i]) || flags_max_conn(im.flags[i]) > 0)
                     work = true;
@@ -443,7 +441,6 @@ void WorldSet::build(const std::vector<Source>& sources) {
             pend = src.from->densityPending_.at(src.fromIndex) != 0;
             strip = src.from->stripChoice_.at(src.fromIndex);
             stage = src.from->stageChoice_.at(src.fromIndex);
-            hint = src.from->stagedHint_.at(src.fromIndex);
             work = src.from->bondWork_.at(src.fromIndex) != 0;
         }
         validate_world(h.desc, h.params);
@@ -461,6 +458,7 @@ void WorldSet::build(const std::vector<Source>& sources) {
         w.binOff = bins;
         w.partOff = partOff;
         w.partCap = live;
+        w.bondWork = work ? 1 : 0;
         strip_shape(w, std::min(strip, stripCap_), stage);
         stripChoice.push_back(strip);
         stageChoice.push_back(stage);
@@ -473,7 +471,6 @@ void WorldSet::build(const std::vector<Source>& sources) {
         timesteps.push_back(ts);
         liveParticles.push_back(live);
         pending.push_back(pend ? 1 : 0);
-        stagedHint.push_back(hint);
         bondWork.push_back(work ? 1 : 0);
     }
     const int cellBins = bins;  // cell bins of all worlds come first, then the particle bins
@@ -540,7 +537,7 @@ void WorldSet::build(const std::vector<Source>& sources) {
     mflags_ = static_cast<uint32_t*>(A(sizeof(uint32_t) * NC));
     newidx_ = static_cast<int*>(A(sizeof(int) * NC));
     psorted_ = static_cast<int*>(A(sizeof(int) * NP));
-    staged_ = static_cast<int2*>(A(sizeof(int2) * NC));
+    stagedMark_ = static_cast<uint8_t*>(A(NC));
     own_ = static_cast<float2*>(A(sizeof(float2) * NC));
     shift_ = static_cast<float2*>(A(sizeof(float2) * NC));
     bstiff_ = static_cast<float*>(A(sizeof(float) * NC));
@@ -625,19 +622,13 @@ void WorldSet::build(const std::vector<Source>& sources) {
     worlds_ = std::move(worlds);
     params_ = std::move(params);
     densityPending_ = std::move(pending);
-    stagedHint_ = std::move(stagedHint);
     bondWork_ = std::move(bondWork);
     active_.assign(worlds_.size(), 1);
     stripChoice_ = std::move(stripChoice);
     stageChoice_ = std::move(stageChoice);
     chunk_shape(worlds_, params_, chunkRows_);
-    shape_.hasBondWork = false;
-    for (uint8_t b : bondWork_) shape_.hasBondWork = shape_.hasBondWork || b != 0;
     shape_.hasParticles = partOff > 0;
     parity_ = 0;
-    int hintTotal = 0;
-    for (int h : stagedHint_) hintTotal += h;
-    shape_.stagedGrid = std::max(16, std::min(kStagedGridMax, (hintTotal + 255) / 256));
     shape_.candGrid = 16;
     // one particle block per workgroup (blocks past a world's live particles exit at once)
     shape_.partGrid = std::max(1, std::min(65536, (partOff + shape_.partThreads - 1) / shape_.partThreads));
@@ -775,12 +766,14 @@ void chunk_shape(std::vector<DevWorld>& worlds, const std::vector<DevParams>& pa
 void WorldSet::buildTables() {
     for (auto& a : tableAllocs_) (void)hipFree(a.p);
     tableAllocs_.clear();
-    std::vector<int2> cells, bins, pbins, parts;
+    std::vector<int2> cells, bonds, bins, pbins, parts;
     const int ct = shape_.cellThreads, pt = shape_.partThreads, bt = shape_.binThreads;
     for (int wi = 0; wi < int(worlds_.size()); ++wi) {
         if (!active_[wi]) continue;
         const DevWorld& w = worlds_[wi];
         for (int s = 0; s < w.numCells; s += ct) cells.push_back(make_int2(wi, w.cellOff + s));
+        if (bondWork_[wi])
+            for (int s = 0; s < w.numCells; s += ct) bonds.push_back(make_int2(wi, w.cellOff + s));
         for (int s = 0; s < w.partCap; s += pt) parts.push_back(make_int2(wi, w.partOff + s));
         for (int s = 0; s < w.nbx * w.nby; s += bt) bins.push_back(make_int2(wi, w.binOff + s));
         for (int s = 0; s < w.pnbx * w.pnby; s += bt) pbins.push_back(make_int2(wi, w.pbinOff + s));
@@ -812,11 +805,15 @@ void WorldSet::buildTables() {
         }
     }
     cellBlocks_ = up(cells);
+    bondBlocks_ = up(bonds);
     binBlocks_ = up(bins);
     pbinBlocks_ = up(pbins);
     partBlocks_ = up(parts);
     tileBlocks_ = up(tileTable);
     shape_.cellBlocks = int(cells.size());
+    shape_.bondBlocks = int(bonds.size());
+    // the bond kernels run when an active world can have a bonded or barrier cell
+    shape_.hasBondWork = !bonds.empty();
     shape_.partBlocks = int(parts.size());
     shape_.binBlocks = int(bins.size());
     shape_.pbinBlocks = int(pbins.size());
@@ -924,7 +921,7 @@ Ctx WorldSet::makeCtx(int parity) const {
     c.mflags = mflags_;
     c.newidx = newidx_;
     c.psorted = psorted_;
-    c.staged = staged_;
+    c.stagedMark = stagedMark_;
     c.own = own_;
     c.shift = shift_;
     c.bstiff = bstiff_;
@@ -942,6 +939,7 @@ Ctx WorldSet::makeCtx(int parity) const {
     c.absorbCell = absorbCell_;
     c.cclaim = cclaim_;
     c.cellBlocks = cellBlocks_;
+    c.bondBlocks = bondBlocks_;
     c.binBlocks = binBlocks_;
     c.pbinBlocks = pbinBlocks_;
     c.partBlocks = partBlocks_;
@@ -1017,10 +1015,8 @@ void WorldSet::checkErrors() {
         check_hip(hipMemcpy(dSc_, &sc, sizeof(sc), hipMemcpyHostToDevice), "H2D");
         throw Error(ALIEN_E_CAPACITY, "fusion candidate list overflow");
     }
-    const int wantStaged = std::min(kStagedGridMax, (sc.staged + 255) / 256);
     const int wantCand = std::min(2048, (std::min(sc.candidates, candCap_) + 255) / 256);
-    if (wantStaged > 2 * shape_.stagedGrid || wantCand > 2 * shape_.candGrid) {
-        shape_.stagedGrid = std::max(shape_.stagedGrid, wantStaged);
+    if (wantCand > 2 * shape_.candGrid) {
         shape_.candGrid = std::max(shape_.candGrid, wantCand);
         invalidateGraphs();
     }

@@ -184,7 +184,7 @@ private:
     float2 *mpos_ = nullptr, *mvel_ = nullptr;
     float* mdens_ = nullptr;
     uint32_t* mflags_ = nullptr;
-    int2* staged_ = nullptr;
+    uint8_t* stagedMark_ = nullptr;
     float2 *own_ = nullptr, *shift_ = nullptr, *s2_ = nullptr, *out1_ = nullptr, *out2_ = nullptr;
     float* bstiff_ = nullptr;
     uint32_t* valid_ = nullptr;
@@ -192,10 +192,10 @@ private:
     unsigned long long* claim_ = nullptr;
     int4* cand_ = nullptr;
     int *want_ = nullptr, *fclaim_ = nullptr, *absorbCell_ = nullptr, *cclaim_ = nullptr;
-    int2 *cellBlocks_ = nullptr, *binBlocks_ = nullptr, *pbinBlocks_ = nullptr, *partBlocks_ = nullptr;
+    int2 *cellBlocks_ = nullptr, *bondBlocks_ = nullptr, *binBlocks_ = nullptr, *pbinBlocks_ = nullptr;
+    int2* partBlocks_ = nullptr;
     int2* tileBlocks_ = nullptr;
     int numPartBlocks_ = 0;
-    std::vector<int> stagedHint_;    // per world: bonded + barrier cells, sizes the staged-list grids
     std::vector<uint8_t> bondWork_;  // per world: a bonded or barrier cell exists or can appear by fusion
     std::vector<uint8_t> active_;    // per world: included in the launch tables
     std::vector<int> stripChoice_;   // per world: strip width chosen from its cell density
