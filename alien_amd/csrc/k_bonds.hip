@@ -219,23 +219,32 @@ __global__ void k_conn_verlet_pos(Ctx c) {
 // moved position is written here, and counted into the histogram of the next step
 __global__ void k_conn_verlet_vel(Ctx c) {
     int x, wi;
-    if (!staged_cell(c, x, wi)) return;
-    {
-        const DevWorld& w = c.worlds[wi];
-        const DevParams& P = c.params[wi];
-        const uint32_t fx = c.B.flags[x];
-        const f2 p2 = torus_pos(add(to_f2(c.B.pos[x]), to_f2(c.shift[x])), w.W, w.H);
-        c.B.pos[x] = to_float2(p2);
-        bin_for_next_step(c, w, x, p2, c.countNext);
-        if (flags_barrier(fx)) return;
-        const f2 s1 = gather_conn(c, w, x, flags_num_conn(fx));
-        const f2 acc = mul(add(s1, to_f2(c.s2[x])), 0.5f);
-        f2 v = to_f2(c.B.vel[x]);
-        v = add(v, mul(acc, P.dt));
-        v = mul(v, 1.0f - P.friction);
-        c.B.vel[x] = to_float2(v);
-        if (c.energy[w.cellOff + c.B.slot[x]] > P.maxBindingEnergy) c.delFlag[x] = 1;
+    const bool st = staged_cell(c, x, wi);  // (every lane stays for the wave-wide binning)
+    const DevWorld& w = c.worlds[wi];
+    const DevParams& P = c.params[wi];
+    const float dt = P.dt, friction = P.friction, maxEnergy = P.maxBindingEnergy;
+    // every load before the first store, so that they are in flight together
+    uint32_t fx = 0u;
+    f2 p = {0.0f, 0.0f}, sh = {0.0f, 0.0f}, v0 = {0.0f, 0.0f}, s2 = {0.0f, 0.0f}, s1 = {0.0f, 0.0f};
+    float energy = 0.0f;
+    if (st) {
+        fx = c.B.flags[x];
+        p = to_f2(c.B.pos[x]);
+        sh = to_f2(c.shift[x]);
+        v0 = to_f2(c.B.vel[x]);
+        s2 = to_f2(c.s2[x]);
+        energy = c.energy[w.cellOff + c.B.slot[x]];
+        if (!flags_barrier(fx)) s1 = gather_conn(c, w, x, flags_num_conn(fx));
     }
+    const f2 p2 = torus_pos(add(p, sh), w.W, w.H);
+    if (st) c.B.pos[x] = to_float2(p2);
+    bin_for_next_step_runs(c, w, x, p2, c.countNext, st);
+    if (!st || flags_barrier(fx)) return;
+    const f2 acc = mul(add(s1, s2), 0.5f);
+    f2 v = add(v0, mul(acc, dt));
+    v = mul(v, 1.0f - friction);
+    c.B.vel[x] = to_float2(v);
+    if (energy > maxEnergy) c.delFlag[x] = 1;
 }
 
 // ---------------------------------------------------------------------------------------------

@@ -68,6 +68,30 @@ __device__ __forceinline__ void bin_for_next_step(const Ctx& c, const DevWorld& 
     c.binOf[k] = gb;
     c.arrival[k] = kArrivalShared | int(atomicAdd(&hist[gb], 1u));
 }
+// bin_for_next_step for a whole wave (every lane calls it; lanes with !valid count nothing): one
+// atomic per run of consecutive lanes whose cells land in the same bin (cells in map order mostly
+// stay together), each cell ranked inside its run's share of the bin's shared arrivals
+__device__ __forceinline__ void bin_for_next_step_runs(const Ctx& c, const DevWorld& w, int k, f2 p, uint32_t* hist,
+                                                       bool valid) {
+    const int lane = int(threadIdx.x) & 63;
+    const int gb = valid ? cell_bin_of(w, p) : -1;
+    const int prev = __shfl_up(gb, 1, 64);
+    const bool start = valid && (lane == 0 || prev != gb);
+    const unsigned long long starts = __ballot(start);
+    const unsigned long long bounds = starts | __ballot(!valid);  // a run ends at a start or an invalid lane
+    const unsigned long long upto = lane == 63 ? ~0ull : (2ull << lane) - 1ull;  // lanes 0..lane
+    const unsigned long long sm = starts & upto;
+    const int rs = sm ? 63 - __clzll(sm) : lane;  // the run's first lane
+    const unsigned long long after = bounds & ~upto;
+    const int re = after ? __ffsll(after) - 1 : 64;  // one past the run's last lane
+    int base = 0;
+    if (start) base = int(atomicAdd(&hist[gb], uint32_t(re - lane)));
+    base = __shfl(base, rs, 64);
+    if (valid) {
+        c.binOf[k] = gb;
+        c.arrival[k] = kArrivalShared | (base + lane - rs);
+    }
+}
 __device__ __forceinline__ void pbin_for_next_step(const Ctx& c, const DevWorld& w, int k, f2 p, uint32_t* hist) {
     int sx, sy;
     slot_of(p, w.W, w.H, sx, sy);
