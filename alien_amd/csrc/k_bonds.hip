@@ -2,7 +2,7 @@
 //
 // Connection forces (CellProcessor::calcConnectionForces, source/EngineGpuKernels/CellProcessor.cuh:
 // 408-491) run twice per step around the Verlet position update (:525-544) and before the velocity
-// update (:546-559), over the staged list (cells with bonds, and barrier cells).  Angle forces that
+// update (:546-559), over the staged cells (cells with bonds, and barrier cells).  Angle forces that
 // the reference adds to the neighbours with float atomics are written by the owner into per-bond
 // slots and gathered by the receiver in its own bond order (R7).  Structural operations follow
 // CellConnectionProcessor (source/EngineGpuKernels/CellConnectionProcessor.cuh): fusion pairs

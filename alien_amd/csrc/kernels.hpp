@@ -8,7 +8,7 @@
 namespace alien {
 
 // Launch shape of one world set: sizes of the per-world block tables and grid sizes of the
-// list-driven kernels.  None of it changes results: no arithmetic depends on the launch shape.
+// candidate-driven kernels.  None of it changes results: no arithmetic depends on the launch shape.
 struct LaunchShape {
     int cellBlocks = 0, cellThreads = 256;
     int partBlocks = 0, partThreads = 256;

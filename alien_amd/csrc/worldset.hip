@@ -1005,8 +1005,8 @@ void WorldSet::setParams(int w, const alien_params& p) {
     if (fluid != shape_.hasFluid || collision != shape_.hasCollision) invalidateGraphs();
 }
 
-// Reads the last step's list sizes: reports overflow and grows the grids of the list-driven
-// kernels when the lists outgrew them (the graphs are captured again on the next step).
+// Reads the last step's candidate-list size: reports overflow and grows the grid of the fusion
+// kernels when the list outgrew it (the graphs are captured again on the next step).
 void WorldSet::checkErrors() {
     StepCounters sc{};
     check_hip(hipMemcpy(&sc, dSc_, sizeof(sc), hipMemcpyDeviceToHost), "D2H");
