@@ -133,6 +133,24 @@ class WorldDesc(ctypes.Structure):
     ]
 
 
+class Statistics(ctypes.Structure):
+    """alien_statistics: per-colour counts and energies (TimestepStatistics subset)."""
+    _fields_ = [
+        ("numCells", ctypes.c_int32 * 7),
+        ("numParticles", ctypes.c_int32 * 7),
+        ("totalEnergy", ctypes.c_double * 7),
+        ("timestep", ctypes.c_uint64),
+    ]
+
+    def as_dict(self):
+        return {
+            "numCells": list(self.numCells),
+            "numParticles": list(self.numParticles),
+            "totalEnergy": list(self.totalEnergy),
+            "timestep": int(self.timestep),
+        }
+
+
 class Totals(ctypes.Structure):
     _fields_ = [
         ("cellEnergy", ctypes.c_double),

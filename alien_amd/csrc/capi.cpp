@@ -243,6 +243,14 @@ int alien_get_totals(alien_world* w, alien_totals* out) {
     });
 }
 
+int alien_get_statistics(alien_world* w, alien_statistics* out) {
+    return guarded([&] {
+        requireHealthy(w);
+        require(out != nullptr, "null argument");
+        *out = w->set->statistics(0);
+    });
+}
+
 int alien_clear(alien_world* w) {
     return alien_set_data(w, nullptr, 0, nullptr, 0);
 }

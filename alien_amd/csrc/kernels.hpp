@@ -40,6 +40,10 @@ void launch_step(const Ctx& c, const LaunchShape& sh, hipStream_t s, Recorder* r
 // cell map of the current state in logical order (sorted/newidx), for introspection; consumes
 // the seeded bins and seeds them again afterwards
 void launch_debug_map(const Ctx& c, const LaunchShape& sh, hipStream_t s);
+// per-colour statistics of one world: block count for its cells and particle slots, and the
+// launch writing per-block counts and energies (kColors per block)
+int statistics_blocks(int numCells, int partCap, int& cellBlocks);
+void launch_statistics(const Ctx& c, int wi, int cellBlocks, int blocks, int* cnt, double* en, hipStream_t s);
 void launch_debug_neighbors(const Ctx& c, const LaunchShape& sh, const uint64_t* offsets, int* out, int* counts,
                             hipStream_t s);
 

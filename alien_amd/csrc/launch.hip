@@ -90,4 +90,70 @@ void launch_debug_neighbors(const Ctx& c, const LaunchShape& sh, const uint64_t*
         hipLaunchKernelGGL(k_debug_neighbors, dim3(sh.cellBlocks), dim3(sh.cellThreads), 0, s, c, offsets, out, counts);
 }
 
+// Per-colour statistics of world wi (StatisticsKernels.cu:11-45): blocks [0, cellBlocks) take 256
+// cells each, the rest 256 particle slots each (living particles only); every block writes its
+// colours' counts and energies, summed in a fixed order (a tree per wave, then the waves in turn),
+// and the host adds the blocks in order.
+constexpr int kStatThreads = 256;
+__global__ void __launch_bounds__(kStatThreads) k_statistics(Ctx c, int wi, int cellBlocks, int* cnt, double* en) {
+    __shared__ int wc[kStatThreads / 64][kColors];
+    __shared__ double we[kStatThreads / 64][kColors];
+    const DevWorld w = c.worlds[wi];
+    const bool cells = int(blockIdx.x) < cellBlocks;
+    const int idx = (cells ? int(blockIdx.x) : int(blockIdx.x) - cellBlocks) * kStatThreads + int(threadIdx.x);
+    int col = -1;
+    double e = 0.0;
+    if (cells) {
+        if (idx < w.numCells) {
+            const int i = w.cellOff + idx;
+            col = flags_color(c.A.flags[i]) % kColors;
+            e = double(c.energy[w.cellOff + c.A.slot[i]]);
+        }
+    } else if (idx < c.ctr[wi].numParticles) {
+        // (particles absorbed in the last step stay in the live range, marked, until the next sort)
+        const int j = w.partOff + idx;
+        const uint8_t pf = c.A.pflags[j];
+        if (!(pf & kParticleDead)) {
+            col = int(pf & kParticleColorMask) % kColors;
+            e = double(c.A.pen[j]);
+        }
+    }
+    const int lane = int(threadIdx.x) & 63, wave = int(threadIdx.x) >> 6;
+#pragma unroll
+    for (int k = 0; k < kColors; ++k) {
+        int n = col == k ? 1 : 0;
+        double s = col == k ? e : 0.0;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            n += __shfl_down(n, o, 64);
+            s += __shfl_down(s, o, 64);
+        }
+        if (lane == 0) {
+            wc[wave][k] = n;
+            we[wave][k] = s;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < kColors) {
+        const int k = int(threadIdx.x);
+        int n = 0;
+        double s = 0.0;
+        for (int v = 0; v < kStatThreads / 64; ++v) {
+            n += wc[v][k];
+            s += we[v][k];
+        }
+        cnt[blockIdx.x * kColors + k] = n;
+        en[blockIdx.x * kColors + k] = s;
+    }
+}
+
+int statistics_blocks(int numCells, int partCap, int& cellBlocks) {
+    cellBlocks = (numCells + kStatThreads - 1) / kStatThreads;
+    return cellBlocks + (partCap + kStatThreads - 1) / kStatThreads;
+}
+
+void launch_statistics(const Ctx& c, int wi, int cellBlocks, int blocks, int* cnt, double* en, hipStream_t s) {
+    if (blocks > 0) hipLaunchKernelGGL(k_statistics, dim3(blocks), dim3(kStatThreads), 0, s, c, wi, cellBlocks, cnt, en);
+}
+
 }  // namespace alien

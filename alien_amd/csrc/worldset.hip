@@ -1157,6 +1157,44 @@ alien_totals WorldSet::totals(int wi) {
     return t;
 }
 
+alien_statistics WorldSet::statistics(int wi) {
+    const DevWorld& w = worlds_.at(wi);
+    alien_statistics st{};
+    st.timestep = timestep(wi);
+    int cellBlocks = 0;
+    const int blocks = statistics_blocks(w.numCells, w.partCap, cellBlocks);
+    if (blocks == 0) return st;
+    synchronize();
+    int* cnt = nullptr;
+    double* en = nullptr;
+    check_hip(hipMalloc(&cnt, sizeof(int) * size_t(blocks) * kColors), "hipMalloc");
+    if (hipMalloc(&en, sizeof(double) * size_t(blocks) * kColors) != hipSuccess) {
+        (void)hipFree(cnt);
+        throw Error(ALIEN_E_OOM, "hipMalloc failed (statistics)");
+    }
+    std::vector<int> hc(size_t(blocks) * kColors);
+    std::vector<double> he(size_t(blocks) * kColors);
+    try {
+        launch_statistics(makeCtx(parity_), wi, cellBlocks, blocks, cnt, en, stream_);
+        check_hip(hipGetLastError(), "statistics");
+        synchronize();
+        check_hip(hipMemcpy(hc.data(), cnt, sizeof(int) * hc.size(), hipMemcpyDeviceToHost), "D2H");
+        check_hip(hipMemcpy(he.data(), en, sizeof(double) * he.size(), hipMemcpyDeviceToHost), "D2H");
+    } catch (...) {
+        (void)hipFree(cnt);
+        (void)hipFree(en);
+        throw;
+    }
+    (void)hipFree(cnt);
+    (void)hipFree(en);
+    for (int b = 0; b < blocks; ++b)
+        for (int k = 0; k < kColors; ++k) {
+            (b < cellBlocks ? st.numCells : st.numParticles)[k] += hc[size_t(b) * kColors + k];
+            st.totalEnergy[k] += he[size_t(b) * kColors + k];
+        }
+    return st;
+}
+
 void WorldSet::cellMap(int wi, std::vector<int>& bins, std::vector<int>& slots) {
     synchronize();
     const Ctx c = makeCtx(parity_);

@@ -109,6 +109,7 @@ public:
     void step(uint64_t n);
     void extract(int w, WorldImage& out);
     alien_totals totals(int w);
+    alien_statistics statistics(int w);  // per-colour counts and energies, reduced on the device
     // cell map of the current state: bin of each cell in map order, and its slot
     void cellMap(int w, std::vector<int>& bins, std::vector<int>& slots);
     // neighbour lists under the step predicate: per slot, neighbour slots in visiting order

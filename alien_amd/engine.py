@@ -11,7 +11,7 @@ import os
 
 import numpy as np
 
-from .abi import CELL_DTYPE, PARTICLE_DTYPE, JobInfo, Params, Totals, WorldDesc
+from .abi import CELL_DTYPE, PARTICLE_DTYPE, JobInfo, Params, Statistics, Totals, WorldDesc
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libalien_hip.so")
@@ -66,6 +66,7 @@ def lib():
         L.alien_get_timestep.restype = u64
         L.alien_set_timestep.argtypes = [vp, u64]
         L.alien_get_totals.argtypes = [vp, ctypes.POINTER(Totals)]
+        L.alien_get_statistics.argtypes = [vp, ctypes.POINTER(Statistics)]
         L.alien_clear.argtypes = [vp]
         L.alien_debug_get_map.argtypes = [vp, vp, vp, u64, pu64, ctypes.POINTER(i32), ctypes.POINTER(i32),
                                           ctypes.POINTER(i32)]
@@ -221,6 +222,12 @@ class Simulation:
         t = Totals()
         check(lib().alien_get_totals(self._h, ctypes.byref(t)))
         return t.as_dict()
+
+    def statistics(self):
+        """Per-colour cell and particle counts and energies, reduced on the device."""
+        s = Statistics()
+        check(lib().alien_get_statistics(self._h, ctypes.byref(s)))
+        return s.as_dict()
 
     def clear(self):
         check(lib().alien_clear(self._h))

@@ -128,6 +128,17 @@ typedef struct alien_totals {
     uint64_t numParticles;
 } alien_totals;
 
+/* Per-colour counts and energies of one world: TimestepStatistics (EngineInterface/
+ * RawStatisticsData.h:6-19) restricted to the physics subset (the replicator, virus, colony and
+ * genome fields are zero without cell functions, and every cell is a free cell).  Energies are
+ * summed in double in a fixed order. */
+typedef struct alien_statistics {
+    int32_t numCells[7];
+    int32_t numParticles[7];
+    double totalEnergy[7]; /* cells and energy particles of the colour */
+    uint64_t timestep;
+} alien_statistics;
+
 typedef struct alien_world alien_world;
 
 /* ---- library ------------------------------------------------------------------------------- */
@@ -166,6 +177,10 @@ uint64_t alien_get_timestep(const alien_world* w);
 int alien_set_timestep(alien_world* w, uint64_t timestep);
 /* Totals for conservation checks (getRawStatistics subset). */
 int alien_get_totals(alien_world* w, alien_totals* out);
+/* Per-colour statistics, computed on the device (StatisticsKernels.cu:11-45, cudaUpdateTimestep-
+ * Statistics_substep2: cell and particle counts and energies by colour; SimulationFacade.h
+ * getStatisticsHistory / getRawStatistics). */
+int alien_get_statistics(alien_world* w, alien_statistics* out);
 /* clear (SimulationCudaFacade.cuh:89): removes all cells and particles. */
 int alien_clear(alien_world* w);
 

@@ -128,6 +128,18 @@ class OracleWorld:
         return off, ids[: m.value]
 
 
+def statistics(cells, particles):
+    """Per-colour counts and energies of a world's data (StatisticsKernels.cu:11-45,
+    cudaUpdateTimestepStatistics_substep2: numCells and energy per cell colour, numParticles and
+    energy per particle colour), energies summed in double."""
+    n_cells = np.bincount(cells["color"].astype(np.int64) % 7, minlength=7)
+    n_parts = np.bincount(particles["color"].astype(np.int64) % 7, minlength=7)
+    energy = np.bincount(cells["color"].astype(np.int64) % 7, weights=cells["energy"].astype(np.float64), minlength=7)
+    energy = energy + np.bincount(particles["color"].astype(np.int64) % 7,
+                                  weights=particles["energy"].astype(np.float64), minlength=7)
+    return {"numCells": n_cells.tolist(), "numParticles": n_parts.tolist(), "totalEnergy": energy.tolist()}
+
+
 def from_world(world, precision=32, threads=1):
     o = OracleWorld(world.size, world.params, precision=precision, threads=threads)
     o.set_data(world.cells, world.particles)

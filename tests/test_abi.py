@@ -53,6 +53,7 @@ CTYPES = {
     "alien_params": abi.Params,
     "alien_world_desc": abi.WorldDesc,
     "alien_totals": abi.Totals,
+    "alien_statistics": abi.Statistics,
     "alien_job_info": abi.JobInfo,
     "alien_kernel_time": engine.KernelTime,
 }

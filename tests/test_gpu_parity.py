@@ -192,6 +192,28 @@ def test_1000_step_drift_within_1e3_of_fp64():
     assert np.abs(mg - mo).max() <= 1e-3 * np.abs(mo).max()
 
 
+@pytest.mark.parametrize("motion", [MOTION_FLUID, MOTION_COLLISION])
+def test_statistics_by_colour_match_the_oracle(motion):
+    """Per-colour counts and energies reduced on the device equal those of the oracle's state
+    (counts exactly, energies to double rounding), and add up to the totals."""
+    from oracle.pyoracle import statistics
+
+    w = worlds.random_mix(91, motion=motion, size=(96, 96), num_free=700, num_particles=400)
+    sim = engine.Simulation.from_world(w)
+    ref = from_world(w, threads=CPU_THREADS)
+    for steps in (0, 9):
+        sim.step(steps)
+        ref.step(steps)
+        st = sim.statistics()
+        exp = statistics(*ref.get_data())
+        assert st["timestep"] == sim.timestep == ref.timestep
+        assert st["numCells"] == exp["numCells"] and st["numParticles"] == exp["numParticles"]
+        np.testing.assert_allclose(st["totalEnergy"], exp["totalEnergy"], rtol=1e-12, atol=1e-12)
+        t = sim.totals()
+        assert sum(st["numCells"]) == t["numCells"] and sum(st["numParticles"]) == t["numParticles"]
+        assert abs(sum(st["totalEnergy"]) - t["energy"]) <= 1e-12 * max(1.0, t["energy"])
+
+
 # --------------------------------------------------------------------------------------------
 # edge cases
 

@@ -16,7 +16,7 @@ import math
 import numpy as np
 import pytest
 
-from alien_amd import worlds
+from alien_amd import abi, worlds
 from alien_amd.abi import MOTION_COLLISION, MOTION_FLUID
 from oracle.pyoracle import OracleWorld, from_world, lib
 
@@ -525,3 +525,19 @@ def test_angle_force_closed_form(oracle_lib):
     o.step(1)
     out1, _ = o.get_data()
     np.testing.assert_allclose(out1["pos"], c["pos"], atol=1e-6)
+
+
+def test_statistics_by_colour_closed_form():
+    """Per-colour counts and energies (StatisticsKernels.cu:11-45) on a hand-made world."""
+    from oracle.pyoracle import statistics
+
+    cells = np.zeros(3, dtype=abi.CELL_DTYPE)
+    cells["color"] = [0, 2, 2]
+    cells["energy"] = [1.5, 2.0, 0.25]
+    parts = np.zeros(2, dtype=abi.PARTICLE_DTYPE)
+    parts["color"] = [2, 6]
+    parts["energy"] = [3.0, 0.5]
+    st = statistics(cells, parts)
+    assert st["numCells"] == [1, 0, 2, 0, 0, 0, 0]
+    assert st["numParticles"] == [0, 0, 1, 0, 0, 0, 1]
+    assert st["totalEnergy"] == [1.5, 0.0, 5.25, 0.0, 0.0, 0.0, 0.5]
