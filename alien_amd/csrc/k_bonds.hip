@@ -260,12 +260,18 @@ __global__ void k_mark(Ctx c) {
     int x, wi;
     if (!staged_cell(c, x, wi)) return;
     {
+        // the cell or a bond partner flagged for deletion; the targets first, then their flags, so
+        // that the loads of all bonds are in flight together
         const DevWorld& w = c.worlds[wi];
-        uint8_t m = c.delFlag[x];
         const int nc = flags_num_conn(c.B.flags[x]);
-        for (int b = 0; b < nc && !m; ++b)
-            if (c.delFlag[w.cellOff + c.B.btgt[b * cap + x]]) m = 1;
-        c.mark[x] = m;
+        int t[kMaxBonds];
+#pragma unroll
+        for (int b = 0; b < kMaxBonds; ++b) t[b] = b < nc ? c.B.btgt[b * cap + x] : 0;
+        uint32_t m = c.delFlag[x];
+#pragma unroll
+        for (int b = 0; b < kMaxBonds; ++b)
+            if (b < nc) m |= c.delFlag[w.cellOff + t[b]];
+        c.mark[x] = m ? 1 : 0;
     }
 }
 
