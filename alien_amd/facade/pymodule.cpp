@@ -92,6 +92,26 @@ PYBIND11_MODULE(_facade, m) {
              })
         .def("getGpuName", &SimulationFacade::getGpuName)
         .def("getTps", &SimulationFacade::getTps)
+        .def("getStatistics",
+             [](SimulationFacade& f) {
+                 alien_statistics s;
+                 {
+                     py::gil_scoped_release rel;
+                     s = f.getStatistics();
+                 }
+                 py::dict d;
+                 py::list nc, np_, e;
+                 for (int k = 0; k < 7; ++k) {
+                     nc.append(s.numCells[k]);
+                     np_.append(s.numParticles[k]);
+                     e.append(s.totalEnergy[k]);
+                 }
+                 d["numCells"] = nc;
+                 d["numParticles"] = np_;
+                 d["totalEnergy"] = e;
+                 d["timestep"] = s.timestep;
+                 return d;
+             })
         .def("getRawStatistics",
              [](SimulationFacade& f) {
                  alien_totals t;

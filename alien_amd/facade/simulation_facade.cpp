@@ -222,6 +222,14 @@ alien_totals SimulationFacade::getRawStatistics() {
     return t;
 }
 
+alien_statistics SimulationFacade::getStatistics() {
+    Guard g(*this);
+    throwIfWorkerFailed();
+    alien_statistics s{};
+    check(alien_get_statistics(_world, &s));
+    return s;
+}
+
 float SimulationFacade::getTps() const { return _tps.load(); }
 
 void SimulationFacade::testOnly_failNextWorkerStep(std::string const& message) {

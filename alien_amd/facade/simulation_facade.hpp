@@ -75,6 +75,8 @@ public:
     GeneralSettings getGeneralSettings() const;
     IntVector2D getWorldSize() const;
     alien_totals getRawStatistics();
+    // per-colour counts and energies (the TimestepStatistics colour vectors of getRawStatistics)
+    alien_statistics getStatistics();
     float getTps() const;
 
     // test hooks: make the worker thread fail on its next step (failure detection path), and set

@@ -454,6 +454,9 @@ def test_cpp_facade_end_to_end():
     assert H.packed_bytes(fc) == H.packed_bytes(oc) and H.packed_bytes(fp) == H.packed_bytes(op)
     assert f.getWorldSize() == tuple(w.size)
     assert f.getGpuName()
+    st, tot = f.getStatistics(), f.getRawStatistics()
+    assert st["timestep"] == 9 and sum(st["numCells"]) == tot["numCells"] == len(oc)
+    assert sum(st["numParticles"]) == tot["numParticles"] == len(op)
     # failure detection: a failing worker step surfaces as an error, then the facade refuses work
     f.testOnly_failNextWorkerStep("injected failure")
     with pytest.raises(RuntimeError, match="injected failure"):
