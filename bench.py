@@ -224,7 +224,7 @@ def main():
         copy_calibration()
         return
     if args.gpus != 1:
-        print(f"bench.py measures one GPU: this build has no multi-GPU path (--gpus {args.gpus} requested); "
+        print(f"bench.py measures one GPU: the library has no multi-GPU path (--gpus {args.gpus} requested); "
               "independent worlds would shard one world per GPU with no exchange.", file=sys.stderr)
         sys.exit(2)
 
