@@ -42,8 +42,10 @@
 //       as products with the correctly rounded reciprocal (one division instead of several).
 // Cells are never detached here (detached cells come from cell functions), so the detached rule
 // of the neighbour predicate never applies and is not restated.
-// Parity status: pinned by tests/test_oracle.py against closed forms, invariants, brute force
-// and the reference's own test scenarios (see the docstring there for which test pins what).
+// Parity status: pinned by tests/test_oracle.py and tests/test_oracle_pinning.py against closed
+// forms written from the reference formulas, invariants, brute force and the reference's own test
+// scenarios (their docstrings say which test pins what).  Not pinned beyond its invariants: which
+// of two conflicting fusions or absorptions wins (R5, R8), a definition of this build.
 #pragma once
 
 #include <algorithm>
@@ -573,10 +575,23 @@ public:
         std::vector<std::vector<std::pair<int, int>>> fusion;  // per cell i: (i, j) with i < j
     };
 
+    // The force the visit of cell `self` to a neighbour adds to `self` (its negation goes to the
+    // neighbour): CellProcessor.cuh:319-339.  posDelta = self - other, velDelta = v_self - v_other.
+    V2<R> collisionVisitForce(const Cell<R>& self, V2<R> posDelta, V2<R> velDelta) const {
+        const R distance = length(posDelta);
+        const bool approaching = dot(posDelta, velDelta) < R(0);
+        const R barrierFactor = self.barrier ? R(2) : R(1);
+        if (length(self.vel) > R(0.5f) && approaching) {
+            R dsq = distance * distance + R(0.25f);
+            return posDelta * dot(velDelta, posDelta) * (R(1) / (R(-2) * dsq)) * barrierFactor;  // R16
+        }
+        return normalized(posDelta) * (p.cellMaxCollisionDistance - length(posDelta)) * p.cellRepulsionStrength *
+               barrierFactor;
+    }
+
     // collision mode: CellProcessor.cuh:285-365 (R3, R4, R13)
     void collisionCell(int i, PassOut& out) const {
         const auto& c = cells[i];
-        const R r = p.cellMaxCollisionDistance;
         const R cmd = p.cellMinDistance;
         V2<R> F{R(0), R(0)};
         V2<R> pc = c.pos;
@@ -589,30 +604,10 @@ public:
             if (bonded(i, j)) return;
             V2<R> velDelta = c.vel - o.vel;
             bool approaching = dot(posDelta, velDelta) < R(0);
-            // f(i,j): force i's thread adds to itself
-            V2<R> fij;
-            {
-                R bf = c.barrier ? R(2) : R(1);
-                if (length(c.vel) > R(0.5f) && approaching) {
-                    R dsq = distance * distance + R(0.25f);
-                    fij = posDelta * dot(velDelta, posDelta) * (R(1) / (R(-2) * dsq)) * bf;  // R16
-                } else {
-                    fij = normalized(posDelta) * (r - length(posDelta)) * p.cellRepulsionStrength * bf;
-                }
-            }
-            // f(j,i): force j's thread adds to itself; i receives its negation
-            V2<R> fji;
-            {
-                V2<R> pd{-posDelta.x, -posDelta.y};
-                V2<R> vd{-velDelta.x, -velDelta.y};
-                R bf = o.barrier ? R(2) : R(1);
-                if (length(o.vel) > R(0.5f) && approaching) {
-                    R dsq = distance * distance + R(0.25f);
-                    fji = pd * dot(vd, pd) * (R(1) / (R(-2) * dsq)) * bf;  // R16
-                } else {
-                    fji = normalized(pd) * (r - length(pd)) * p.cellRepulsionStrength * bf;
-                }
-            }
+            // f(i,j): force i's visit adds to i; f(j,i): force j's visit adds to j, i receives its
+            // negation
+            const V2<R> fij = collisionVisitForce(c, posDelta, velDelta);
+            const V2<R> fji = collisionVisitForce(o, V2<R>{-posDelta.x, -posDelta.y}, V2<R>{-velDelta.x, -velDelta.y});
             F = F + fij;
             F = F + V2<R>{-fji.x, -fji.y};
             if (i < j && c.numConn < c.maxConn && o.numConn < o.maxConn && length(velDelta) >= p.cellFusionVelocity &&

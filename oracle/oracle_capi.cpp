@@ -184,3 +184,16 @@ double oracle_subtract_angle(int precision, double a, double b) {
 }
 
 }  // extern "C"
+
+extern "C" {
+// Math::crossing of segment (s, e) with (os, oe), exposed for its closed-form test
+int oracle_crossing(int precision, const double* s, const double* e, const double* os, const double* oe) {
+    if (precision == 32) {
+        using V = oracle::V2<float>;
+        return oracle::crossing<float>(V{float(s[0]), float(s[1])}, V{float(e[0]), float(e[1])},
+                                       V{float(os[0]), float(os[1])}, V{float(oe[0]), float(oe[1])});
+    }
+    using V = oracle::V2<double>;
+    return oracle::crossing<double>(V{s[0], s[1]}, V{e[0], e[1]}, V{os[0], os[1]}, V{oe[0], oe[1]});
+}
+}  // extern "C"
