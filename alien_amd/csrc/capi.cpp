@@ -152,7 +152,31 @@ int alien_set_params(alien_world* w, const alien_params* params) {
     return guarded([&] {
         require(w && params, "null argument");
         alien::validate_world(w->desc, *params);
-        if (!w->poisoned) w->set->setParams(0, *params);
+        if (!w->poisoned) {
+            if (alien::bin_size_for(*params) != alien::bin_size_for(w->params)) {
+                // the bin grid changes: a fresh set is built around the new parameters (density
+                // reset pending), and replaces the old one only once it is complete
+                WorldImage img;
+                w->set->extract(0, img);
+                img.host.params = *params;
+                img.densityResetPending = true;
+                std::vector<WorldSet::Source> src(1);
+                src[0].image = &img;
+                auto fresh = std::make_unique<WorldSet>();
+                fresh->setBlockSize(w->set->blockSize());
+                fresh->setKernelTiming(w->set->kernelTiming());
+                fresh->build(src);
+                w->set = std::move(fresh);
+            } else {
+                try {
+                    w->set->setParams(0, *params);
+                } catch (const Error& e) {
+                    w->poisoned = true;
+                    w->poisonReason = e.what();
+                    throw;
+                }
+            }
+        }
         w->params = *params;
     });
 }
@@ -175,6 +199,7 @@ int alien_set_data(alien_world* w, const alien_cell_to* cells, uint64_t numCells
         src[0].image = &img;
         auto fresh = std::make_unique<WorldSet>();
         fresh->setBlockSize(w->set->blockSize());
+        fresh->setKernelTiming(w->set->kernelTiming());
         fresh->build(src);
         w->set = std::move(fresh);
         w->poisoned = false;
@@ -344,6 +369,7 @@ struct alien_batch {
     std::vector<WorldImage> pending;  // added but not yet resident
     std::unique_ptr<WorldSet> set;
     int resident = 0;
+    uint64_t cells = 0, particles = 0;  // totals of the resident and pending worlds
     int blockSize = 256;
     int stripCap = kStripCapMax;  // launch shape chosen by the autotuner, kept when worlds are added
     int chunkRows = 0;
@@ -394,7 +420,11 @@ int alien_batch_add_world(alien_batch* b, const alien_world_desc* desc, const al
                           uint64_t numParticles, int32_t* worldIndex) {
     return guarded([&] {
         require(b && desc && params, "null argument");
+        // the batch is one world set: its totals must stay within the set limits
+        alien::check_set_size(b->cells + numCells, b->particles + numParticles);
         b->pending.push_back(WorldImage::fromTO(*desc, *params, cells, numCells, particles, numParticles));
+        b->cells += numCells;
+        b->particles += numParticles;
         if (worldIndex) *worldIndex = b->resident + int(b->pending.size()) - 1;
     });
 }
@@ -612,8 +642,12 @@ struct alien_server {
                 }
             }
             std::vector<uint64_t> admitted;
-            uint64_t used = 0;
-            for (auto id : keep) used += jobs.at(id)->bytes;
+            uint64_t used = 0, cells = 0, parts = 0;
+            for (auto id : keep) {
+                used += jobs.at(id)->bytes;
+                cells += jobs.at(id)->numCells;
+                parts += jobs.at(id)->numParticles;
+            }
             while (!queue.empty()) {
                 uint64_t id = queue.front();
                 auto it = jobs.find(id);
@@ -623,8 +657,13 @@ struct alien_server {
                     continue;
                 }
                 Job& j = *it->second;
-                if (used + j.bytes > budget) break;
+                // the budget, and the size limits of one world set (alien::kMaxSetCells)
+                if (used + j.bytes > budget || cells + j.numCells > uint64_t(alien::kMaxSetCells) ||
+                    parts + j.numParticles > uint64_t(alien::kMaxSetParticles))
+                    break;
                 used += j.bytes;
+                cells += j.numCells;
+                parts += j.numParticles;
                 admitted.push_back(id);
                 queue.pop_front();
             }

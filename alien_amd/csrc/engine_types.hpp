@@ -25,6 +25,13 @@ constexpr int kRowPool = 256;  // entries of three consecutive staged rows
 constexpr int kStripMax = 64;
 constexpr int kChunkMax = 8;  // bin rows per neighbour-pass chunk
 constexpr int kClaimFree = 0x7fffffff;
+// Size limits of one world set.  Cell indices, bin offsets and the bond planes ([kMaxBonds][cap],
+// addressed as b * cap + x) are 32-bit ints in the kernels, so a set holds at most
+// (2^31 - 1) / kMaxBonds cells; particle indices and the uint32 bin scan over cells plus particles
+// stay in range with at most 2^30 particles.  At ~470 B per bonded cell the cell limit is ~168 GB
+// of the 288 GB of HBM.
+constexpr int64_t kMaxSetCells = (int64_t(0x7fffffff)) / kMaxBonds;  // 357,913,941
+constexpr int64_t kMaxSetParticles = int64_t(1) << 30;
 
 // cell flags word
 constexpr uint32_t kNumConnMask = 0xFu;        // bits 0..3

@@ -40,6 +40,14 @@ void validate_params(const alien_params& p) {
     if (r > 64.0f) throw Error(ALIEN_E_ARG, "interaction radius above 64 is not supported");
 }
 
+void check_set_size(uint64_t cells, uint64_t particles) {
+    if (cells > uint64_t(kMaxSetCells))
+        throw Error(ALIEN_E_CAPACITY, "more than " + std::to_string(kMaxSetCells) +
+                                          " cells in one world set (32-bit bond-plane offsets)");
+    if (particles > uint64_t(kMaxSetParticles))
+        throw Error(ALIEN_E_CAPACITY, "more than " + std::to_string(kMaxSetParticles) + " energy particles in one world set");
+}
+
 void validate_world(const alien_world_desc& d, const alien_params& p) {
     validate_params(p);
     int s = bin_size_for(p);
@@ -121,8 +129,7 @@ static inline void host_wrap(float& x, float& y, int W, int H) {
 WorldImage WorldImage::fromTO(const alien_world_desc& desc, const alien_params& params, const alien_cell_to* cells,
                               uint64_t numCells, const alien_particle_to* particles, uint64_t numParticles) {
     validate_world(desc, params);
-    if (numCells > uint64_t(0x3fffffff) || numParticles > uint64_t(0x3fffffff))
-        throw Error(ALIEN_E_ARG, "too many cells or particles for one world");
+    check_set_size(numCells, numParticles);
     if ((numCells && !cells) || (numParticles && !particles)) throw Error(ALIEN_E_ARG, "null transfer array");
     WorldImage img;
     const int n = int(numCells), np = int(numParticles);
@@ -155,6 +162,7 @@ WorldImage WorldImage::fromTO(const alien_world_desc& desc, const alien_params& 
     for (int i = 0; i < n; ++i) {
         const alien_cell_to& c = cells[i];
         if (c.numConnections > kMaxBonds) throw Error(ALIEN_E_ARG, "numConnections above 6");
+        if (c.color >= kColors) throw Error(ALIEN_E_ARG, "cell color must be below 7 (MAX_COLORS)");
         float x = c.pos[0], y = c.pos[1];
         if (!std::isfinite(x) || !std::isfinite(y)) throw Error(ALIEN_E_ARG, "non-finite cell position");
         host_wrap(x, y, W, H);
@@ -202,6 +210,7 @@ WorldImage WorldImage::fromTO(const alien_world_desc& desc, const alien_params& 
     h.particleSelected.resize(np);
     for (int i = 0; i < np; ++i) {
         const alien_particle_to& p = particles[i];
+        if (p.color >= kColors) throw Error(ALIEN_E_ARG, "particle color must be below 7 (MAX_COLORS)");
         float x = p.pos[0], y = p.pos[1];
         if (!std::isfinite(x) || !std::isfinite(y)) throw Error(ALIEN_E_ARG, "non-finite particle position");
         host_wrap(x, y, W, H);
@@ -410,7 +419,25 @@ void WorldSet::build(const std::vector<Source>& sources) {
     std::vector<uint8_t> pending;
     std::vector<uint8_t> bondWork;
     std::vector<int> stripChoice, stageChoice;
-    int cellOff = 0, partOff = 0, bins = 0;
+    // A set is built once, into fresh allocations; a failed build then leaves nothing behind (the
+    // destructor frees what was allocated).  Sizes are checked before anything is read or allocated.
+    if (!allocs_.empty() || !worlds_.empty())
+        throw Error(ALIEN_E_STATE, "a world set is built once; build a fresh set to change its worlds");
+    {
+        uint64_t cells = 0, parts = 0;
+        for (const auto& src : sources) {
+            if (src.from == this) throw Error(ALIEN_E_ARG, "a world set cannot be rebuilt from itself");
+            if (src.image) {
+                cells += uint64_t(src.image->host.numCells);
+                parts += uint64_t(src.image->numParticles);
+            } else {
+                cells += uint64_t(src.from->worlds_.at(src.fromIndex).numCells);
+                parts += uint64_t(src.from->worlds_.at(src.fromIndex).partCap);
+            }
+        }
+        check_set_size(cells, parts);
+    }
+    int64_t cellOff = 0, partOff = 0, bins = 0;
     for (const auto& src : sources) {
         HostWorld h;
         int live = 0;
@@ -419,7 +446,6 @@ void WorldSet::build(const std::vector<Source>& sources) {
         int strip = kStripMax, stage = 1;
         bool work = false;
         if (src.image) {
-            if (src.from == this) throw Error(ALIEN_E_ARG, "a world set cannot be rebuilt from itself");
             h = src.image->host;
             live = src.image->numParticles;
             ts = src.image->timestep;
@@ -434,7 +460,6 @@ void WorldSet::build(const std::vector<Source>& sources) {
             strip = choose_strip(im.pos, h.desc.worldSizeX, h.desc.worldSizeY, bs,
                                  h.params.motionType == ALIEN_MOTION_COLLISION, stage);
         } else {
-            if (src.from == this) throw Error(ALIEN_E_ARG, "a world set cannot be rebuilt from itself");
             h = src.from->hosts_.at(src.fromIndex);
             live = src.from->numParticles(src.fromIndex);
             ts = src.from->timestep(src.fromIndex);
@@ -453,10 +478,10 @@ void WorldSet::build(const std::vector<Source>& sources) {
         w.nby = w.H / w.binSize;
         w.pnbx = std::max(1, w.W / kParticleBin);
         w.pnby = std::max(1, w.H / kParticleBin);
-        w.cellOff = cellOff;
+        w.cellOff = int(cellOff);
         w.numCells = h.numCells;
-        w.binOff = bins;
-        w.partOff = partOff;
+        w.binOff = int(bins);
+        w.partOff = int(partOff);
         w.partCap = live;
         w.bondWork = work ? 1 : 0;
         strip_shape(w, std::min(strip, stripCap_), stage);
@@ -464,7 +489,9 @@ void WorldSet::build(const std::vector<Source>& sources) {
         stageChoice.push_back(stage);
         cellOff += h.numCells;
         partOff += live;
-        bins += w.nbx * w.nby;
+        bins += int64_t(w.nbx) * w.nby;
+        if (bins + int64_t(w.pnbx) * w.pnby >= (int64_t(1) << 31) - 2)
+            throw Error(ALIEN_E_CAPACITY, "too many spatial bins in one world set");
         hosts.push_back(std::move(h));
         worlds.push_back(w);
         params.push_back(to_dev_params(hosts.back().params));
@@ -473,24 +500,18 @@ void WorldSet::build(const std::vector<Source>& sources) {
         pending.push_back(pend ? 1 : 0);
         bondWork.push_back(work ? 1 : 0);
     }
-    const int cellBins = bins;  // cell bins of all worlds come first, then the particle bins
+    const int cellBins = int(bins);  // cell bins of all worlds come first, then the particle bins
     for (auto& w : worlds) {
-        w.pbinOff = bins;
-        bins += w.pnbx * w.pnby;
+        w.pbinOff = int(bins);
+        bins += int64_t(w.pnbx) * w.pnby;
     }
+    if (bins >= (int64_t(1) << 31) - 2) throw Error(ALIEN_E_CAPACITY, "too many spatial bins in one world set");
 
-    // allocate the new buffers into a fresh set of allocations, copy, then swap in
     WorldSet& me = *this;
-    std::vector<Arr> oldAllocs;
-    oldAllocs.swap(allocs_);
-    size_t oldBytes = deviceBytes_;
-    deviceBytes_ = 0;
-    HotBuffers oldBuf[2] = {buf_[0], buf_[1]};
-    (void)oldBuf;
-    (void)oldBytes;
     invalidateGraphs();
 
-    const int NC = std::max(cellOff, 1), NP = std::max(partOff, 1), NB = std::max(bins, 1);
+    const int NC = int(std::max<int64_t>(cellOff, 1)), NP = int(std::max<int64_t>(partOff, 1));
+    const int NB = int(std::max<int64_t>(bins, 1));
     cellCap_ = NC;
     partCap_ = NP;
     numBins_ = NB;
@@ -615,8 +636,6 @@ void WorldSet::build(const std::vector<Source>& sources) {
         }
     }
     check_hip(hipStreamSynchronize(stream_), "hipStreamSynchronize");
-    // release the previous buffers of this set (sources from other sets were copied already)
-    for (auto& a : oldAllocs) (void)hipFree(a.p);
 
     hosts_ = std::move(hosts);
     worlds_ = std::move(worlds);
@@ -631,7 +650,7 @@ void WorldSet::build(const std::vector<Source>& sources) {
     parity_ = 0;
     shape_.candGrid = 16;
     // one particle block per workgroup (blocks past a world's live particles exit at once)
-    shape_.partGrid = std::max(1, std::min(65536, (partOff + shape_.partThreads - 1) / shape_.partThreads));
+    shape_.partGrid = int(std::max<int64_t>(1, std::min<int64_t>(65536, (partOff + shape_.partThreads - 1) / shape_.partThreads)));
 
     std::vector<DevCounters> ctr(worlds_.size());
     for (size_t i = 0; i < worlds_.size(); ++i) {
@@ -982,18 +1001,8 @@ void WorldSet::setTimestep(int w, uint64_t t) {
 
 void WorldSet::setParams(int w, const alien_params& p) {
     validate_world(hosts_.at(w).desc, p);
-    const bool geometryChanges = bin_size_for(p) != bin_size_for(hosts_.at(w).params);
-    if (geometryChanges) {
-        // bin grid changes: rebuild the layout around the new parameters
-        std::vector<WorldImage> imgs(worlds_.size());
-        for (int i = 0; i < numWorlds(); ++i) extract(i, imgs[i]);
-        imgs[w].host.params = p;
-        imgs[w].densityResetPending = true;
-        std::vector<Source> src(imgs.size());
-        for (size_t i = 0; i < imgs.size(); ++i) src[i].image = &imgs[i];
-        build(src);
-        return;
-    }
+    if (bin_size_for(p) != bin_size_for(hosts_.at(w).params))
+        throw Error(ALIEN_E_STATE, "the bin grid changes with these parameters: build a fresh set around them");
     hosts_.at(w).params = p;
     params_.at(w) = to_dev_params(p);
     check_hip(hipMemcpy(dParams_ + w, &params_[w], sizeof(DevParams), hipMemcpyHostToDevice), "H2D");

@@ -32,6 +32,9 @@ void check_hip(hipError_t e, const char* what);
 int bin_size_for(const alien_params& p);
 void validate_params(const alien_params& p);
 void validate_world(const alien_world_desc& d, const alien_params& p);
+// throws ALIEN_E_CAPACITY when a world set of this many cells / particles exceeds kMaxSetCells /
+// kMaxSetParticles (checked before any transfer array is read or any device memory is allocated)
+void check_set_size(uint64_t cells, uint64_t particles);
 DevParams to_dev_params(const alien_params& p);
 int choose_strip(const std::vector<float2>& pos, int W, int H, int binSize, bool twoRowStages, int& rowsPerStage);
 void strip_shape(DevWorld& w, int strip, int rowsPerStage);
@@ -103,6 +106,8 @@ public:
     int liveParticles(int w);  // the particles get_data returns
     uint64_t timestep(int w);
     void setTimestep(int w, uint64_t t);
+    // new parameters of world w with the same bin grid (bin_size_for unchanged); a change of the
+    // grid needs a fresh set built around the new parameters
     void setParams(int w, const alien_params& p);
 
     // advance every world by n timesteps
@@ -129,6 +134,7 @@ public:
     void setUseGraph(bool on);
     // kernel timing mode: no graphs, every launch bracketed by events (see alien_set_kernel_timing)
     void setKernelTiming(bool on);
+    bool kernelTiming() const { return timing_; }
     // neighbour-pass fallback counts since the data was set: chunks, cells
     void pathCounts(uint64_t& fallbackChunks, uint64_t& directCells);
     struct KernelTime {

@@ -400,6 +400,7 @@ public:
             d.activationTime = s.activationTime;
             d.genomeComplexity = s.genomeComplexity;
             if (d.numConn > ALIEN_MAX_CELL_BONDS) throw std::invalid_argument("numConnections > 6");
+            if (d.color >= ALIEN_MAX_COLORS) throw std::invalid_argument("cell color must be below 7");
             for (int b = 0; b < d.numConn; ++b) {
                 int t = s.connections[b].cellIndex;
                 if (t < 0 || uint64_t(t) >= n) throw std::invalid_argument("bond index out of range");
@@ -423,6 +424,7 @@ public:
             d.vel = {R(q[i].vel[0]), R(q[i].vel[1])};
             d.energy = R(q[i].energy);
             d.color = q[i].color;
+            if (d.color >= ALIEN_MAX_COLORS) throw std::invalid_argument("particle color must be below 7");
             d.selected = q[i].selected;
             d.hasLast = false;
             d.lastAbsorbedCellId = 0;

@@ -132,3 +132,85 @@ def test_cell_record_round_trips_through_numpy():
     raw = c.tobytes()
     assert len(raw) == 3 * abi.CELL_DTYPE.itemsize
     assert np.frombuffer(raw, dtype=abi.CELL_DTYPE)["connections"]["cellIndex"][1, 2] == 7
+
+
+MAX_SET_CELLS = (2**31 - 1) // 6  # bond planes are addressed as b * cap + x in 32-bit ints
+
+
+def _desc(size=64):
+    d = abi.WorldDesc()
+    d.worldSizeX = d.worldSizeY = size
+    return d
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def test_set_size_limit_is_checked_before_any_read():
+    """A world set holds at most (2^31 - 1) / 6 cells: larger requests fail with ALIEN_E_CAPACITY
+    before the transfer array is read (here a one-cell buffer with a count of 400M), for one
+    batch world, for a batch's running total, and for a serving job.  No device call is made."""
+    L = engine.lib()
+    one = np.zeros(1, dtype=abi.CELL_DTYPE)
+    nopart = np.zeros(1, dtype=abi.PARTICLE_DTYPE)
+    d, p = _desc(), abi.default_params(motionType=abi.MOTION_COLLISION)
+    b = ctypes.c_void_p()
+    assert L.alien_batch_create(ctypes.byref(b)) == abi.OK
+    idx = ctypes.c_int32()
+    add = lambda cells, n: L.alien_batch_add_world(b, ctypes.byref(d), ctypes.byref(p), _ptr(cells), ctypes.c_uint64(n),
+                                                   _ptr(nopart), ctypes.c_uint64(0), ctypes.byref(idx))
+    assert add(one, 400_000_000) == abi.E_CAPACITY
+    assert b"cells in one world set" in L.alien_last_error()
+    assert add(one, MAX_SET_CELLS + 1) == abi.E_CAPACITY
+    small = np.zeros(200, dtype=abi.CELL_DTYPE)
+    small["pos"] = np.random.default_rng(0).uniform(0, 64, size=(200, 2))
+    assert add(small, 200) == abi.OK
+    assert add(one, MAX_SET_CELLS - 100) == abi.E_CAPACITY  # 200 + that exceeds the limit
+    assert L.alien_batch_num_worlds(b) == 1
+    assert L.alien_batch_destroy(b) == abi.OK
+
+    s = ctypes.c_void_p()
+    assert L.alien_server_create(ctypes.c_uint64(1 << 40), 50, ctypes.byref(s)) == abi.OK
+    jid = ctypes.c_uint64()
+    rc = L.alien_server_submit(s, ctypes.byref(d), ctypes.byref(p), _ptr(one), ctypes.c_uint64(400_000_000),
+                               _ptr(nopart), ctypes.c_uint64(0), ctypes.c_uint64(10), ctypes.byref(jid))
+    assert rc == abi.E_CAPACITY
+    assert L.alien_server_destroy(s) == abi.OK
+
+
+def test_estimator_at_the_set_limit():
+    """The admission estimate grows linearly with cells and particles and counts every per-cell
+    array at the largest set (a 400M-cell request would need more than 160 GB)."""
+    L = engine.lib()
+    L.alien_server_estimate_bytes.restype = ctypes.c_uint64
+    L.alien_server_estimate_bytes.argtypes = [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int32, ctypes.c_int32]
+    e0 = L.alien_server_estimate_bytes(0, 0, 4096, 4096)
+    e1 = L.alien_server_estimate_bytes(100_000_000, 0, 4096, 4096)
+    e4 = L.alien_server_estimate_bytes(400_000_000, 0, 4096, 4096)
+    assert e4 - e0 == 4 * (e1 - e0)
+    assert (e1 - e0) / 100_000_000 > 400  # bytes per cell: two hot copies with six bond planes, scratch
+    assert e4 > 160e9
+
+
+@pytest.mark.parametrize("what", ["cell", "particle"])
+def test_colour_above_the_palette_is_rejected(what):
+    """ALIEN has colours 0..6 (MAX_COLORS = 7); a transfer object with colour 7 or more is refused
+    with ALIEN_E_ARG instead of being read as colour % 7."""
+    L = engine.lib()
+    cells = np.zeros(2, dtype=abi.CELL_DTYPE)
+    cells["pos"] = [[1.0, 1.0], [5.0, 5.0]]
+    parts = np.zeros(1, dtype=abi.PARTICLE_DTYPE)
+    parts["pos"] = [2.0, 2.0]
+    (cells if what == "cell" else parts)["color"][0] = 9
+    b = ctypes.c_void_p()
+    assert L.alien_batch_create(ctypes.byref(b)) == abi.OK
+    d, p = _desc(), abi.default_params(motionType=abi.MOTION_COLLISION)
+    rc = L.alien_batch_add_world(b, ctypes.byref(d), ctypes.byref(p), _ptr(cells), ctypes.c_uint64(2), _ptr(parts),
+                                 ctypes.c_uint64(1), None)
+    assert rc == abi.E_ARG and b"color" in L.alien_last_error()
+    (cells if what == "cell" else parts)["color"][0] = 6
+    rc = L.alien_batch_add_world(b, ctypes.byref(d), ctypes.byref(p), _ptr(cells), ctypes.c_uint64(2), _ptr(parts),
+                                 ctypes.c_uint64(1), None)
+    assert rc == abi.OK
+    assert L.alien_batch_destroy(b) == abi.OK
