@@ -45,6 +45,10 @@ def parse():
                     help="large_fluid: the headline line; batch: 256 uneven worlds in one graph vs per-world "
                          "sequential runs; serving: Poisson job arrivals through the job server")
     ap.add_argument("--batch-worlds", type=int, default=256)
+    ap.add_argument("--batch-steps", type=int, default=500,
+                    help="steps per call of the batched-worlds leg of the default line (config 3: 500)")
+    ap.add_argument("--no-batch", action="store_true",
+                    help="default line without its batched-worlds leg (batched vs per-world sequential)")
     ap.add_argument("--serving-seconds", type=float, default=60.0, help="arrival window of the serving run")
     ap.add_argument("--serving-rate", type=float, default=20.0)
     ap.add_argument("--budget-gb", type=float, default=200.0)
@@ -70,14 +74,29 @@ def copy_calibration():
     print(json.dumps({"copy_calibration": {"bytes_read": 4 * n, "bytes_written": 4 * n, "launches": 5}}), flush=True)
 
 
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(world, seconds):
     from oracle.pyoracle import OracleWorld
 
+    # every core this process may run on, unless OMP_NUM_THREADS says fewer (a shared host's
+    # share of cores)
     try:
         cores = len(os.sched_getaffinity(0))
     except AttributeError:
         cores = os.cpu_count() or 1
-    cores = max(1, min(cores, int(os.environ.get("OMP_NUM_THREADS", cores)), 16))
+    if os.environ.get("OMP_NUM_THREADS", "").isdigit():
+        cores = min(cores, int(os.environ["OMP_NUM_THREADS"]))
+    cores = max(1, cores)
     o = OracleWorld(world.size, world.params, precision=32, threads=cores)
     o.set_data(world.cells, world.particles)
     o.step(1)
@@ -94,6 +113,7 @@ def cpu_baseline(world, seconds):
         "value": n * steps / el,
         "unit": "cell-timesteps/s",
         "cores": cores,
+        "cpu_model": cpu_model(),
         "kind": "port",
         "sample": f"oracle FP32 OpenMP, same {n}-cell world, steps 2..{steps + 1} after 1 warm-up step "
                   f"({el:.1f} s)",
@@ -112,53 +132,68 @@ def gen_processes():
     return max(1, min(n, 16))
 
 
-def bench_batch(args):
+def batch_vs_sequential(count, steps, warmup):
     """256 worlds of uneven size (side log-uniform in [256, 2048], density U(0.02, 0.3), soft-body
     patches, collision mode) advanced together, one captured graph replay per step, against the
     same worlds run one after another as plain single-world simulations of the same library."""
     from alien_amd import engine, worlds
 
     t0 = time.perf_counter()
-    ws = worlds.batched_worlds(count=args.batch_worlds, steps=args.steps, processes=gen_processes())
+    ws = worlds.batched_worlds(count=count, steps=steps, processes=gen_processes())
     gen_s = time.perf_counter() - t0
     cells = sum(len(w.cells) for w in ws)
     progress(f"generated {len(ws)} worlds, {cells} cells in {gen_s:.1f} s")
     b = engine.Batch()
     for w in ws:
         b.add(w)
-    b.step(args.warmup)
-    tune = b.autotune(trial_steps=max(2, min(10, args.steps)))
+    b.step(warmup)
+    tune = b.autotune(trial_steps=max(2, min(10, steps)))
     t0 = time.perf_counter()
-    b.step(args.steps)
+    b.step(steps)
     batched_s = time.perf_counter() - t0
     b.close()
-    progress(f"batched: {args.steps} steps in {batched_s:.2f} s (autotune block {tune[0]}, strip cap {tune[1]}, "
+    progress(f"batched: {steps} steps in {batched_s:.2f} s (autotune block {tune[0]}, strip cap {tune[1]}, "
              f"chunk rows {tune[2]})")
     # per-world sequential: each world alone, same steps, same library
     seq_s = 0.0
-    for w in ws:
+    for k, w in enumerate(ws):
         sim = engine.Simulation.from_world(w)
-        sim.step(args.warmup)
+        sim.step(warmup)
         t0 = time.perf_counter()
-        sim.step(args.steps)
+        sim.step(steps)
         seq_s += time.perf_counter() - t0
         sim.close()
-        if (ws.index(w) + 1) % 32 == 0:
-            progress(f"sequential: {ws.index(w) + 1} worlds, {seq_s:.2f} s of stepping")
-    value = cells * args.steps / batched_s
-    seq_value = cells * args.steps / seq_s
-    print(json.dumps({
-        "metric": "cell-timesteps/s (256 uneven worlds, batched, 1 GPU)", "value": value,
-        "unit": "cell-timesteps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": 1000.0 * batched_s / args.steps, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-        "config": {"workload": "batched_worlds", "worlds": len(ws), "cells": cells,
-                   "sides": [int(min(w.size[0] for w in ws)), int(max(w.size[0] for w in ws))],
-                   "seed": worlds.SEED_BASE + 3, "device": engine.gpu_name()},
-        "per_world_sequential": {"value": seq_value, "seconds": seq_s},
+        if (k + 1) % 32 == 0:
+            progress(f"sequential: {k + 1} worlds, {seq_s:.2f} s of stepping")
+    value = cells * steps / batched_s
+    seq_value = cells * steps / seq_s
+    return {
+        "value": value, "ms_per_step": 1000.0 * batched_s / steps, "steps": steps, "warmup": warmup,
+        "worlds": len(ws), "cells": cells,
+        "sides": [int(min(w.size[0] for w in ws)), int(max(w.size[0] for w in ws))],
+        "seed": worlds.SEED_BASE + 3,
+        "per_world_sequential": {"value": seq_value, "seconds": seq_s, "ms_per_step": 1000.0 * seq_s / steps},
         "batched_over_sequential": value / seq_value,
         "autotune": {"block": tune[0], "strip_cap": tune[1], "chunk_rows": tune[2], "trial_ms_per_step": tune[3]},
         "generation_s": gen_s,
+    }
+
+
+def bench_batch(args):
+    from alien_amd import engine
+
+    r = batch_vs_sequential(args.batch_worlds, args.steps, args.warmup)
+    print(json.dumps({
+        "metric": "cell-timesteps/s (256 uneven worlds, batched, 1 GPU)", "value": r["value"],
+        "unit": "cell-timesteps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": r["ms_per_step"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "batched_worlds", "worlds": r["worlds"], "cells": r["cells"], "sides": r["sides"],
+                   "seed": r["seed"], "device": engine.gpu_name()},
+        "per_world_sequential": r["per_world_sequential"],
+        "batched_over_sequential": r["batched_over_sequential"],
+        "autotune": r["autotune"],
+        "generation_s": r["generation_s"],
     }), flush=True)
 
 
@@ -298,17 +333,27 @@ def main():
                    "particles": args.particles, "motion": "fluid (SPH, radius 1.3)", "seed": world.seed,
                    "device": device},
         "roofline": roofline,
+        # the whole step against HBM: algorithmic bytes of all cells and particles per step
         "step_bandwidth_GBs": step_bytes / (elapsed / args.steps) / 1e9,
+        "step_frac": step_bytes / (elapsed / args.steps) / 1e9 / HBM_PEAK_GBS,
         # the host boundary: one set_data before and one get_data after the run (not per step)
         "host_boundary": {"set_data_s": set_s, "get_data_s": get_s,
                           "value_with_set_and_get": n * args.steps / (elapsed + set_s + get_s)},
         "live_particles_after": live_particles,
         "kernels": kernels,
     }
+    sim.close()
     if not args.no_cpu_baseline:
+        progress("cpu baseline")
         result["cpu_baseline"] = cpu_baseline(world, args.cpu_seconds)
     else:
         result["cpu_baseline"] = None
+    if not args.no_batch:
+        # the 256-world batch (500 steps per call, its own seed) against per-world sequential runs
+        progress("batched worlds")
+        b = batch_vs_sequential(args.batch_worlds, args.batch_steps, 10)
+        result["batch"] = b
+        result["batched_over_sequential"] = b["batched_over_sequential"]
     print(json.dumps(result), flush=True)
 
 

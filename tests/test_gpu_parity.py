@@ -153,13 +153,19 @@ def test_set_get_round_trip_is_exact():
 # tolerances against the FP64 oracle
 
 
-@pytest.mark.parametrize("name", ["plain_soft_body", "fluid_256k", "mix_collision"])
+@pytest.mark.parametrize("name", ["plain_soft_body", "fluid_256k", "mix_collision", "large_fluid_full",
+                                  "large_soft_body_full"])
 def test_single_step_within_1e5_of_fp64(name):
+    """One step, norm-wise relative error against the FP64 oracle, on the benchmark worlds at full
+    size (4M-cell fluid world with 500k particles; 3.4M-cell soft-body world) and on small ones."""
     w = {
         "plain_soft_body": lambda: worlds.plain_soft_body(),
         "fluid_256k": lambda: worlds.large_fluid(num_cells=256_000, num_particles=32_000, size=(1024, 1024)),
         "mix_collision": lambda: worlds.random_mix(13, motion=MOTION_COLLISION),
+        "large_fluid_full": lambda: worlds.large_fluid(),
+        "large_soft_body_full": lambda: worlds.large_soft_body(),
     }[name]()
+    say(f"single step vs FP64: {name}, {len(w.cells)} cells, {len(w.particles)} particles")
     sim = engine.Simulation.from_world(w)
     ref = from_world(w, precision=64, threads=CPU_THREADS)
     sim.step(1)
@@ -177,10 +183,19 @@ def test_single_step_within_1e5_of_fp64(name):
         assert H.normwise_rel(a["vel"][keep], b["vel"][keep]) <= 1e-5
 
 
-def test_1000_step_drift_within_1e3_of_fp64():
-    w = worlds.random_mix(81, motion=MOTION_COLLISION, num_barrier=0)
-    p = w.params.copy()
-    p.cellMaxVelocity = 1e9
+@pytest.mark.parametrize("name", ["mix_collision", "fluid_64k"])
+def test_1000_step_drift_within_1e3_of_fp64(name):
+    """Total energy and momentum after 1000 steps within 1e-3 (relative) of the FP64 oracle: a
+    collision mix without barriers and clamp, and a 65,536-cell fluid world with 8,192 energy
+    particles (the SPH path of the 4M-cell benchmark world at 1/64 of its size)."""
+    if name == "mix_collision":
+        w = worlds.random_mix(81, motion=MOTION_COLLISION, num_barrier=0)
+        p = w.params.copy()
+        p.cellMaxVelocity = 1e9
+    else:
+        w = worlds.large_fluid(num_cells=65_536, num_particles=8_192, size=(512, 512))
+        p = w.params
+    say(f"1000-step drift vs FP64: {name}, {len(w.cells)} cells")
     sim = engine.Simulation(w.size, p, w.cells, w.particles)
     ref = OracleWorld(w.size, p, precision=64, threads=CPU_THREADS)
     ref.set_data(w.cells, w.particles)
