@@ -25,6 +25,16 @@ E_NOTFOUND = -7
 
 JOB_QUEUED, JOB_RUNNING, JOB_DONE, JOB_FAILED = 0, 1, 2, 3
 
+# processes a settings file may switch on that the physics path does not simulate
+SETTINGS_ZONES = 0x1
+SETTINGS_RADIATION_SOURCES = 0x2
+SETTINGS_RADIATION = 0x4
+SETTINGS_CELL_DEATH = 0x8
+SETTINGS_PARTICLE_SPLIT = 0x10
+SETTINGS_RIGIDITY = 0x20
+SETTINGS_PARTICLE_TRANSFORMATION = 0x40
+SETTINGS_ABSORPTION_CONTROL = 0x80
+
 CONNECTION_DTYPE = np.dtype(
     [("cellIndex", "<i4"), ("distance", "<f4"), ("angleFromPrevious", "<f4")], align=True)
 
@@ -87,6 +97,8 @@ class Params(ctypes.Structure):
         ("cellMaxForceDecayProb", ctypes.c_float),
         ("cellMaxBindingDistance", ctypes.c_float * MAX_COLORS),
         ("radiationAbsorption", ctypes.c_float * MAX_COLORS),
+        ("cellMaxForce", ctypes.c_float * MAX_COLORS),
+        ("cellMinEnergy", ctypes.c_float * MAX_COLORS),
     ]
 
     def copy(self):
@@ -95,10 +107,22 @@ class Params(ctypes.Structure):
         return other
 
 
+def alien_default_params(**overrides):
+    """ALIEN's defaults for the physics subset (Motion.h, SimulationParameters.h,
+    SimulationParametersZoneValues.h), as alien_params_default() fills them: inner friction 0.3,
+    force decay probability 0.2 above cellMaxForce 0.8, cellMinEnergy 50."""
+    p = default_params(innerFriction=0.3, cellMaxForceDecayProb=0.2)
+    for k, v in overrides.items():
+        if not hasattr(p, k):
+            raise AttributeError(k)
+        setattr(p, k, v)
+    return p
+
+
 def default_params(**overrides):
-    """ALIEN defaults for the physics subset (Motion.h, SimulationParameters.h,
-    SimulationParametersZoneValues.h) with the parity values for the unimplemented stochastic
-    fields (innerFriction 0, cellMaxForceDecayProb 0)."""
+    """The parameter set of the benchmark workloads and most tests: ALIEN's defaults for the
+    physics subset except inner friction 0 and force decay probability 0 (the workloads are
+    specified without them; the tests of those two processes set them explicitly)."""
     p = Params()
     p.timestepSize = 1.0
     p.motionType = MOTION_FLUID
@@ -117,6 +141,8 @@ def default_params(**overrides):
     for c in range(MAX_COLORS):
         p.cellMaxBindingDistance[c] = 3.6
         p.radiationAbsorption[c] = 1.0
+        p.cellMaxForce[c] = 0.8
+        p.cellMinEnergy[c] = 50.0
     for k, v in overrides.items():
         if not hasattr(p, k):
             raise AttributeError(k)
@@ -130,6 +156,7 @@ class WorldDesc(ctypes.Structure):
         ("worldSizeY", ctypes.c_int32),
         ("cellCapacity", ctypes.c_uint64),
         ("particleCapacity", ctypes.c_uint64),
+        ("randomSeed", ctypes.c_uint64),
     ]
 
 

@@ -20,30 +20,20 @@
 #include <vector>
 
 #include "../../include/alien_hip.h"
+#include "capi_guard.hpp"
 #include "worldset.hpp"
 
 using alien::Error;
 using alien::WorldImage;
 using alien::WorldSet;
 
-namespace {
-thread_local std::string g_lastError;
-
-template <class F> int guarded(F&& f) {
-    try {
-        f();
-        return ALIEN_OK;
-    } catch (const Error& e) {
-        g_lastError = e.what();
-        return e.code;
-    } catch (const std::bad_alloc&) {
-        g_lastError = "host allocation failed";
-        return ALIEN_E_OOM;
-    } catch (const std::exception& e) {
-        g_lastError = e.what();
-        return ALIEN_E_STATE;
-    }
+std::string& alien::last_error() {
+    thread_local std::string message;
+    return message;
 }
+
+namespace {
+using alien::guarded;
 
 void require(bool cond, const char* msg) {
     if (!cond) throw Error(ALIEN_E_ARG, msg);
@@ -87,7 +77,7 @@ void requireHealthy(const alien_world* w) {
 
 extern "C" {
 
-const char* alien_last_error(void) { return g_lastError.c_str(); }
+const char* alien_last_error(void) { return alien::last_error().c_str(); }
 int alien_abi_version(void) { return ALIEN_ABI_VERSION; }
 
 int alien_get_device_name(char* buf, size_t cap) {
@@ -116,15 +106,17 @@ void alien_params_default(alien_params* p) {
     p->cellMaxCollisionDistance = 1.3f;
     p->cellRepulsionStrength = 0.08f;
     p->friction = 0.001f;
-    p->innerFriction = 0.0f;
+    p->innerFriction = 0.3f;
     p->cellMaxVelocity = 2.0f;
     p->cellMinDistance = 0.3f;
     p->cellFusionVelocity = 0.6f;
     p->cellMaxBindingEnergy = std::numeric_limits<float>::infinity();
-    p->cellMaxForceDecayProb = 0.0f;
+    p->cellMaxForceDecayProb = 0.2f;
     for (int c = 0; c < ALIEN_MAX_COLORS; ++c) {
         p->cellMaxBindingDistance[c] = 3.6f;
         p->radiationAbsorption[c] = 1.0f;
+        p->cellMaxForce[c] = 0.8f;
+        p->cellMinEnergy[c] = 50.0f;
     }
 }
 
@@ -153,13 +145,15 @@ int alien_set_params(alien_world* w, const alien_params* params) {
         require(w && params, "null argument");
         alien::validate_world(w->desc, *params);
         if (!w->poisoned) {
-            if (alien::bin_size_for(*params) != alien::bin_size_for(w->params)) {
-                // the bin grid changes: a fresh set is built around the new parameters (density
-                // reset pending), and replaces the old one only once it is complete
+            if (alien::bin_size_for(*params) != alien::bin_size_for(w->params) ||
+                alien::needs_fresh_set(w->params, *params)) {
+                // the bin grid or cellMinEnergy changes: a fresh set is built around the new
+                // parameters (density reset pending), and replaces the old one only once complete
                 WorldImage img;
                 w->set->extract(0, img);
                 img.host.params = *params;
                 img.densityResetPending = true;
+                img.updateLivingWork();
                 std::vector<WorldSet::Source> src(1);
                 src[0].image = &img;
                 auto fresh = std::make_unique<WorldSet>();
@@ -871,7 +865,7 @@ int alien_server_poll(alien_server* s, uint64_t jobId, alien_job_info* info) {
         info->startTime = j.startTime;
         info->finishTime = j.finishTime;
         info->deviceBytes = j.bytes;
-        if (j.state == ALIEN_JOB_FAILED) g_lastError = j.error;
+        if (j.state == ALIEN_JOB_FAILED) alien::last_error() = j.error;
     });
 }
 

@@ -162,6 +162,32 @@ __device__ __forceinline__ void rot_ccw(f2& v) {  // Math.cuh:164-169
     v.y = -t;
 }
 
+// Counter-based random draws (the reference's table with a shared atomic cursor,
+// CudaNumberGenerator.cuh:12-104, depends on thread timing).  Philox4x32-10 of Salmon, Moraes,
+// Dror and Shaw ("Parallel random numbers: as easy as 1, 2, 3", SC 2011) with counter
+// {id lo, id hi, timestep lo, purpose} and key {seed lo, seed hi}; the first output word's upper
+// 24 bits give a float in [0, 1).  A draw is a pure function of its arguments.
+constexpr uint32_t kStreamForceDecay = 1;  // checkForces, CellProcessor.cuh:367-386
+__host__ __device__ inline float uniform_draw(unsigned long long seed, unsigned long long timestep,
+                                              unsigned long long id, uint32_t purpose) {
+    uint32_t c0 = uint32_t(id), c1 = uint32_t(id >> 32), c2 = uint32_t(timestep), c3 = purpose;
+    uint32_t k0 = uint32_t(seed), k1 = uint32_t(seed >> 32);
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        const unsigned long long p0 = (unsigned long long)0xD2511F53u * c0;
+        const unsigned long long p1 = (unsigned long long)0xCD9E8D57u * c2;
+        const uint32_t n0 = uint32_t(p1 >> 32) ^ c1 ^ k0, n1 = uint32_t(p1);
+        const uint32_t n2 = uint32_t(p0 >> 32) ^ c3 ^ k1, n3 = uint32_t(p0);
+        c0 = n0;
+        c1 = n1;
+        c2 = n2;
+        c3 = n3;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    return float(c0 >> 8) * (1.0f / 16777216.0f);
+}
+
 // SPH cubic spline and derivative (CellProcessor.cuh:92-119)
 __device__ __forceinline__ float sph_w(float q) {
     float r;

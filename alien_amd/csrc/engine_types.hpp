@@ -70,6 +70,10 @@ struct DevParams {
     // |d| < minDistance <=> |d|^2 <= overlapBelowSq
     float maxVelSq, fastSq, fusionBelowSq, overlapBelowSq;
     float invH, invH2, invH3;  // 1/h, 1/h^2, 1/h^3 (fluid)
+    float innerFriction;       // applied every third step to bonded cells (0: off)
+    float decayProb;           // cellMaxForceDecayProb (0: off)
+    float maxForceSq[kColors];  // |F| > cellMaxForce  <=>  |F|^2 > maxForceSq (as reachSq)
+    float minEnergy[kColors];   // cellMinEnergy
 };
 
 // per-world geometry and placement in the set
@@ -85,6 +89,8 @@ struct DevWorld {
     int ntx, nty;           // strips, chunks
     int rowsPerStage;       // bin rows per staged row of the neighbour pass (1, or 2 for sparse rows)
     int bondWork;           // 1 if a bonded or barrier cell exists or can appear by fusion
+    int livingWork;         // 1 if living states are tracked on the device (see WorldSet::build)
+    unsigned long long seed;  // key of the world's random draws
 };
 
 // per-world device counters
@@ -136,6 +142,15 @@ struct Ctx {
     HotBuffers B;  // state at the end of the step (written)
     float* energy;  // cold [cellCap], indexed by cellOff + slot
     float* stiff;   // cold [cellCap]
+    unsigned long long* cid;  // cold [cellCap]: cell id (key of the cell's random draws)
+    // living states of the worlds with livingWork, cold by slot: `living` holds the state at the
+    // start of the step, `livingNext` the other buffer (the step's phases alternate between the
+    // two and leave the end-of-step state in `livingNext`, the next step's `living`)
+    uint8_t* living;
+    uint8_t* livingNext;
+    uint32_t* activation;     // cold [cellCap]: activationTime (worlds with livingWork)
+    uint32_t* creature;       // cold [cellCap]: creatureId (worlds with livingWork)
+    uint8_t* stretched;       // [cellCap] end-of-step staged cell had a bond beyond cellMaxBindingDistance
     int* binOf;     // [cellCap] global bin of A-cell i
     int* pbinOf;    // [partCap] global bin of A-particle i, -1 if dead
     // Bin sizes.  A cell bin's size is priv[b] + count[b]: priv holds the cells counted by the
@@ -183,6 +198,7 @@ struct Ctx {
     const int2* pbinBlocks;  // per block: (world, first global bin index) over particle bins
     const int2* partBlocks;  // per block: (world, first global particle index)
     const int2* tileBlocks;  // per block: (world, tile index), -1 for padding blocks
+    const int2* livingBlocks;  // per block: (world, first global cell index) over livingWork worlds
     int numPartBlocks;
 };
 

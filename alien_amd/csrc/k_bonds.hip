@@ -70,6 +70,7 @@ __global__ void __attribute__((amdgpu_waves_per_eu(6))) k_conn_own(Ctx c, int pa
         const int nc = flags_num_conn(fx);
         if (nc == 0 || flags_barrier(fx)) {
             c.valid[x] = 0u;
+            if (pass == 1) c.stretched[x] = 0;  // checkConnections skips barrier cells
             return;
         }
         const float Wf = float(w.W), Hf = float(w.H);
@@ -163,6 +164,7 @@ __global__ void __attribute__((amdgpu_waves_per_eu(6))) k_conn_own(Ctx c, int pa
         c.own[x] = to_float2(force);
         c.valid[x] = v;
         if (tooLong) c.delFlag[x] = 1;
+        if (pass == 1) c.stretched[x] = tooLong ? 1 : 0;
     }
 }
 
@@ -228,6 +230,8 @@ __global__ void k_conn_verlet_vel(Ctx c) {
     const DevWorld& w = c.worlds[wi];
     const DevParams& P = c.params[wi];
     const float dt = P.dt, friction = P.friction, maxEnergy = P.maxBindingEnergy;
+    // on an inner-friction step the velocity is left before friction for k_inner_friction
+    const bool inner = P.innerFriction > 0.0f && (c.ctr[wi].timestep % 3ull) == 0ull;
     // every load before the first store, so that they are in flight together
     uint32_t fx = 0u;
     f2 p = {0.0f, 0.0f}, sh = {0.0f, 0.0f}, v0 = {0.0f, 0.0f}, s2 = {0.0f, 0.0f}, s1 = {0.0f, 0.0f};
@@ -247,9 +251,157 @@ __global__ void k_conn_verlet_vel(Ctx c) {
     if (!st || flags_barrier(fx)) return;
     const f2 acc = mul(add(s1, s2), 0.5f);
     f2 v = add(v0, mul(acc, dt));
-    v = mul(v, 1.0f - friction);
+    if (inner) {
+        c.s2[x] = to_float2(v);  // the start-of-pass velocity of the inner friction
+    } else {
+        v = mul(v, 1.0f - friction);
+    }
     c.B.vel[x] = to_float2(v);
     if (energy > maxEnergy) c.delFlag[x] = 1;
+}
+
+// applyInnerFriction (CellProcessor.cuh:678-705) every third step, then applyFriction (:707-723).
+// Jacobi (rule R11): each bond is relaxed twice for the cell, once for its own visit and once for
+// the partner's visit of the bond, toward the partner's start-of-pass velocity, in bond order;
+// barrier partners are skipped.
+__global__ void k_inner_friction(Ctx c) {
+    const int cap = c.cellCap;
+    int x, wi;
+    if (!staged_cell(c, x, wi)) return;
+    const DevParams& P = c.params[wi];
+    if (!(P.innerFriction > 0.0f) || (c.ctr[wi].timestep % 3ull) != 0ull) return;
+    const uint32_t fx = c.B.flags[x];
+    if (flags_barrier(fx)) return;
+    const DevWorld& w = c.worlds[wi];
+    const int nc = flags_num_conn(fx);
+    const float f = P.innerFriction;
+    f2 vp[kMaxBonds];
+    bool use[kMaxBonds];
+#pragma unroll
+    for (int b = 0; b < kMaxBonds; ++b) {
+        use[b] = false;
+        vp[b] = f2{0.0f, 0.0f};
+        if (b < nc) {
+            const int p = w.cellOff + c.B.btgt[b * cap + x];
+            use[b] = !flags_barrier(c.B.flags[p]);
+            if (use[b]) vp[b] = to_f2(c.s2[p]);
+        }
+    }
+    f2 v = to_f2(c.s2[x]);
+    for (int visit = 0; visit < 2; ++visit)
+#pragma unroll
+        for (int b = 0; b < kMaxBonds; ++b)
+            if (b < nc && use[b]) {
+                const f2 avg = dvd(add(v, vp[b]), 2.0f);
+                v = add(mul(v, 1.0f - f), mul(avg, f));
+            }
+    v = mul(v, 1.0f - P.friction);
+    c.B.vel[x] = to_float2(v);
+}
+
+// ---------------------------------------------------------------------------------------------
+// living states of the worlds that track them (rule R17, cellDeathConsequences None), one thread
+// per end-of-step cell.  The three phases alternate between the two state buffers (living at the
+// start of the step -> livingNext -> living -> livingNext, the next step's start).
+// ---------------------------------------------------------------------------------------------
+constexpr uint8_t kReady = 0, kUnderConstruction = 1, kActivating = 2, kDetaching = 3, kReviving = 4, kDying = 5;
+
+__device__ __forceinline__ bool living_cell(const Ctx& c, int& x, int& wi) {
+    const int2 blk = c.livingBlocks[blockIdx.x];
+    wi = blk.x;
+    x = blk.y + int(threadIdx.x);
+    const DevWorld& w = c.worlds[wi];
+    return x < w.cellOff + w.numCells;
+}
+
+// checkConnections' Detaching marks on the partners of cells with an overstretched bond
+// (CellProcessor.cuh:515-520), then aging (:561-594)
+__global__ void k_living_mark_age(Ctx c) {
+    const int cap = c.cellCap;
+    int x, wi;
+    if (!living_cell(c, x, wi)) return;
+    const DevWorld& w = c.worlds[wi];
+    const uint32_t fx = c.B.flags[x];
+    const int sl = w.cellOff + c.B.slot[x];
+    uint8_t st = c.living[sl];
+    if (w.bondWork) {  // (stretched marks exist for the staged cells of worlds with bond work)
+        const int nc = flags_num_conn(fx);
+        for (int b = 0; b < nc; ++b)
+            if (c.stretched[w.cellOff + c.B.btgt[b * cap + x]]) st = kDetaching;
+    }
+    if (!flags_barrier(fx)) {
+        const uint32_t act = c.activation[sl];
+        if (st == kReady && act > 0u) c.activation[sl] = act - 1u;
+    }
+    c.livingNext[sl] = st;
+}
+
+// the state transitions (CellProcessor.cuh:596-676) from the states after the marks
+__global__ void k_living_transition(Ctx c) {
+    const int cap = c.cellCap;
+    int x, wi;
+    if (!living_cell(c, x, wi)) return;
+    const DevWorld& w = c.worlds[wi];
+    const uint32_t fx = c.B.flags[x];
+    const int sl = w.cellOff + c.B.slot[x];
+    const uint8_t st = c.livingNext[sl];
+    uint8_t next = st;
+    if (flags_barrier(fx)) {
+        next = kReady;
+    } else if (st == kActivating || st == kReviving || st == kDetaching) {
+        next = kReady;  // Detaching -> Ready under cellDeathConsequences None
+    } else if (st == kUnderConstruction) {
+        const uint32_t cr = c.creature[sl];
+        const int nc = flags_num_conn(fx);
+        for (int b = 0; b < nc; ++b) {
+            const int ps = w.cellOff + c.B.slot[w.cellOff + c.B.btgt[b * cap + x]];
+            if (c.creature[ps] == cr && c.livingNext[ps] == kActivating) next = kActivating;
+        }
+    }
+    c.living[sl] = next;
+}
+
+// decay (CellProcessor.cuh:775-858): energy below cellMinEnergy -> Dying; the partners of a cell
+// that becomes Dying -> Detaching unless they are or become Dying (cellDeathProbability 0)
+__device__ __forceinline__ bool below_min_energy(const Ctx& c, const DevWorld& w, const DevParams& P, uint32_t f, int sl) {
+    return !flags_barrier(f) && c.energy[sl] < P.minEnergy[flags_color(f)];
+}
+
+__global__ void k_living_decay(Ctx c) {
+    const int cap = c.cellCap;
+    int x, wi;
+    if (!living_cell(c, x, wi)) return;
+    const DevWorld& w = c.worlds[wi];
+    const DevParams& P = c.params[wi];
+    const uint32_t fx = c.B.flags[x];
+    const int sl = w.cellOff + c.B.slot[x];
+    const uint8_t st = c.living[sl];
+    uint8_t next = st;
+    if (below_min_energy(c, w, P, fx, sl)) {
+        next = kDying;
+    } else if (st != kDying) {
+        const int nc = flags_num_conn(fx);
+        for (int b = 0; b < nc; ++b) {
+            const int p = w.cellOff + c.B.btgt[b * cap + x];
+            const int ps = w.cellOff + c.B.slot[p];
+            if (below_min_energy(c, w, P, c.B.flags[p], ps) && c.living[ps] != kDying) next = kDetaching;
+        }
+    }
+    c.livingNext[sl] = next;
+}
+
+void launch_living_transitions(const Ctx& c, const LaunchShape& sh, hipStream_t s, Recorder* rec) {
+    if (!sh.livingBlocks) return;
+    if (rec) rec->mark("living_mark_age");
+    hipLaunchKernelGGL(k_living_mark_age, dim3(sh.livingBlocks), dim3(sh.cellThreads), 0, s, c);
+    if (rec) rec->mark("living_transition");
+    hipLaunchKernelGGL(k_living_transition, dim3(sh.livingBlocks), dim3(sh.cellThreads), 0, s, c);
+}
+
+void launch_living_decay(const Ctx& c, const LaunchShape& sh, hipStream_t s, Recorder* rec) {
+    if (!sh.livingBlocks) return;
+    if (rec) rec->mark("living_decay");
+    hipLaunchKernelGGL(k_living_decay, dim3(sh.livingBlocks), dim3(sh.cellThreads), 0, s, c);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -502,13 +654,21 @@ void launch_bonds(const Ctx& c, const LaunchShape& sh, hipStream_t s, Recorder* 
     } while (0)
     // no world can have a bonded or barrier cell this step (none has one and none can fuse): no
     // cell is staged, and the scan at the start of the step advanced the timesteps
-    if (!sh.hasBondWork || sh.bondBlocks == 0) return;
+    if (!sh.hasBondWork || sh.bondBlocks == 0) {
+        launch_living_transitions(c, sh, s, rec);
+        launch_living_decay(c, sh, s, rec);
+        return;
+    }
     const dim3 b(sh.cellThreads), cb(256);
     L("remap", k_remap, sg, b, c);
     L("conn_own", k_conn_own, sg, b, c, 0);
     L("conn_verlet_pos", k_conn_verlet_pos, sg, b, c);
     L("conn_own", k_conn_own, sg, b, c, 1);
     L("conn_verlet_vel", k_conn_verlet_vel, sg, b, c);
+    // SimulationKernelsLauncher.cu:52-71: aging and living states, inner friction, friction and decay
+    launch_living_transitions(c, sh, s, rec);
+    if (sh.hasInnerFriction) L("inner_friction", k_inner_friction, sg, b, c);
+    launch_living_decay(c, sh, s, rec);
     L("mark", k_mark, sg, b, c);
     L("fuse_claim", k_fuse_claim, cg, cb, c);
     L("fuse_apply", k_fuse_apply, cg, cb, c);

@@ -357,6 +357,13 @@ __device__ __forceinline__ void cell_finish(const Ctx& c, const DevWorld& w, con
         if (nci > 0) c.bstiff[k] = stiff;
     }
     if (nci > 0) c.newidx[i] = k;
+    // checkForces (CellProcessor.cuh:367-386): a bonded cell whose force is above cellMaxForce loses
+    // all its bonds with probability cellMaxForceDecayProb (deleting the bonds of an unbonded cell
+    // would change nothing)
+    if (P.decayProb > 0.0f && nci > 0 && !bari && lensq(F) > P.maxForceSq[flags_color(fi)]) {
+        const unsigned long long id = c.cid[w.cellOff + slot];
+        if (uniform_draw(w.seed, c.ctr[wi].timestep, id, kStreamForceDecay) < P.decayProb) c.delFlag[k] = 1;
+    }
     f2 v = vi;
     if (!bari) {  // applyForces
         v = add(v, F);

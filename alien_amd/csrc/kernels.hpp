@@ -22,6 +22,8 @@ struct LaunchShape {
     // some world has live energy particles (nothing on this path creates particles, so once all
     // are absorbed or merged the particle kernels, which would find no work, are not launched)
     bool hasParticles = true;
+    int livingBlocks = 0;           // cell blocks of the worlds whose living states live on the device
+    bool hasInnerFriction = false;  // some active world with bond work has innerFriction > 0
 };
 
 // Optional per-launch hook: mark(name) is called right before each kernel launch and

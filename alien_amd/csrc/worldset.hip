@@ -27,10 +27,13 @@ int bin_size_for(const alien_params& p) {
 void validate_params(const alien_params& p) {
     if (p.motionType != ALIEN_MOTION_FLUID && p.motionType != ALIEN_MOTION_COLLISION)
         throw Error(ALIEN_E_ARG, "motionType must be ALIEN_MOTION_FLUID or ALIEN_MOTION_COLLISION");
-    if (p.innerFriction != 0.0f)
-        throw Error(ALIEN_E_ARG, "innerFriction must be 0 (inner friction is not part of this engine)");
-    if (p.cellMaxForceDecayProb != 0.0f)
-        throw Error(ALIEN_E_ARG, "cellMaxForceDecayProb must be 0 (random bond decay is not part of this engine)");
+    if (!(p.innerFriction >= 0.0f && p.innerFriction <= 1.0f)) throw Error(ALIEN_E_ARG, "innerFriction must lie in [0, 1]");
+    if (!(p.cellMaxForceDecayProb >= 0.0f && p.cellMaxForceDecayProb <= 1.0f))
+        throw Error(ALIEN_E_ARG, "cellMaxForceDecayProb must lie in [0, 1]");
+    for (int c = 0; c < kColors; ++c) {
+        if (!(p.cellMaxForce[c] >= 0.0f)) throw Error(ALIEN_E_ARG, "cellMaxForce must not be negative");
+        if (std::isnan(p.cellMinEnergy[c])) throw Error(ALIEN_E_ARG, "cellMinEnergy must be a number");
+    }
     if (!(p.timestepSize > 0.0f)) throw Error(ALIEN_E_ARG, "timestepSize must be positive");
     if (p.motionType == ALIEN_MOTION_FLUID && !(p.smoothingLength > 0.0f))
         throw Error(ALIEN_E_ARG, "smoothingLength must be positive");
@@ -102,6 +105,13 @@ DevParams to_dev_params(const alien_params& p) {
     d.fastSq = sqrt_threshold(0.5f);
     d.fusionBelowSq = sqrt_below(d.fusionVelocity);
     d.overlapBelowSq = sqrt_below(d.minDistance);
+    d.innerFriction = p.innerFriction;
+    d.decayProb = p.cellMaxForceDecayProb;
+    for (int c = 0; c < kColors; ++c) {
+        d.maxForceSq[c] = std::isinf(p.cellMaxForce[c]) ? std::numeric_limits<float>::infinity()
+                                                        : sqrt_threshold(p.cellMaxForce[c]);
+        d.minEnergy[c] = p.cellMinEnergy[c];
+    }
     d.invH = 1.0f / d.smoothing;
     d.invH2 = 1.0f / (d.smoothing * d.smoothing);
     d.invH3 = 1.0f / (d.smoothing * d.smoothing * d.smoothing);
@@ -223,7 +233,26 @@ WorldImage WorldImage::fromTO(const alien_world_desc& desc, const alien_params& 
         h.particleSelected[i] = p.selected;
     }
     img.numParticles = np;
+    // Living states need tracking on the device only if some cell is not trivially Ready: a state
+    // other than Ready, an activation count, or (non-barrier) energy below cellMinEnergy.  Energy
+    // only grows on this path, so a world without any of these keeps every cell Ready with no
+    // count for good, and its ages follow from the number of steps.
+    for (int i = 0; i < n && !img.livingWork; ++i) {
+        const alien_cell_to& c = cells[i];
+        img.livingWork = c.livingState != 0 || c.activationTime != 0 ||
+                         (!c.barrier && c.energy < params.cellMinEnergy[c.color]);
+    }
     return img;
+}
+
+void WorldImage::updateLivingWork() {
+    const HostWorld& h = host;
+    for (int i = 0; i < h.numCells && !livingWork; ++i) {
+        const int sl = slot[size_t(i)];
+        const uint32_t f = flags[size_t(i)];
+        livingWork = h.livingState[sl] != 0 || h.activationTime[sl] != 0 ||
+                     (!flags_barrier(f) && energy[sl] < h.params.cellMinEnergy[flags_color(f)]);
+    }
 }
 
 void WorldImage::toTO(std::vector<alien_cell_to>& cells, std::vector<alien_particle_to>& particles) const {
@@ -238,7 +267,6 @@ void WorldImage::toTO(std::vector<alien_cell_to>& cells, std::vector<alien_parti
     std::vector<int> posOf(n);
     for (int k = 0; k < n; ++k) posOf[order[k]] = k;
     cells.assign(n, alien_cell_to{});
-    const uint64_t s = h.stepsSinceSet;
     for (int k = 0; k < n; ++k) {
         const int i = order[k];
         const int sl = slot[i];
@@ -260,29 +288,9 @@ void WorldImage::toTO(std::vector<alien_cell_to>& cells, std::vector<alien_parti
             c.connections[b].distance = brest[size_t(b) * n + i];
             c.connections[b].angleFromPrevious = bang[size_t(b) * n + i];
         }
-        // aging and living-state transitions (CellProcessor.cuh:561-676) evaluated in closed form
-        // for the steps since the data was set: every step ages non-barrier cells by one, counts a
-        // Ready cell's activationTime down, and turns Activating/Detaching/Reviving into Ready
-        uint8_t ls = h.livingState[sl];
-        uint32_t at = h.activationTime[sl];
-        uint32_t age = h.age[sl];
-        if (s > 0) {
-            if (c.barrier) {
-                ls = 0;
-            } else {
-                age = uint32_t(uint64_t(age) + s);
-                if (ls == 0) {
-                    at = uint64_t(at) > s ? uint32_t(at - s) : 0u;
-                } else if (ls == 2 || ls == 3 || ls == 4) {
-                    uint64_t d = s - 1;
-                    at = uint64_t(at) > d ? uint32_t(at - d) : 0u;
-                    ls = 0;
-                }
-            }
-        }
-        c.age = age;
-        c.livingState = ls;
-        c.activationTime = at;
+        c.age = h.age[sl];
+        c.livingState = h.livingState[sl];
+        c.activationTime = h.activationTime[sl];
         c.cellFunction = h.cellFunction[sl];
         c.ancestorMutationId = h.ancestorMutationId[sl];
         c.selected = h.selected[sl];
@@ -417,7 +425,7 @@ void WorldSet::build(const std::vector<Source>& sources) {
     std::vector<uint64_t> timesteps;
     std::vector<int> liveParticles;
     std::vector<uint8_t> pending;
-    std::vector<uint8_t> bondWork;
+    std::vector<uint8_t> bondWork, livingWork;
     std::vector<int> stripChoice, stageChoice;
     // A set is built once, into fresh allocations; a failed build then leaves nothing behind (the
     // destructor frees what was allocated).  Sizes are checked before anything is read or allocated.
@@ -444,12 +452,13 @@ void WorldSet::build(const std::vector<Source>& sources) {
         uint64_t ts = 0;
         bool pend = false;
         int strip = kStripMax, stage = 1;
-        bool work = false;
+        bool work = false, living = false;
         if (src.image) {
             h = src.image->host;
             live = src.image->numParticles;
             ts = src.image->timestep;
             pend = src.image->densityResetPending;
+            living = src.image->livingWork;
             const WorldImage& im = *src.image;
             for (int i = 0; i < h.numCells && !work; ++i) {
                 // bonds can appear only by fusion, which needs a free bond slot
@@ -467,6 +476,7 @@ void WorldSet::build(const std::vector<Source>& sources) {
             strip = src.from->stripChoice_.at(src.fromIndex);
             stage = src.from->stageChoice_.at(src.fromIndex);
             work = src.from->bondWork_.at(src.fromIndex) != 0;
+            living = src.from->livingWork_.at(src.fromIndex) != 0;
         }
         validate_world(h.desc, h.params);
         h.particleCap = live;
@@ -484,6 +494,8 @@ void WorldSet::build(const std::vector<Source>& sources) {
         w.partOff = int(partOff);
         w.partCap = live;
         w.bondWork = work ? 1 : 0;
+        w.livingWork = living ? 1 : 0;
+        w.seed = h.desc.randomSeed;
         strip_shape(w, std::min(strip, stripCap_), stage);
         stripChoice.push_back(strip);
         stageChoice.push_back(stage);
@@ -499,6 +511,7 @@ void WorldSet::build(const std::vector<Source>& sources) {
         liveParticles.push_back(live);
         pending.push_back(pend ? 1 : 0);
         bondWork.push_back(work ? 1 : 0);
+        livingWork.push_back(living ? 1 : 0);
     }
     const int cellBins = int(bins);  // cell bins of all worlds come first, then the particle bins
     for (auto& w : worlds) {
@@ -540,6 +553,12 @@ void WorldSet::build(const std::vector<Source>& sources) {
     }
     energy_ = static_cast<float*>(A(sizeof(float) * NC));
     stiff_ = static_cast<float*>(A(sizeof(float) * NC));
+    cid_ = static_cast<unsigned long long*>(A(sizeof(unsigned long long) * NC));
+    living_[0] = static_cast<uint8_t*>(A(NC));
+    living_[1] = static_cast<uint8_t*>(A(NC));
+    activation_ = static_cast<uint32_t*>(A(sizeof(uint32_t) * NC));
+    creature_ = static_cast<uint32_t*>(A(sizeof(uint32_t) * NC));
+    stretched_ = static_cast<uint8_t*>(A(NC));
     binOf_ = static_cast<int*>(A(sizeof(int) * NC));
     pbinOf_ = static_cast<int*>(A(sizeof(int) * NP));
     count_[0] = static_cast<uint32_t*>(A(sizeof(uint32_t) * NB));
@@ -601,6 +620,11 @@ void WorldSet::build(const std::vector<Source>& sources) {
             }
             h2d(energy_ + w.cellOff, im.energy.data(), sizeof(float) * n);
             h2d(stiff_ + w.cellOff, im.stiff.data(), sizeof(float) * n);
+            static_assert(sizeof(unsigned long long) == sizeof(uint64_t), "cell ids");
+            h2d(cid_ + w.cellOff, im.host.cellId.data(), sizeof(uint64_t) * n);
+            h2d(living_[0] + w.cellOff, im.host.livingState.data(), n);
+            h2d(activation_ + w.cellOff, im.host.activationTime.data(), sizeof(uint32_t) * n);
+            h2d(creature_ + w.cellOff, im.host.creatureId.data(), sizeof(uint32_t) * n);
             h2d(D.ppos + w.partOff, im.ppos.data(), sizeof(float2) * np);
             h2d(D.pvel + w.partOff, im.pvel.data(), sizeof(float2) * np);
             h2d(D.pen + w.partOff, im.pen.data(), sizeof(float) * np);
@@ -627,6 +651,10 @@ void WorldSet::build(const std::vector<Source>& sources) {
             }
             d2d(energy_ + w.cellOff, o.energy_ + ow.cellOff, sizeof(float) * n);
             d2d(stiff_ + w.cellOff, o.stiff_ + ow.cellOff, sizeof(float) * n);
+            d2d(cid_ + w.cellOff, o.cid_ + ow.cellOff, sizeof(unsigned long long) * n);
+            d2d(living_[0] + w.cellOff, o.living_[o.parity_] + ow.cellOff, n);
+            d2d(activation_ + w.cellOff, o.activation_ + ow.cellOff, sizeof(uint32_t) * n);
+            d2d(creature_ + w.cellOff, o.creature_ + ow.cellOff, sizeof(uint32_t) * n);
             d2d(D.ppos + w.partOff, S.ppos + ow.partOff, sizeof(float2) * np);
             d2d(D.pvel + w.partOff, S.pvel + ow.partOff, sizeof(float2) * np);
             d2d(D.pen + w.partOff, S.pen + ow.partOff, sizeof(float) * np);
@@ -642,6 +670,7 @@ void WorldSet::build(const std::vector<Source>& sources) {
     params_ = std::move(params);
     densityPending_ = std::move(pending);
     bondWork_ = std::move(bondWork);
+    livingWork_ = std::move(livingWork);
     active_.assign(worlds_.size(), 1);
     stripChoice_ = std::move(stripChoice);
     stageChoice_ = std::move(stageChoice);
@@ -785,7 +814,7 @@ void chunk_shape(std::vector<DevWorld>& worlds, const std::vector<DevParams>& pa
 void WorldSet::buildTables() {
     for (auto& a : tableAllocs_) (void)hipFree(a.p);
     tableAllocs_.clear();
-    std::vector<int2> cells, bonds, bins, pbins, parts;
+    std::vector<int2> cells, bonds, bins, pbins, parts, living;
     const int ct = shape_.cellThreads, pt = shape_.partThreads, bt = shape_.binThreads;
     for (int wi = 0; wi < int(worlds_.size()); ++wi) {
         if (!active_[wi]) continue;
@@ -793,6 +822,8 @@ void WorldSet::buildTables() {
         for (int s = 0; s < w.numCells; s += ct) cells.push_back(make_int2(wi, w.cellOff + s));
         if (bondWork_[wi])
             for (int s = 0; s < w.numCells; s += ct) bonds.push_back(make_int2(wi, w.cellOff + s));
+        if (livingWork_[wi])
+            for (int s = 0; s < w.numCells; s += ct) living.push_back(make_int2(wi, w.cellOff + s));
         for (int s = 0; s < w.partCap; s += pt) parts.push_back(make_int2(wi, w.partOff + s));
         for (int s = 0; s < w.nbx * w.nby; s += bt) bins.push_back(make_int2(wi, w.binOff + s));
         for (int s = 0; s < w.pnbx * w.pnby; s += bt) pbins.push_back(make_int2(wi, w.pbinOff + s));
@@ -829,6 +860,8 @@ void WorldSet::buildTables() {
     pbinBlocks_ = up(pbins);
     partBlocks_ = up(parts);
     tileBlocks_ = up(tileTable);
+    livingBlocks_ = up(living);
+    shape_.livingBlocks = int(living.size());
     shape_.cellBlocks = int(cells.size());
     shape_.bondBlocks = int(bonds.size());
     // the bond kernels run when an active world can have a bonded or barrier cell
@@ -840,7 +873,17 @@ void WorldSet::buildTables() {
     numPartBlocks_ = int(parts.size());
     shape_.hasFluid = shape_.hasCollision = false;
     for (const DevParams& p : params_) (p.motion == 0 ? shape_.hasFluid : shape_.hasCollision) = true;
+    updateInnerFriction();
     invalidateGraphs();
+}
+
+// the inner-friction kernel runs when an active world with bond work has innerFriction > 0
+void WorldSet::updateInnerFriction() {
+    bool any = false;
+    for (size_t i = 0; i < worlds_.size(); ++i)
+        any = any || (active_[i] && bondWork_[i] && params_[i].innerFriction > 0.0f);
+    if (any != shape_.hasInnerFriction) invalidateGraphs();
+    shape_.hasInnerFriction = any;
 }
 
 void WorldSet::setBlockSize(int threads) {
@@ -927,6 +970,13 @@ Ctx WorldSet::makeCtx(int parity) const {
     c.B = buf_[parity ^ 1];
     c.energy = energy_;
     c.stiff = stiff_;
+    c.cid = cid_;
+    c.living = living_[parity];
+    c.livingNext = living_[parity ^ 1];
+    c.activation = activation_;
+    c.creature = creature_;
+    c.stretched = stretched_;
+    c.livingBlocks = livingBlocks_;
     c.binOf = binOf_;
     c.pbinOf = pbinOf_;
     c.count = count_[parity];
@@ -1003,6 +1053,8 @@ void WorldSet::setParams(int w, const alien_params& p) {
     validate_world(hosts_.at(w).desc, p);
     if (bin_size_for(p) != bin_size_for(hosts_.at(w).params))
         throw Error(ALIEN_E_STATE, "the bin grid changes with these parameters: build a fresh set around them");
+    if (!livingWork_.at(w) && needs_fresh_set(hosts_.at(w).params, p))
+        throw Error(ALIEN_E_STATE, "cellMinEnergy changes: build a fresh set around the new parameters");
     hosts_.at(w).params = p;
     params_.at(w) = to_dev_params(p);
     check_hip(hipMemcpy(dParams_ + w, &params_[w], sizeof(DevParams), hipMemcpyHostToDevice), "H2D");
@@ -1012,6 +1064,16 @@ void WorldSet::setParams(int w, const alien_params& p) {
     shape_.hasFluid = shape_.hasCollision = false;
     for (const DevParams& q : params_) (q.motion == 0 ? shape_.hasFluid : shape_.hasCollision) = true;
     if (fluid != shape_.hasFluid || collision != shape_.hasCollision) invalidateGraphs();
+    updateInnerFriction();
+}
+
+// A world whose living states are not tracked on the device stays trivially Ready only while no
+// cell can fall below cellMinEnergy: a change of it needs a fresh set (built from the world's
+// current state, which decides anew whether to track).
+bool needs_fresh_set(const alien_params& before, const alien_params& after) {
+    for (int c = 0; c < kColors; ++c)
+        if (before.cellMinEnergy[c] != after.cellMinEnergy[c]) return true;
+    return false;
 }
 
 // Reads the last step's candidate-list size: reports overflow and grows the grid of the fusion
@@ -1064,7 +1126,8 @@ void WorldSet::step(uint64_t n) {
         }
         parity_ ^= 1;
     }
-    for (auto& h : hosts_) h.stepsSinceSet += n;
+    for (size_t i = 0; i < hosts_.size(); ++i)
+        if (active_[i]) hosts_[i].stepsSinceSet += n;  // (an inactive world is frozen)
     synchronize();
     if (timing_) recorder_->collect(kernelTimes_);
     checkErrors();
@@ -1117,6 +1180,25 @@ void WorldSet::extract(int wi, WorldImage& img) {
     }
     d2h(img.energy.data(), energy_ + w.cellOff, sizeof(float) * n);
     d2h(img.stiff.data(), stiff_ + w.cellOff, sizeof(float) * n);
+    // the current age, living state and activation count (CellProcessor.cuh:561-676): aging adds
+    // one per step to every non-barrier cell; a world with livingWork has its states and counts on
+    // the device, any other world is Ready without counts throughout (its barrier cells too)
+    HostWorld& h = img.host;
+    img.livingWork = livingWork_.at(wi) != 0;
+    if (img.livingWork) {
+        d2h(h.livingState.data(), living_[parity_] + w.cellOff, size_t(n));
+        d2h(h.activationTime.data(), activation_ + w.cellOff, sizeof(uint32_t) * n);
+    }
+    if (h.stepsSinceSet > 0)
+        for (int i = 0; i < n; ++i) {
+            const int sl = img.slot[i];
+            if (flags_barrier(img.flags[i])) {
+                h.livingState[sl] = 0;
+                continue;
+            }
+            h.age[sl] = uint32_t(uint64_t(h.age[sl]) + h.stepsSinceSet);
+        }
+    h.stepsSinceSet = 0;
     img.ppos.resize(np);
     img.pvel.resize(np);
     img.pen.resize(np);

@@ -35,13 +35,18 @@ void validate_world(const alien_world_desc& d, const alien_params& p);
 // throws ALIEN_E_CAPACITY when a world set of this many cells / particles exceeds kMaxSetCells /
 // kMaxSetParticles (checked before any transfer array is read or any device memory is allocated)
 void check_set_size(uint64_t cells, uint64_t particles);
+// whether new parameters for a world without device-tracked living states need a fresh set
+bool needs_fresh_set(const alien_params& before, const alien_params& after);
 DevParams to_dev_params(const alien_params& p);
 int choose_strip(const std::vector<float2>& pos, int W, int H, int binSize, bool twoRowStages, int& rowsPerStage);
 void strip_shape(DevWorld& w, int strip, int rowsPerStage);
 void chunk_shape(std::vector<DevWorld>& worlds, const std::vector<DevParams>& params, int forcedRows = 0);
 constexpr int kXcds = 8;  // accelerator complex dies of an MI355X
 
-// host-side identity and pass-through data of one world (fields the physics path never touches)
+// host-side identity and pass-through data of one world (fields the physics path never touches).
+// age, activationTime and livingState hold the values at the last set (stepsSinceSet steps ago):
+// aging adds one per step to every non-barrier cell, and a world whose living states are all
+// trivially Ready (no livingWork) keeps them so; otherwise the device arrays are authoritative.
 struct HostWorld {
     alien_world_desc desc{};
     alien_params params{};
@@ -69,6 +74,7 @@ struct WorldImage {
     std::vector<int> btgt;  // [kMaxBonds][numCells]
     std::vector<float> brest, bang;
     std::vector<float> energy, stiff;  // by slot
+    bool livingWork = false;           // living states tracked on the device (see WorldSet::build)
     std::vector<float2> ppos, pvel;
     std::vector<float> pen;
     std::vector<int> plast, pslot;
@@ -79,6 +85,8 @@ struct WorldImage {
                              uint64_t numCells, const alien_particle_to* particles, uint64_t numParticles);
     // to transfer objects, cells and particles in ascending id order
     void toTO(std::vector<alien_cell_to>& cells, std::vector<alien_particle_to>& particles) const;
+    // livingWork for the image's current states, energies and parameters (once set, it stays set)
+    void updateLivingWork();
 };
 
 class WorldSet {
@@ -181,6 +189,13 @@ private:
     HotBuffers buf_[2]{};
     float* energy_ = nullptr;
     float* stiff_ = nullptr;
+    unsigned long long* cid_ = nullptr;
+    uint8_t* living_[2] = {nullptr, nullptr};  // living states by slot, alternating with the parity
+    uint32_t* activation_ = nullptr;
+    uint32_t* creature_ = nullptr;
+    uint8_t* stretched_ = nullptr;
+    int2* livingBlocks_ = nullptr;
+    std::vector<uint8_t> livingWork_;  // per world
     int *binOf_ = nullptr, *pbinOf_ = nullptr;
     uint32_t* count_[2] = {nullptr, nullptr};  // bin histograms, alternating with the state buffers
     uint32_t* priv_[2] = {nullptr, nullptr};   // owner-counted part of the cell-bin histograms
@@ -210,6 +225,7 @@ private:
     int stripCap_ = kStripMax;
     int chunkRows_ = 0;
     void adaptGrids();
+    void updateInnerFriction();
     std::vector<Arr> tableAllocs_;
 
     hipGraphExec_t graph_[2] = {nullptr, nullptr};

@@ -99,6 +99,10 @@ def lib():
         L.alien_server_wait.argtypes = [vp, u64, ctypes.c_double]
         L.alien_server_get_snapshot.argtypes = [vp, u64, vp, u64, pu64, vp, u64, pu64]
         L.alien_server_release.argtypes = [vp, u64]
+        L.alien_settings_read.argtypes = [ctypes.c_char_p, ctypes.c_size_t, ctypes.POINTER(WorldDesc),
+                                          ctypes.POINTER(Params), pu64, ctypes.POINTER(ctypes.c_uint32)]
+        L.alien_settings_write.argtypes = [ctypes.POINTER(WorldDesc), ctypes.POINTER(Params), u64, ctypes.c_char_p,
+                                           ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]
         L.alien_server_estimate_bytes.argtypes = [u64, u64, i32, i32]
         L.alien_server_estimate_bytes.restype = u64
         _lib = L
@@ -111,6 +115,27 @@ def check(rc):
         if rc == -2:
             raise AlienMemoryError(rc, msg)
         raise AlienError(rc, msg)
+
+
+def read_settings(text):
+    """ALIEN's .settings.json text -> (world size, alien_params, timestep, unsimulated bits)."""
+    raw = text.encode() if isinstance(text, str) else bytes(text)
+    d, p = WorldDesc(), Params()
+    t, u = ctypes.c_uint64(), ctypes.c_uint32()
+    check(lib().alien_settings_read(raw, len(raw), ctypes.byref(d), ctypes.byref(p), ctypes.byref(t), ctypes.byref(u)))
+    return (d.worldSizeX, d.worldSizeY), p, t.value, u.value
+
+
+def write_settings(size, params, timestep=0):
+    """The physics subset as ALIEN's .settings.json text."""
+    d = _desc(size)
+    n = ctypes.c_size_t()
+    rc = lib().alien_settings_write(ctypes.byref(d), ctypes.byref(params), int(timestep), None, 0, ctypes.byref(n))
+    if rc not in (0, -3):
+        check(rc)
+    buf = ctypes.create_string_buffer(n.value + 1)
+    check(lib().alien_settings_write(ctypes.byref(d), ctypes.byref(params), int(timestep), buf, len(buf), ctypes.byref(n)))
+    return buf.value.decode()
 
 
 def gpu_name():
@@ -129,9 +154,10 @@ def _parts(a):
     return a, (a.ctypes.data if len(a) else None)
 
 
-def _desc(size):
+def _desc(size, seed=0):
     d = WorldDesc()
     d.worldSizeX, d.worldSizeY = int(size[0]), int(size[1])
+    d.randomSeed = int(seed) & 0xFFFFFFFFFFFFFFFF
     return d
 
 
@@ -153,11 +179,11 @@ def _read(fn_counts, fn_data):
 class Simulation:
     """One world on the GPU (ALIEN's _SimulationFacade physics subset over the C ABI)."""
 
-    def __init__(self, size, params, cells=None, particles=None, timestep=0):
+    def __init__(self, size, params, cells=None, particles=None, timestep=0, seed=0):
         self._h = None
         h = ctypes.c_void_p()
         self._size = (int(size[0]), int(size[1]))
-        check(lib().alien_world_create(ctypes.byref(_desc(size)), ctypes.byref(params), ctypes.byref(h)))
+        check(lib().alien_world_create(ctypes.byref(_desc(size, seed)), ctypes.byref(params), ctypes.byref(h)))
         self._h = h
         if timestep:
             self.set_timestep(timestep)
@@ -166,7 +192,7 @@ class Simulation:
 
     @classmethod
     def from_world(cls, world):
-        return cls(world.size, world.params, world.cells, world.particles)
+        return cls(world.size, world.params, world.cells, world.particles, seed=world.seed)
 
     def close(self):
         if self._h:
@@ -290,16 +316,16 @@ class Batch:
         except Exception:
             pass
 
-    def add_world(self, size, params, cells, particles):
+    def add_world(self, size, params, cells, particles, seed=0):
         c, cp = _cells(cells)
         p, pp = _parts(particles)
         idx = ctypes.c_int32()
-        check(lib().alien_batch_add_world(self._h, ctypes.byref(_desc(size)), ctypes.byref(params), cp, len(c), pp,
+        check(lib().alien_batch_add_world(self._h, ctypes.byref(_desc(size, seed)), ctypes.byref(params), cp, len(c), pp,
                                           len(p), ctypes.byref(idx)))
         return idx.value
 
     def add(self, world):
-        return self.add_world(world.size, world.params, world.cells, world.particles)
+        return self.add_world(world.size, world.params, world.cells, world.particles, seed=world.seed)
 
     def __len__(self):
         return lib().alien_batch_num_worlds(self._h)
@@ -367,11 +393,11 @@ class Server:
         except Exception:
             pass
 
-    def submit(self, size, params, cells, particles, steps):
+    def submit(self, size, params, cells, particles, steps, seed=0):
         c, cp = _cells(cells)
         p, pp = _parts(particles)
         jid = ctypes.c_uint64()
-        check(lib().alien_server_submit(self._h, ctypes.byref(_desc(size)), ctypes.byref(params), cp, len(c), pp,
+        check(lib().alien_server_submit(self._h, ctypes.byref(_desc(size, seed)), ctypes.byref(params), cp, len(c), pp,
                                         len(p), int(steps), ctypes.byref(jid)))
         return jid.value
 

@@ -20,7 +20,7 @@
 extern "C" {
 #endif
 
-#define ALIEN_ABI_VERSION 1
+#define ALIEN_ABI_VERSION 2
 
 /* ---- error codes ---------------------------------------------------------------------------- */
 #define ALIEN_OK 0
@@ -84,12 +84,10 @@ typedef struct alien_particle_to {
 
 /*
  * Physics subset of SimulationParameters (source/EngineInterface/SimulationParameters.h:18-206,
- * Motion.h:7-43, SimulationParametersZoneValues.h:10-23).  alien_params_default() fills ALIEN's
- * defaults for the physics fields and the parity values for the stochastic/bookkeeping fields
- * that the path does not implement (those must keep the value shown, or the call fails with
- * ALIEN_E_ARG):
- *   innerFriction 0, cellMaxForceDecayProb 0, radiation emission off, particle splitting off,
- *   particle transformation off, numZones 0, rigidity 0.
+ * Motion.h:7-43, SimulationParametersZoneValues.h:10-24).  alien_params_default() fills ALIEN's
+ * defaults.  The processes of the reference that this path does not implement are fixed at these
+ * values (no field here): radiation emission off, particle splitting and transformation off,
+ * numZones 0, rigidity 0, cellDeathConsequences None, cellDeathProbability 0.
  */
 typedef struct alien_params {
     float timestepSize;                /* SimulationParameters.h:61, default 1.0           */
@@ -100,22 +98,27 @@ typedef struct alien_params {
     float cellMaxCollisionDistance;    /* Motion.h:22, default 1.3                         */
     float cellRepulsionStrength;       /* Motion.h:23, default 0.08                        */
     float friction;                    /* ZoneValues.h:12, default 0.001                   */
-    float innerFriction;               /* SimulationParameters.h:65; must be 0             */
+    float innerFriction;               /* SimulationParameters.h:65, default 0.3, in [0,1] */
     float cellMaxVelocity;             /* SimulationParameters.h:66, default 2.0           */
     float cellMinDistance;             /* SimulationParameters.h:70, default 0.3           */
     float cellFusionVelocity;          /* ZoneValues.h:23, default 0.6                     */
     float cellMaxBindingEnergy;        /* ZoneValues.h:24, default +inf                    */
-    float cellMaxForceDecayProb;       /* SimulationParameters.h:71; must be 0             */
+    float cellMaxForceDecayProb;       /* SimulationParameters.h:71, default 0.2, in [0,1] */
     float cellMaxBindingDistance[ALIEN_MAX_COLORS]; /* SimulationParameters.h:67, 3.6   */
     float radiationAbsorption[ALIEN_MAX_COLORS];    /* ZoneValues.h:14, 1.0             */
+    float cellMaxForce[ALIEN_MAX_COLORS];           /* ZoneValues.h:19, 0.8             */
+    float cellMinEnergy[ALIEN_MAX_COLORS];          /* ZoneValues.h:20, 50.0            */
 } alien_params;
 
-/* World geometry and capacity (GeneralSettings worldSizeX/worldSizeY). */
+/* World geometry and capacity (GeneralSettings worldSizeX/worldSizeY), and the key of the world's
+ * random draws.  The reference draws from a table refilled by the host; here a draw is a function
+ * of (randomSeed, timestep, cell id), so a world's run is reproducible from its seed. */
 typedef struct alien_world_desc {
     int32_t worldSizeX;
     int32_t worldSizeY;
     uint64_t cellCapacity;     /* 0 = size to the first alien_set_data call     */
     uint64_t particleCapacity; /* 0 = size to the first alien_set_data call     */
+    uint64_t randomSeed;
 } alien_world_desc;
 
 /* Engine-wide totals (IntegrationTestFramework.cpp:30-40 getEnergy, plus momentum). */
@@ -290,6 +293,34 @@ int alien_server_release(alien_server* s, uint64_t jobId);
 /* Estimated device bytes a job of this size needs (the admission figure). */
 uint64_t alien_server_estimate_bytes(uint64_t numCells, uint64_t numParticles, int32_t worldSizeX,
                                      int32_t worldSizeY);
+
+/* ---- settings files ---------------------------------------------------------------------- */
+
+/* Processes that a .settings.json may switch on and that this path does not simulate (bits of
+ * the `unsimulated` output of alien_settings_read). */
+#define ALIEN_SETTINGS_ZONES 0x1u                   /* spots (numZones > 0)                       */
+#define ALIEN_SETTINGS_RADIATION_SOURCES 0x2u       /* particle sources                           */
+#define ALIEN_SETTINGS_RADIATION 0x4u               /* cell radiation / high radiation factors    */
+#define ALIEN_SETTINGS_CELL_DEATH 0x8u              /* death consequences / death probability     */
+#define ALIEN_SETTINGS_PARTICLE_SPLIT 0x10u         /* finite particle split energy               */
+#define ALIEN_SETTINGS_RIGIDITY 0x20u               /* rigidity != 0                              */
+#define ALIEN_SETTINGS_PARTICLE_TRANSFORMATION 0x40u
+#define ALIEN_SETTINGS_ABSORPTION_CONTROL 0x80u     /* features.additional absorption control     */
+
+/* Reads ALIEN's .settings.json (AuxiliaryDataParserService::decodeAuxiliaryData,
+ * PersisterInterface/AuxiliaryDataParserService.cpp:1100-1137; the file is written by
+ * SerializerService::serializeSimulationToFiles, SerializerService.cpp:850-916): world size and
+ * timestep of the general part, the physics subset of the simulation parameters (absent entries
+ * take ALIEN's defaults, like ParameterParser::encodeDecode), and in *unsimulated the processes
+ * the file switches on that this path does not simulate.  desc->cellCapacity/particleCapacity are
+ * 0 and desc->randomSeed 0.  timestep and unsimulated may be NULL. */
+int alien_settings_read(const char* json, size_t length, alien_world_desc* desc, alien_params* params,
+                        uint64_t* timestep, uint32_t* unsimulated);
+/* Writes the same subset in ALIEN's layout (nested objects, every value a string), with the
+ * processes this path keeps off written as off; *length is the text length without the final 0
+ * (ALIEN_E_CAPACITY, with *length set, if it does not fit `capacity`). */
+int alien_settings_write(const alien_world_desc* desc, const alien_params* params, uint64_t timestep, char* buf,
+                         size_t capacity, size_t* length);
 
 #ifdef __cplusplus
 }

@@ -26,7 +26,16 @@
 //       bond order after its own force.
 //   R8  absorption: one absorbing particle per cell per step (lowest index), one fusing particle
 //       per sink particle per step (lowest index); a sink does not absorb in the same step.
-//   R9  RNG-driven processes are disabled by the supported parameter set.
+//   R9  random draws are counter-based (seed, timestep, cell id, purpose) instead of a shared
+//       table cursor (CudaNumberGenerator.cuh:99-103); see uniformDraw.
+//   R11 inner friction (CellProcessor.cuh:678-705) is Jacobi: every bond is relaxed twice per
+//       pass for each of its cells (the visit of the cell and the visit of its partner, each of
+//       which averages both velocities in the reference), each time toward the partner's velocity
+//       at the start of the pass, in bond order; the reference's tryLock skips do not happen.
+//   R17 living states under cellDeathConsequences = None (CellProcessor.cuh:493-523, 561-676,
+//       775-858): a cell whose energy is below cellMinEnergy becomes Dying; the partners of a cell
+//       that becomes Dying become Detaching unless they are or become Dying themselves (the
+//       reference's atomicExch order is resolved that way).
 //   R10 dead particles are removed with a stable compaction.
 //   R12 SPH sums gathered per cell in visiting order.
 //   R13 collision neighbours use the uncorrected (seam-blind) distance like Map.cuh:189.
@@ -184,6 +193,33 @@ template <class R> inline bool crossing(V2<R> s, V2<R> e, V2<R> os, V2<R> oe) {
     return lambda >= K<R>::NEAR_ZERO && lambda <= R(1) - K<R>::NEAR_ZERO;
 }
 
+// Counter-based random numbers (R9).  The reference draws from a pre-generated table through a
+// shared atomic cursor (CudaNumberGenerator.cuh:12-104), so its draws depend on thread timing; here
+// a draw is a pure function of (world seed, timestep, cell id, purpose): Philox4x32-10 of Salmon,
+// Moraes, Dror and Shaw ("Parallel random numbers: as easy as 1, 2, 3", SC 2011), counter
+// {id lo, id hi, timestep lo, purpose}, key {seed lo, seed hi}; the first output word's upper 24
+// bits give a float in [0, 1).
+inline void philox4x32_10(uint32_t c[4], uint32_t k0, uint32_t k1) {
+    for (int r = 0; r < 10; ++r) {
+        const uint64_t p0 = uint64_t(0xD2511F53u) * c[0];
+        const uint64_t p1 = uint64_t(0xCD9E8D57u) * c[2];
+        const uint32_t n0 = uint32_t(p1 >> 32) ^ c[1] ^ k0, n1 = uint32_t(p1);
+        const uint32_t n2 = uint32_t(p0 >> 32) ^ c[3] ^ k1, n3 = uint32_t(p0);
+        c[0] = n0;
+        c[1] = n1;
+        c[2] = n2;
+        c[3] = n3;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+}
+constexpr uint32_t kStreamForceDecay = 1;  // CellProcessor.cuh:367-386
+inline float uniformDraw(uint64_t seed, uint64_t timestep, uint64_t id, uint32_t purpose) {
+    uint32_t c[4] = {uint32_t(id), uint32_t(id >> 32), uint32_t(timestep), purpose};
+    philox4x32_10(c, uint32_t(seed), uint32_t(seed >> 32));
+    return float(c[0] >> 8) * (1.0f / 16777216.0f);
+}
+
 // SPH cubic spline (CellProcessor.cuh:92-119)
 template <class R> inline R calcKernel(R q) {
     R r;
@@ -251,8 +287,11 @@ template <class R> struct Params {
     R smoothingLength, viscosityStrength, pressureStrength;
     R cellMaxCollisionDistance, cellRepulsionStrength;
     R friction, cellMaxVelocity, cellMinDistance, cellFusionVelocity, cellMaxBindingEnergy;
+    R innerFriction, cellMaxForceDecayProb;
     R cellMaxBindingDistance[ALIEN_MAX_COLORS];
     R radiationAbsorption[ALIEN_MAX_COLORS];
+    R cellMaxForce[ALIEN_MAX_COLORS];
+    R cellMinEnergy[ALIEN_MAX_COLORS];
     static Params from(const alien_params& p) {
         Params q;
         q.timestepSize = R(p.timestepSize);
@@ -267,9 +306,13 @@ template <class R> struct Params {
         q.cellMinDistance = R(p.cellMinDistance);
         q.cellFusionVelocity = R(p.cellFusionVelocity);
         q.cellMaxBindingEnergy = R(p.cellMaxBindingEnergy);
+        q.innerFriction = R(p.innerFriction);
+        q.cellMaxForceDecayProb = R(p.cellMaxForceDecayProb);
         for (int c = 0; c < ALIEN_MAX_COLORS; ++c) {
             q.cellMaxBindingDistance[c] = R(p.cellMaxBindingDistance[c]);
             q.radiationAbsorption[c] = R(p.radiationAbsorption[c]);
+            q.cellMaxForce[c] = R(p.cellMaxForce[c]);
+            q.cellMinEnergy[c] = R(p.cellMinEnergy[c]);
         }
         return q;
     }
@@ -312,6 +355,7 @@ public:
     Params<R> p;
     alien_params rawParams;
     uint64_t timestep = 0;
+    uint64_t seed = 0;  // key of the counter-based random draws (R9)
     std::vector<Cell<R>> cells;
     std::vector<Particle<R>> parts;
     bool densityResetPending = false;
@@ -332,8 +376,10 @@ public:
     }
 
     void setParams(const alien_params& prm) {
-        if (prm.innerFriction != 0.0f || prm.cellMaxForceDecayProb != 0.0f)
-            throw std::invalid_argument("innerFriction and cellMaxForceDecayProb must be 0");
+        if (!(prm.innerFriction >= 0.0f && prm.innerFriction <= 1.0f))
+            throw std::invalid_argument("innerFriction must lie in [0, 1]");
+        if (!(prm.cellMaxForceDecayProb >= 0.0f && prm.cellMaxForceDecayProb <= 1.0f))
+            throw std::invalid_argument("cellMaxForceDecayProb must lie in [0, 1]");
         if (prm.motionType != ALIEN_MOTION_FLUID && prm.motionType != ALIEN_MOTION_COLLISION)
             throw std::invalid_argument("bad motionType");
         rawParams = prm;
@@ -1067,6 +1113,86 @@ public:
         }
     }
 
+    // ---- inner friction: CellProcessor.cuh:678-705 (R11) ----------------------------------------
+    void innerFriction() {
+        const R f = p.innerFriction;
+        const int n = int(cells.size());
+        std::vector<V2<R>> v0(n);
+        for (int i = 0; i < n; ++i) v0[i] = cells[i].vel;
+        for (int i = 0; i < n; ++i) {
+            auto& c = cells[i];
+            if (c.barrier || c.numConn == 0) continue;
+            V2<R> v = v0[i];
+            for (int visit = 0; visit < 2; ++visit)  // the cell's visit of the bond, then the partner's
+                for (int b = 0; b < c.numConn; ++b) {
+                    const int j = c.bonds[b].target;
+                    if (cells[j].barrier) continue;
+                    const V2<R> avg = (v + v0[j]) / R(2);
+                    v = v * (R(1) - f) + avg * f;
+                }
+            c.vel = v;
+        }
+    }
+
+    // ---- living states (R17), cellDeathConsequences = None ------------------------------------
+    static constexpr uint8_t kReady = 0, kUnderConstruction = 1, kActivating = 2, kDetaching = 3, kReviving = 4,
+                             kDying = 5;  // CellFunctionConstants.h:24-33
+
+    // checkConnections' Detaching marks (CellProcessor.cuh:515-520), aging (:561-594) and the state
+    // transitions (:596-676)
+    void agingAndTransitions(const std::vector<uint8_t>& stretched) {
+        const int n = int(cells.size());
+        for (int i = 0; i < n; ++i)
+            if (stretched[i])
+                for (int b = 0; b < cells[i].numConn; ++b) cells[cells[i].bonds[b].target].livingState = kDetaching;
+        for (auto& c : cells) {
+            if (c.barrier) continue;
+            ++c.age;
+            if (c.livingState == kReady && c.activationTime > 0) --c.activationTime;
+        }
+        std::vector<uint8_t> st(n);
+        for (int i = 0; i < n; ++i) st[i] = cells[i].livingState;
+        for (int i = 0; i < n; ++i) {
+            auto& c = cells[i];
+            uint8_t next = st[i];
+            if (c.barrier) {
+                next = kReady;
+            } else if (st[i] == kActivating || st[i] == kReviving || st[i] == kDetaching) {
+                next = kReady;  // Detaching -> Ready under None
+            } else if (st[i] == kUnderConstruction) {
+                for (int b = 0; b < c.numConn; ++b) {
+                    const int j = c.bonds[b].target;
+                    if (cells[j].creatureId == c.creatureId && st[j] == kActivating) next = kActivating;
+                }
+            }
+            c.livingState = next;
+        }
+    }
+
+    // decay (CellProcessor.cuh:775-858): energy below cellMinEnergy -> Dying, the partners of a
+    // newly dying cell -> Detaching (cellDeathProbability 0: no cell is deleted)
+    void decayToDying() {
+        const int n = int(cells.size());
+        std::vector<uint8_t> low(n, 0), fresh(n, 0);
+        for (int i = 0; i < n; ++i) {
+            const auto& c = cells[i];
+            if (c.barrier) continue;
+            low[i] = c.energy < p.cellMinEnergy[c.color] ? 1 : 0;
+            fresh[i] = low[i] && c.livingState != kDying;
+        }
+        std::vector<uint8_t> before(n);
+        for (int i = 0; i < n; ++i) before[i] = cells[i].livingState;
+        for (int i = 0; i < n; ++i) {
+            if (low[i]) {
+                cells[i].livingState = kDying;
+                continue;
+            }
+            if (before[i] == kDying) continue;
+            for (int b = 0; b < cells[i].numConn; ++b)
+                if (fresh[cells[i].bonds[b].target]) cells[i].livingState = kDetaching;
+        }
+    }
+
     // ---- one timestep: SimulationKernelsLauncher.cu:25-91 ----------------------------------------
     void step() {
         computeGeometry();
@@ -1094,11 +1220,16 @@ public:
                 collisionCell(i, po);
         }
         // checkForces + applyForces (CellProcessor.cuh:367-406)
+        std::vector<uint8_t> delFlag(n, 0);
         for (int i = 0; i < n; ++i) {
             auto& c = cells[i];
             c.pos = po.posCorr[i];
             if (fluid) c.density = po.rho[i];
             if (c.barrier) continue;
+            // a bond break by force (deleting all bonds of an unbonded cell does nothing)
+            if (p.cellMaxForceDecayProb > R(0) && c.numConn > 0 && length(po.force[i]) > p.cellMaxForce[c.color] &&
+                R(uniformDraw(seed, timestep, c.id, kStreamForceDecay)) < p.cellMaxForceDecayProb)
+                delFlag[i] = 1;
             c.vel = c.vel + po.force[i];
             if (length(c.vel) > p.cellMaxVelocity) c.vel = normalized(c.vel) * p.cellMaxVelocity;
         }
@@ -1118,14 +1249,15 @@ public:
                 s2[i] = s1[i];
             }
         }
-        std::vector<uint8_t> delFlag(n, 0);
+        std::vector<uint8_t> stretched(n, 0);
         for (int i = 0; i < n; ++i) {  // checkConnections, CellProcessor.cuh:493-523
             const auto& c = cells[i];
             if (c.barrier) continue;
             for (int b = 0; b < c.numConn; ++b) {
                 V2<R> disp = bondVector(c.bonds[b].target, i, &shift);
-                if (length(disp) > p.cellMaxBindingDistance[c.color % ALIEN_MAX_COLORS]) delFlag[i] = 1;
+                if (length(disp) > p.cellMaxBindingDistance[c.color % ALIEN_MAX_COLORS]) stretched[i] = 1;
             }
+            if (stretched[i]) delFlag[i] = 1;
         }
         connectionForces(considerAngles, s1, &shift);
         for (int i = 0; i < n; ++i) {
@@ -1133,24 +1265,22 @@ public:
             c.pos = c.pos + shift[i];
             torus.correctPosition(c.pos);
         }
-        for (int i = 0; i < n; ++i) {  // verletVelocityUpdate + applyFriction + decay
+        for (int i = 0; i < n; ++i) {  // verletVelocityUpdate, CellProcessor.cuh:546-559
             auto& c = cells[i];
             if (c.barrier) continue;
             V2<R> acc = (s1[i] + s2[i]) / R(2);
             c.vel = c.vel + acc * dt;
+        }
+        agingAndTransitions(stretched);  // SimulationKernels.cu:81-88
+        // applyInnerFriction every third step (SimulationKernelsLauncher.cu:32, 69-71)
+        if (timestep % 3 == 0 && p.innerFriction != R(0)) innerFriction();
+        for (int i = 0; i < n; ++i) {  // applyFriction + decay (SimulationKernels.cu:153-158)
+            auto& c = cells[i];
+            if (c.barrier) continue;
             c.vel = c.vel * (R(1) - p.friction);
             if (c.energy > p.cellMaxBindingEnergy) delFlag[i] = 1;
         }
-        // aging + living state (CellProcessor.cuh:561-676), cellDeathConsequences = None
-        for (auto& c : cells) {
-            if (c.barrier) {
-                c.livingState = 0;
-                continue;
-            }
-            ++c.age;
-            if (c.livingState == 0 && c.activationTime > 0) --c.activationTime;
-            if (c.livingState == 2 || c.livingState == 3 || c.livingState == 4) c.livingState = 0;
-        }
+        decayToDying();
         std::vector<std::pair<int, int>> cands;
         for (int i = 0; i < n; ++i)
             for (auto& pr : po.fusion[i]) cands.push_back(pr);

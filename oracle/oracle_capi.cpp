@@ -68,6 +68,21 @@ void oracle_set_threads(void* hv, int n) {
     });
 }
 
+void oracle_set_seed(void* hv, uint64_t seed) {
+    dispatch(static_cast<Handle*>(hv), [&](auto& w) {
+        w.seed = seed;
+        return 0;
+    });
+}
+
+// one uniform draw of the counter-based generator (for its known-answer test)
+float oracle_uniform_draw(uint64_t seed, uint64_t timestep, uint64_t id, uint32_t purpose) {
+    return oracle::uniformDraw(seed, timestep, id, purpose);
+}
+
+// Philox4x32-10 on a raw counter and key (for the generator's published known-answer vectors)
+void oracle_philox4x32_10(uint32_t* counter, uint32_t k0, uint32_t k1) { oracle::philox4x32_10(counter, k0, k1); }
+
 int oracle_set_params(void* hv, const alien_params* params) {
     return guard([&] { dispatch(static_cast<Handle*>(hv), [&](auto& w) { w.setParams(*params); return 0; }); });
 }

@@ -31,6 +31,10 @@ def lib():
         L.oracle_create.argtypes = [i32, i32, i32, ctypes.POINTER(Params)]
         L.oracle_destroy.argtypes = [vp]
         L.oracle_set_threads.argtypes = [vp, i32]
+        L.oracle_set_seed.argtypes = [vp, u64]
+        L.oracle_uniform_draw.argtypes = [u64, u64, u64, ctypes.c_uint32]
+        L.oracle_uniform_draw.restype = ctypes.c_float
+        L.oracle_philox4x32_10.argtypes = [ctypes.POINTER(ctypes.c_uint32), ctypes.c_uint32, ctypes.c_uint32]
         L.oracle_set_params.argtypes = [vp, ctypes.POINTER(Params)]
         L.oracle_set_data.argtypes = [vp, vp, u64, vp, u64]
         L.oracle_get_counts.argtypes = [vp, ctypes.POINTER(u64), ctypes.POINTER(u64)]
@@ -54,13 +58,14 @@ def _check(rc):
 
 
 class OracleWorld:
-    def __init__(self, size, params, precision=32, threads=1):
+    def __init__(self, size, params, precision=32, threads=1, seed=0):
         self._h = lib().oracle_create(precision, int(size[0]), int(size[1]), ctypes.byref(params))
         if not self._h:
             raise RuntimeError(lib().oracle_last_error().decode())
         self.size = size
         self.precision = precision
         lib().oracle_set_threads(self._h, threads)
+        lib().oracle_set_seed(self._h, int(seed))
 
     def __del__(self):
         if getattr(self, "_h", None):
@@ -141,6 +146,6 @@ def statistics(cells, particles):
 
 
 def from_world(world, precision=32, threads=1):
-    o = OracleWorld(world.size, world.params, precision=precision, threads=threads)
+    o = OracleWorld(world.size, world.params, precision=precision, threads=threads, seed=world.seed)
     o.set_data(world.cells, world.particles)
     return o

@@ -100,7 +100,7 @@ def test_host_only_calls():
     assert L.alien_abi_version() >= 1
     p = abi.Params()
     L.alien_params_default(ctypes.byref(p))
-    ref = abi.default_params()
+    ref = abi.alien_default_params()
     for f, _ in abi.Params._fields_:
         a, b = getattr(p, f), getattr(ref, f)
         if isinstance(a, (int, float)):

@@ -532,7 +532,7 @@ def test_server_results_equal_alone():
     ws = mixed_worlds(10)
     steps = [30, 5, 80, 12, 50, 7, 64, 20, 3, 41]
     srv = engine.Server(budget_bytes=2 << 30, chunk_steps=16)
-    jobs = [srv.submit(w.size, w.params, w.cells, w.particles, s) for w, s in zip(ws, steps)]
+    jobs = [srv.submit(w.size, w.params, w.cells, w.particles, s, seed=w.seed) for w, s in zip(ws, steps)]
     for j in jobs:
         srv.wait(j, timeout=120)
     for j, w, s in zip(jobs, ws, steps):
@@ -550,7 +550,7 @@ def test_server_budget_admission():
     ws = mixed_worlds(4)
     need = max(engine.Server.estimate_bytes(len(w.cells), len(w.particles), w.size) for w in ws)
     srv = engine.Server(budget_bytes=need, chunk_steps=8)
-    jobs = [srv.submit(w.size, w.params, w.cells, w.particles, 10) for w in ws]
+    jobs = [srv.submit(w.size, w.params, w.cells, w.particles, 10, seed=w.seed) for w in ws]
     for j in jobs:
         srv.wait(j, timeout=120)
     infos = [srv.poll(j) for j in jobs]
