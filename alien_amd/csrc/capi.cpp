@@ -270,6 +270,14 @@ int alien_get_statistics(alien_world* w, alien_statistics* out) {
     });
 }
 
+int alien_get_statistics_noalloc(alien_world* w, alien_statistics* out) {
+    return guarded([&] {
+        requireHealthy(w);
+        require(out != nullptr, "null argument");
+        w->set->statisticsNoAlloc(0, out);
+    });
+}
+
 int alien_clear(alien_world* w) {
     return alien_set_data(w, nullptr, 0, nullptr, 0);
 }

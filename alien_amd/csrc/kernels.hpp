@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include "../../include/alien_hip.h"
 #include "engine_types.hpp"
 
 namespace alien {
@@ -46,6 +47,14 @@ void launch_debug_map(const Ctx& c, const LaunchShape& sh, hipStream_t s);
 // launch writing per-block counts and energies (kColors per block)
 int statistics_blocks(int numCells, int partCap, int& cellBlocks);
 void launch_statistics(const Ctx& c, int wi, int cellBlocks, int blocks, int* cnt, double* en, hipStream_t s);
+// the same without allocating: partials and result in the library's per-device `scratch` array
+// (at most kStatScratchBlocks blocks); statistics_scratch gives the current device's address of
+// it (looked up once per device), launch_statistics_noalloc writes the result into *hostOut and
+// returns when it is there (timestep left 0)
+constexpr int kStatScratchBlocks = 65536;
+hipError_t statistics_scratch(void** ptr);
+hipError_t launch_statistics_noalloc(const Ctx& c, int wi, int cellBlocks, int blocks, void* scratchPtr,
+                                     alien_statistics* hostOut, hipStream_t s);
 void launch_debug_neighbors(const Ctx& c, const LaunchShape& sh, const uint64_t* offsets, int* out, int* counts,
                             hipStream_t s);
 

@@ -3,6 +3,9 @@
 // Step order follows _SimulationKernelsLauncher::calcTimestep
 // (source/EngineGpuKernels/SimulationKernelsLauncher.cu:25-91): maps, neighbour forces, particle
 // movement/collision, connection forces around the Verlet updates, friction, structural operations.
+#include <mutex>
+
+#include "../../include/alien_hip.h"
 #include "kernels_common.hpp"
 #include "kernels.hpp"
 
@@ -155,5 +158,91 @@ int statistics_blocks(int numCells, int partCap, int& cellBlocks) {
 void launch_statistics(const Ctx& c, int wi, int cellBlocks, int blocks, int* cnt, double* en, hipStream_t s) {
     if (blocks > 0) hipLaunchKernelGGL(k_statistics, dim3(blocks), dim3(kStatThreads), 0, s, c, wi, cellBlocks, cnt, en);
 }
+
+// ---------------------------------------------------------------------------------------------
+// Statistics without allocation.  The per-block partials of k_statistics and the final record
+// live in one file-scope device array, `scratch`, that every GPU the library runs on holds until
+// the process ends.  Host code never passes the array's name to a kernel (on the host it is only
+// a stand-in, not a device address): the address of each device's copy is looked up once per
+// device with hipGetSymbolAddress, that device current, and only that address is used.
+// Every call on one device shares the array: calls that overlap on one device race.  Nothing in
+// it needs to start at zero: k_statistics writes every partial it covers and k_statistics_total
+// the whole final record.
+// ---------------------------------------------------------------------------------------------
+constexpr size_t kStatEnBytes = sizeof(double) * size_t(kStatScratchBlocks) * kColors;
+constexpr size_t kStatCntBytes = sizeof(int) * size_t(kStatScratchBlocks) * kColors;
+constexpr size_t kStatTotalOff = (kStatEnBytes + kStatCntBytes + 255) / 256 * 256;
+constexpr size_t kStatScratchBytes = kStatTotalOff + (sizeof(alien_statistics) + 255) / 256 * 256;
+static __device__ __align__(256) unsigned char scratch[kStatScratchBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratchMutex;
+void* g_scratchPtr[kMaxDevices] = {};
+}  // namespace
+
+hipError_t statistics_scratch(void** ptr) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratchMutex);
+    if (!g_scratchPtr[dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratchPtr[dev] = p;
+    }
+    *ptr = g_scratchPtr[dev];
+    return hipSuccess;
+}
+
+// The final record from the partials: per colour, counts summed (order-free in integers) and
+// energies added block after block from +0, the same sequence of double additions as the host
+// loop of the allocating path, so both paths return the same bits.  One wave per colour; a wave
+// reads 64 partials at once and its first lane adds them in order.
+__global__ void __launch_bounds__(64 * kColors) k_statistics_total(const int* cnt, const double* en, int cellBlocks,
+                                                                   int blocks, alien_statistics* out) {
+    const int k = int(threadIdx.x) >> 6, lane = int(threadIdx.x) & 63;
+    int cells = 0, parts = 0;
+    double e = 0.0;
+    for (int b0 = 0; b0 < blocks; b0 += 64) {
+        const int b = b0 + lane;
+        const int n = b < blocks ? cnt[size_t(b) * kColors + k] : 0;
+        const double v = b < blocks ? en[size_t(b) * kColors + k] : 0.0;
+        cells += b < cellBlocks ? n : 0;
+        parts += b < cellBlocks ? 0 : n;
+        const int m = min(64, blocks - b0);
+        for (int l = 0; l < m; ++l) e += __shfl(v, l, 64);
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        cells += __shfl_down(cells, o, 64);
+        parts += __shfl_down(parts, o, 64);
+    }
+    if (lane == 0) {
+        out->numCells[k] = cells;
+        out->numParticles[k] = parts;
+        out->totalEnergy[k] = e;
+        if (k == 0) out->timestep = 0;
+    }
+}
+
+hipError_t launch_statistics_noalloc(const Ctx& c, int wi, int cellBlocks, int blocks, void* scratchPtr,
+                                     alien_statistics* hostOut, hipStream_t s) {
+    if (blocks <= 0 || blocks > kStatScratchBlocks) return hipErrorInvalidValue;
+    unsigned char* base = static_cast<unsigned char*>(scratchPtr);
+    double* en = reinterpret_cast<double*>(base);
+    int* cnt = reinterpret_cast<int*>(base + kStatEnBytes);
+    alien_statistics* total = reinterpret_cast<alien_statistics*>(base + kStatTotalOff);
+    hipLaunchKernelGGL(k_statistics, dim3(blocks), dim3(kStatThreads), 0, s, c, wi, cellBlocks, cnt, en);
+    hipLaunchKernelGGL(k_statistics_total, dim3(1), dim3(64 * kColors), 0, s, cnt, en, cellBlocks, blocks, total);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    e = hipMemcpyAsync(hostOut, total, sizeof(alien_statistics), hipMemcpyDeviceToHost, s);
+    if (e != hipSuccess) return e;
+    return hipStreamSynchronize(s);
+}
+
 
 }  // namespace alien

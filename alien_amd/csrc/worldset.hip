@@ -1286,6 +1286,26 @@ alien_statistics WorldSet::statistics(int wi) {
     return st;
 }
 
+void WorldSet::statisticsNoAlloc(int wi, alien_statistics* out) {
+    const DevWorld& w = worlds_.at(wi);
+    int cellBlocks = 0;
+    const int blocks = statistics_blocks(w.numCells, w.partCap, cellBlocks);
+    if (blocks > kStatScratchBlocks)
+        throw Error(ALIEN_E_CAPACITY, "world too large for the allocation-free statistics: " + std::to_string(blocks) +
+                                          " blocks of 256 cells or particle slots, at most " +
+                                          std::to_string(kStatScratchBlocks));
+    if (blocks == 0) {
+        synchronize();
+        *out = alien_statistics{};
+    } else {
+        void* base = nullptr;
+        check_hip(statistics_scratch(&base), "hipGetSymbolAddress (statistics scratch)");
+        check_hip(launch_statistics_noalloc(makeCtx(parity_), wi, cellBlocks, blocks, base, out, stream_),
+                  "statistics");
+    }
+    out->timestep = timestep(wi);  // (the launch returned after the stream drained)
+}
+
 void WorldSet::cellMap(int wi, std::vector<int>& bins, std::vector<int>& slots) {
     synchronize();
     const Ctx c = makeCtx(parity_);

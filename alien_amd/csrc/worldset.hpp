@@ -123,6 +123,9 @@ public:
     void extract(int w, WorldImage& out);
     alien_totals totals(int w);
     alien_statistics statistics(int w);  // per-colour counts and energies, reduced on the device
+    // the same result without allocating device memory (the library's per-device scratch array);
+    // ALIEN_E_CAPACITY beyond kStatScratchBlocks blocks of 256 cells or particle slots
+    void statisticsNoAlloc(int w, alien_statistics* out);
     // cell map of the current state: bin of each cell in map order, and its slot
     void cellMap(int w, std::vector<int>& bins, std::vector<int>& slots);
     // neighbour lists under the step predicate: per slot, neighbour slots in visiting order

@@ -67,6 +67,7 @@ def lib():
         L.alien_set_timestep.argtypes = [vp, u64]
         L.alien_get_totals.argtypes = [vp, ctypes.POINTER(Totals)]
         L.alien_get_statistics.argtypes = [vp, ctypes.POINTER(Statistics)]
+        L.alien_get_statistics_noalloc.argtypes = [vp, ctypes.POINTER(Statistics)]
         L.alien_clear.argtypes = [vp]
         L.alien_debug_get_map.argtypes = [vp, vp, vp, u64, pu64, ctypes.POINTER(i32), ctypes.POINTER(i32),
                                           ctypes.POINTER(i32)]

@@ -184,6 +184,13 @@ int alien_get_totals(alien_world* w, alien_totals* out);
  * Statistics_substep2: cell and particle counts and energies by colour; SimulationFacade.h
  * getStatisticsHistory / getRawStatistics). */
 int alien_get_statistics(alien_world* w, alien_statistics* out);
+/* alien_get_statistics without any device allocation.  No counterpart in the reference: it is
+ * for callers that cannot allocate (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  Partial sums and the result live in a device array of the library
+ * (about 5.5 MB per GPU, held until the process ends), so the world's cells plus particle slots
+ * may number at most 65536 x 256 = 16,777,216 (ALIEN_E_CAPACITY beyond).  Same result as
+ * alien_get_statistics, bit for bit.  Calls that overlap on one GPU race on that array. */
+int alien_get_statistics_noalloc(alien_world* w, alien_statistics* out);
 /* clear (SimulationCudaFacade.cuh:89): removes all cells and particles. */
 int alien_clear(alien_world* w);
 
