@@ -25,6 +25,8 @@ constexpr int kRowPool = 256;  // entries of three consecutive staged rows
 constexpr int kStripMax = 64;
 constexpr int kChunkMax = 8;  // bin rows per neighbour-pass chunk
 constexpr int kClaimFree = 0x7fffffff;
+constexpr uint32_t kMoverBit = 0x80000000u;  // binOf mark of a cell that changed bins
+constexpr int kXcdCount = 8;  // XCDs of an MI355X: workgroup b of a launch runs on XCD b % 8
 // Size limits of one world set.  Cell indices, bin offsets and the bond planes ([kMaxBonds][cap],
 // addressed as b * cap + x) are 32-bit ints in the kernels, so a set holds at most
 // (2^31 - 1) / kMaxBonds cells; particle indices and the uint32 bin scan over cells plus particles
@@ -80,6 +82,7 @@ struct DevParams {
 struct DevWorld {
     int W, H;
     int binSize, nbx, nby;  // cell bins (row-major)
+    int binShift;           // log2(binSize) when binSize is a power of two, else -1
     int pnbx, pnby;         // particle bins of kParticleBin units
     int cellOff, numCells;  // hot and cold cell arrays
     int binOff;             // first cell bin in the combined bin arrays
@@ -151,21 +154,27 @@ struct Ctx {
     uint32_t* activation;     // cold [cellCap]: activationTime (worlds with livingWork)
     uint32_t* creature;       // cold [cellCap]: creatureId (worlds with livingWork)
     uint8_t* stretched;       // [cellCap] end-of-step staged cell had a bond beyond cellMaxBindingDistance
-    int* binOf;     // [cellCap] global bin of A-cell i
+    int* binOf;     // [cellCap] global bin of A-cell i, | kMoverBit if it changed bins
     int* pbinOf;    // [partCap] global bin of A-particle i, -1 if dead
-    // Bin sizes.  A cell bin's size is priv[b] + count[b]: priv holds the cells counted by the
-    // neighbour-pass wave that owns the bin (plain stores, every cell bin written every step),
-    // count the cells counted with global atomics (those that left their wave's region, bonded
-    // and barrier cells, seeding, and all particle bins).  The scatters bring `count` back to zero.
+    // Bin sizes.  Cell bins: written by the map count at the start of every step (the cells that
+    // land in the bin); particle bins: counted with atomics by the kernels that produce the
+    // end-of-step particle positions (and by seeding), brought back to zero by the particle scatter.
     uint32_t* count;      // [numBins] start-of-step state
     uint32_t* countNext;  // [numBins] end-of-step state (next step's `count`)
-    uint32_t* priv;       // [numCellBins] start-of-step state
-    uint32_t* privNext;   // [numCellBins] end-of-step state
-    int* arrival;         // [cellCap] per end-of-step cell: its place among its bin's arrivals
     int numCellBins;
-    uint32_t* start;      // [numBins + 1] exclusive scan
+    uint32_t* start;      // [numBins + 1] exclusive scan of this step's map
+    const uint32_t* startPrev;  // [numBins + 1] the last step's map (all zero after seeding)
+    // Cells that change bins ("movers"; after seeding every cell) are listed per new cell bin:
+    // moverHead[bin] is one of them (A index), moverNext[i] the next (kNone ends a list).
+    // moverHeadNext: the lists of the next step (built during this step).
+    int* moverHead;
+    int* moverHeadNext;
+    int* moverNext;  // [cellCap]
+    // per cell bin: movers in minus movers out (cell-bin size = last size + delta)
+    int* delta;
+    int* deltaNext;
     uint32_t* blockSums;
-    int* sorted;   // [cellCap] map order (bin, then arrival) -> A index
+    int* sorted;   // [cellCap] map order (bin, then previous index) -> A index
     // [cellCap] the start-of-step state of the cells in map order, written next to `sorted`, so
     // that the neighbour pass reads each row of bins as one contiguous range
     float2* mpos;

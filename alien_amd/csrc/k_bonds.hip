@@ -223,10 +223,10 @@ __global__ void k_conn_verlet_pos(Ctx c) {
 }
 
 // Verlet velocity step, friction and decay flag (CellProcessor.cuh:546-559, 707-723, 786-790); the
-// moved position is written here, and counted into the histogram of the next step
+// moved position is written here, and its bin recorded for the next step's map
 __global__ void k_conn_verlet_vel(Ctx c) {
     int x, wi;
-    const bool st = staged_cell(c, x, wi);  // (every lane stays for the wave-wide binning)
+    const bool st = staged_cell(c, x, wi);
     const DevWorld& w = c.worlds[wi];
     const DevParams& P = c.params[wi];
     const float dt = P.dt, friction = P.friction, maxEnergy = P.maxBindingEnergy;
@@ -245,10 +245,12 @@ __global__ void k_conn_verlet_vel(Ctx c) {
         energy = c.energy[w.cellOff + c.B.slot[x]];
         if (!flags_barrier(fx)) s1 = gather_conn(c, w, x, flags_num_conn(fx));
     }
+    if (!st) return;
     const f2 p2 = torus_pos(add(p, sh), w.W, w.H);
-    if (st) c.B.pos[x] = to_float2(p2);
-    bin_for_next_step_runs(c, w, x, p2, c.countNext, st);
-    if (!st || flags_barrier(fx)) return;
+    c.B.pos[x] = to_float2(p2);
+    // (the map copy holds the start-of-step position at the cell's end-of-step index)
+    bin_next(c, w, x, p2, cell_bin_of(w, to_f2(c.mpos[x])));
+    if (flags_barrier(fx)) return;
     const f2 acc = mul(add(s1, s2), 0.5f);
     f2 v = add(v0, mul(acc, dt));
     if (inner) {

@@ -245,42 +245,11 @@ __device__ __forceinline__ bool is_bonded(const int bt[kMaxBonds], int jLocal) {
     return bonded;
 }
 
-// The bins a neighbour-pass wave owns (x0..x0+IX-1, y0..y0+IY-1) and its LDS histogram of the
-// next step's cells in them, two 16-bit counters per word (bin b in half b & 1 of word b >> 1; a
-// wave counts at most its own cells, far below 2^16); hist == nullptr counts every cell in the
-// shared histogram.
-struct Region {
-    uint32_t* hist;
-    int x0, y0, IX, IY;
-};
-
-// Next-step bin of an unbonded cell: counted in the owning wave's LDS histogram when the bin is
-// in the wave's region, else in the shared histogram.
-__device__ __forceinline__ void bin_next(const Ctx& c, const DevWorld& w, int k, f2 p, const Region& R) {
-    int sx, sy;
-    slot_of(p, w.W, w.H, sx, sy);
-    const int bx = min(div_small(sx, w.binSize), w.nbx - 1), by = min(div_small(sy, w.binSize), w.nby - 1);
-    const int gb = w.binOff + by * w.nbx + bx;
-    c.binOf[k] = gb;
-    const int lx = bx - R.x0, ly = by - R.y0;
-    if (R.hist && lx >= 0 && lx < R.IX && ly >= 0 && ly < R.IY) {
-        const int b = ly * R.IX + lx;
-        const uint32_t sh = uint32_t(b & 1) << 4;
-        c.arrival[k] = int((atomicAdd(&R.hist[b >> 1], 1u << sh) >> sh) & 0xFFFFu);
-    } else {
-        c.arrival[k] = kArrivalShared | int(atomicAdd(&c.countNext[gb], 1u));
-    }
-}
-
-__device__ __forceinline__ uint32_t hist_count(const uint32_t* hist, int b) {
-    return (hist[b >> 1] >> (uint32_t(b & 1) << 4)) & 0xFFFFu;
-}
-
 // Barrier reflection, applyForces, and the end-of-step writes of one cell.  A bonded or barrier
 // cell is marked staged for the bond kernels of the step.
 template <bool kFluid>
 __device__ __forceinline__ void cell_finish(const Ctx& c, const DevWorld& w, const DevParams& P, int wi, const Nb& s,
-                                            int slot, const int bt[kMaxBonds], Acc& a, const Region& R) {
+                                            int slot, const int bt[kMaxBonds], Acc& a, int gbOld = -1) {
     const int cap = c.cellCap;
     const float Wf = float(w.W), Hf = float(w.H);
     const int i = s.aidx, k = s.lidx;
@@ -331,11 +300,14 @@ __device__ __forceinline__ void cell_finish(const Ctx& c, const DevWorld& w, con
     c.B.slot[k] = slot;
     c.B.dens[k] = kFluid ? a.rho : s.dens;
     if constexpr (kFluid) {
-        for (int b = 0; b < nci; ++b) {
-            c.B.btgt[b * cap + k] = bt[b];
-            c.B.brest[b * cap + k] = c.A.brest[b * cap + i];
-            c.B.bang[b * cap + k] = c.A.bang[b * cap + i];
-        }
+        // (unrolled: a loop to nci would index bt at run time and place it in scratch memory)
+#pragma unroll
+        for (int b = 0; b < kMaxBonds; ++b)
+            if (b < nci) {
+                c.B.btgt[b * cap + k] = bt[b];
+                c.B.brest[b * cap + k] = c.A.brest[b * cap + i];
+                c.B.bang[b * cap + k] = c.A.bang[b * cap + i];
+            }
         if (nci > 0) c.bstiff[k] = c.stiff[w.cellOff + slot];
     } else {
         // (collision worlds are mostly bonded cells: their bond data is loaded before any of it
@@ -360,7 +332,9 @@ __device__ __forceinline__ void cell_finish(const Ctx& c, const DevWorld& w, con
     // checkForces (CellProcessor.cuh:367-386): a bonded cell whose force is above cellMaxForce loses
     // all its bonds with probability cellMaxForceDecayProb (deleting the bonds of an unbonded cell
     // would change nothing)
-    if (P.decayProb > 0.0f && nci > 0 && !bari && lensq(F) > P.maxForceSq[flags_color(fi)]) {
+    // (the per-colour threshold is read from memory: indexing the register copy of the
+    // parameters by a run-time colour would place the whole copy in scratch memory)
+    if (P.decayProb > 0.0f && nci > 0 && !bari && lensq(F) > c.params[wi].maxForceSq[flags_color(fi)]) {
         const unsigned long long id = c.cid[w.cellOff + slot];
         if (uniform_draw(w.seed, c.ctr[wi].timestep, id, kStreamForceDecay) < P.decayProb) c.delFlag[k] = 1;
     }
@@ -380,7 +354,7 @@ __device__ __forceinline__ void cell_finish(const Ctx& c, const DevWorld& w, con
         v = mul(v, 1.0f - P.friction);
         c.B.pos[k] = to_float2(p2);
         c.B.vel[k] = to_float2(v);
-        bin_next(c, w, k, p2, R);
+        bin_next(c, w, k, p2, gbOld >= 0 ? gbOld : cell_bin_of(w, pi));
     } else {
         c.B.pos[k] = to_float2(pc);
         c.B.vel[k] = to_float2(v);
@@ -391,7 +365,7 @@ __device__ __forceinline__ void cell_finish(const Ctx& c, const DevWorld& w, con
 // One cell, neighbours visited one after another (forEachNb calls visit(o) in stencil order).
 template <bool kFluid, class ForEach>
 __device__ __forceinline__ void cell_direct(const Ctx& c, const DevWorld& w, const DevParams& P, int wi, const Nb& s,
-                                            const Region& R, ForEach&& forEachNb) {
+                                            ForEach&& forEachNb) {
     const float Wf = float(w.W), Hf = float(w.H);
     int bt[kMaxBonds];
     load_bonds(c, s, bt);
@@ -404,7 +378,7 @@ __device__ __forceinline__ void cell_direct(const Ctx& c, const DevWorld& w, con
                                              [&] { return make_int2(s.lidx, o.lidx); });
         acc_add<kFluid>(a, f, t);
     });
-    cell_finish<kFluid>(c, w, P, wi, s, c.A.slot[s.aidx], bt, a, R);
+    cell_finish<kFluid>(c, w, P, wi, s, c.A.slot[s.aidx], bt, a);
 }
 
 
@@ -461,7 +435,6 @@ struct RowShared {
             uint16_t pnb[kPairCap];
         } pr;
     } u;
-    uint32_t hist[kStripMax * kChunkMax / 2];  // next-step cells of the wave's own bins (16-bit pairs)
 };
 
 // registers of a row in flight: bin counters of the bin slots lane, lane + 64, lane + 128, then
@@ -641,7 +614,7 @@ __device__ __forceinline__ int pick3(const int (&a)[3], int r) { return r == 0 ?
 // The cells of staged row Y (ring row rc; rows Y-1 and Y+1 in rm_ and rp).
 template <bool kFluid, int kSub, class RS>
 __device__ __forceinline__ void row_compute(const Ctx& c, const DevWorld& w, const DevParams& P, int wi,
-                                            const StripGeom& g, RS& S, const Region& R, int rm_, int rc, int rp) {
+                                            const StripGeom& g, RS& S, int rm_, int rc, int rp) {
     const int lane = threadIdx.x;
     const float Wf = float(w.W), Hf = float(w.H);
     // the cells: the interior bins of each bin row of the staged row
@@ -825,7 +798,7 @@ __device__ __forceinline__ void row_compute(const Ctx& c, const DevWorld& w, con
             self.flags = c.A.flags[self.aidx];
             if (direct) {
                 atomicAdd(&c.sc->directCells, 1ull);
-                cell_direct<kFluid>(c, w, P, wi, self, R, [&](auto&& visit) {
+                cell_direct<kFluid>(c, w, P, wi, self, [&](auto&& visit) {
 #pragma unroll
                     for (int r = 0; r < 3; ++r)
                         for (int u = a0s[r]; u < a0s[r] + ns[r]; ++u)
@@ -836,7 +809,7 @@ __device__ __forceinline__ void row_compute(const Ctx& c, const DevWorld& w, con
                 // work (few cells have bonds in a fluid world; the loads hit the cache)
                 int bt2[kMaxBonds];
                 load_bonds(c, self, bt2);
-                cell_finish<kFluid>(c, w, P, wi, self, slot, bt2, a, R);
+                cell_finish<kFluid>(c, w, P, wi, self, slot, bt2, a);
             }
         }
     }
@@ -848,7 +821,6 @@ template <bool kFluid>
 __device__ __forceinline__ void rows_direct(const Ctx& c, const DevWorld& w, const DevParams& P, int wi, const StripGeom& g, int y0,
                             int IY) {
     const uint32_t base = g.base;
-    const Region R{nullptr, 0, 0, 0, 0};  // every cell in the shared histogram
     auto binRange = [&](int gb, int& s0, int& n) {
         s0 = g.cellOff + int(c.start[gb] - base);
         n = int(c.start[gb + 1] - c.start[gb]);
@@ -872,7 +844,7 @@ __device__ __forceinline__ void rows_direct(const Ctx& c, const DevWorld& w, con
             binRange(strip_bin(g, y, hx), bs, bn);
             const int i = c.sorted[m];
             const Nb self = global_nb<kFluid>(c, i, bs + rank_in_bin(c.sorted, bs, bs + bn, i));
-            cell_direct<kFluid>(c, w, P, wi, self, R, [&](auto&& visit) {
+            cell_direct<kFluid>(c, w, P, wi, self, [&](auto&& visit) {
                 for (int dy = -1; dy <= 1; ++dy)
                     for (int dx = -1; dx <= 1; ++dx) {
                         int s0, n;
@@ -926,8 +898,6 @@ __device__ __forceinline__ bool chunk_rows(const Ctx& c, const DevWorld& w, cons
     // world is at least four: the torus difference is the plain difference there, bit for bit.
     g.seam = g.x0 == 0 || g.x0 + g.IX >= g.nbx || y0 == 0 || y0 + IY >= nbyS || g.nbx < 8 || g.nby < 8;
     if (!stage_fits<kSub>(c, g, y0, IY)) return false;
-    const Region R{S.hist, g.x0, kSub * y0, g.IX, kSub * IY};
-    for (int b = lane; b < (g.IX * kSub * IY + 1) / 2; b += kRowThreads) S.hist[b] = 0u;
 
     RowFlight<kSub> F, G;
     int head = 0;  // pool position of the next committed row
@@ -947,7 +917,7 @@ __device__ __forceinline__ bool chunk_rows(const Ctx& c, const DevWorld& w, cons
             row_issue_entries<kSub>(c, g, S, F);
             if (y + 3 <= y0 + IY) row_issue_counts<kSub>(c, g, y + 3, G);
         }
-        row_compute<kFluid, kSub>(c, w, P, wi, g, S, R, rm_, rc, rp);
+        row_compute<kFluid, kSub>(c, w, P, wi, g, S, rm_, rc, rp);
         if (next) {
             row_commit<kSub>(g, S, F, rm_, head);  // row y+2 replaces row y-1
             head += F.total;
@@ -958,11 +928,6 @@ __device__ __forceinline__ bool chunk_rows(const Ctx& c, const DevWorld& w, cons
             }
         }
     }
-    // the region's owner-counted bin sizes for the next step (every bin written, zeros included)
-    __syncthreads();
-    for (int r = 0; r < kSub * IY; ++r)
-        for (int x = lane; x < g.IX; x += kRowThreads)
-            c.privNext[g.binOff + (kSub * y0 + r) * g.nbx + g.x0 + x] = hist_count(S.hist, r * g.IX + x);
     return true;
 }
 
@@ -1008,10 +973,296 @@ k_neighbor_rows(Ctx c) {
         // owner-counted part of the region's bins is empty
         if (threadIdx.x == 0) atomicAdd(&c.sc->fallbackChunks, 1ull);
         rows_direct<kFluid>(c, w, P, wi, g, by0, bIY);
-        for (int r = 0; r < bIY; ++r)
-            for (int x = threadIdx.x; x < g.IX; x += kRowThreads)
-                c.privNext[g.binOff + (by0 + r) * g.nbx + g.x0 + x] = 0u;
     }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Cell-parallel fluid pass.  The map is in logical order (cells by bin, inside a bin by previous
+// index), so the three stencil bins of a cell in one bin row are one contiguous range of the map
+// copy, and a wave of 64 consecutive map positions reads its candidates straight from it (the
+// ranges of neighbouring lanes overlap and stay in the caches).  Per wave: every lane tests its
+// candidates into a mask in visiting order; the wave's neighbour pairs are listed back to back in
+// LDS and evaluated one pair per lane; every lane then adds its own pairs' terms in visiting order
+// (the float sums of a sequential stencil walk, bit for bit) and finishes its cell.  A cell in the
+// first or last bin column (its stencil wraps around the torus there), one with more than 64
+// candidates, and every cell of a grid narrower than three bins walks its stencil alone
+// (cell_direct, same order and results).  Workgroups of 1..4 waves (the cell block size), each
+// wave on its own with its own LDS (wave-level synchronisation only); block b
+// runs table entry (b mod 8) * ceil(T / 8) + b / 8, so each XCD sweeps one contiguous band of the
+// map and the stencil rows its waves share stay in its L2.
+// ---------------------------------------------------------------------------------------------
+constexpr int kCellWaves = 4;
+constexpr int kCellRec = 256;  // pair records of one wave at a time
+
+constexpr int kCellStage = 192;  // staged candidate positions per stencil row (three per lane)
+
+struct CellShared {
+    // the wave's own cells (the self side of its pairs)
+    float2 pos[64];
+    float2 vel[64];
+    float dens[64];
+    uint32_t flags[64];
+    int aidx[64];
+    union {
+        // while candidates are tested: positions of the union of the lanes' ranges, per stencil row
+        float2 cpos[3][kCellStage];
+        struct {
+            // pair records: neighbour map position, owner lane | bonded << 7
+            int rnb[kCellRec];
+            uint8_t rown[kCellRec];
+            // results of the pairs evaluated in one round: terms and the term flags
+            float res[kPairTerms + 1][64];
+        } pr;
+    } u;
+};
+
+__device__ __forceinline__ int wave_min_int(int v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
+    return v;
+}
+__device__ __forceinline__ int wave_max_int(int v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+// the map-copy entry at map position m as a neighbour
+__device__ __forceinline__ Nb map_nb(const Ctx& c, int m) {
+    Nb o;
+    o.pos = to_f2(c.mpos[m]);
+    o.vel = to_f2(c.mvel[m]);
+    o.dens = c.mdens[m];
+    o.flags = c.mflags[m];
+    o.aidx = c.sorted[m];
+    o.lidx = m;
+    return o;
+}
+
+__global__ void __launch_bounds__(64 * kCellWaves) k_neighbor_cells(Ctx c, int tableBlocks) {
+    __shared__ CellShared SS[kCellWaves];
+    const int per = (tableBlocks + kXcdCount - 1) / kXcdCount;
+    const int e = int(blockIdx.x % kXcdCount) * per + int(blockIdx.x / kXcdCount);
+    if (e >= tableBlocks) return;
+    const int2 blk = c.cellBlocks[e];
+    const int wi = blk.x;
+    const DevWorld w = c.worlds[wi];
+    const DevParams P = c.params[wi];
+    if (P.motion != 0) return;
+    const int lane = int(threadIdx.x) & 63, wave = int(threadIdx.x) >> 6;
+    CellShared& S = SS[wave];
+    const int m = blk.y + int(threadIdx.x);
+    const int mEnd = w.cellOff + w.numCells;
+    if (__builtin_amdgcn_readfirstlane(blk.y + wave * 64) >= mEnd) return;  // the whole wave is past the world
+    const bool active = m < mEnd;
+    const float Wf = float(w.W), Hf = float(w.H);
+    const float w0 = sph_w(0.0f) * P.invH2;  // a cell's own density term
+    const uint32_t base = c.start[w.binOff];
+    auto S0 = [&](int b) { return w.cellOff + int(c.start[w.binOff + b] - base); };
+
+    Nb self;
+    self.pos = {0.0f, 0.0f};
+    self.vel = {0.0f, 0.0f};
+    self.dens = 0.0f;
+    self.flags = 0u;
+    self.aidx = 0;
+    self.lidx = m;
+    int bx = 0, by = 0;
+    if (active) {
+        self = map_nb(c, m);
+        cell_bin_xy(w, self.pos, bx, by);
+    }
+    S.pos[lane] = to_float2(self.pos);
+    S.vel[lane] = to_float2(self.vel);
+    S.dens[lane] = self.dens;
+    S.flags[lane] = self.flags;
+    S.aidx[lane] = self.aidx;
+    // stencil rows above, own, below: one map range each (bins bx-1 .. bx+1)
+    int lo[3] = {0, 0, 0}, ns[3] = {0, 0, 0};
+    bool direct = active && (bx == 0 || bx == w.nbx - 1);
+    if (active && !direct) {
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+            const int rb = wrap_small(by + r - 1, w.nby) * w.nbx;
+            lo[r] = S0(rb + bx - 1);
+            ns[r] = S0(rb + bx + 2) - lo[r];
+        }
+        if (ns[0] + ns[1] + ns[2] > 64) direct = true;
+    }
+    if (direct) ns[0] = ns[1] = ns[2] = 0;
+    // away from the seam rows (and in grids of at least eight bins each way) the plain difference
+    // equals the torus difference bit for bit: two stencil neighbours lie less than three bin
+    // edges apart, less than half the world
+    const bool seam = __any(active && (by == 0 || by == w.nby - 1)) || w.nbx < 8 || w.nby < 8;
+    unsigned long long mask = 0ull, bmask = 0ull;
+    int selfIdx = -1;  // pairs before the cell's own density term
+    int bt[kMaxBonds];
+    load_bonds(c, self, bt);
+    // The union of the lanes' ranges of each stencil row is staged in LDS (all its loads in flight
+    // together) when it fits; a wave whose cells span two bin rows tests from global memory.
+    const bool tests = active && !direct;
+    int u0[3], un[3];  // the staged range of each stencil row: [u0, u0 + un)
+    bool staged = __any(tests);
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+        u0[r] = wave_min_int(tests ? lo[r] : 0x7fffffff);
+        const int u1 = wave_max_int(tests ? lo[r] + ns[r] : 0);
+        un[r] = staged ? u1 - u0[r] : 0;
+        staged = staged && un[r] <= kCellStage;
+    }
+    if (staged) {
+        float2 v[3][3];
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+            for (int q = 0; q < 3; ++q)
+                v[r][q] = lane + 64 * q < un[r] ? c.mpos[u0[r] + lane + 64 * q] : float2{0.0f, 0.0f};
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+            for (int q = 0; q < 3; ++q) S.u.cpos[r][lane + 64 * q] = v[r][q];
+    }
+    __syncwarp();
+    if (tests) {
+        // the three ranges as one list in visiting order: candidate t lies in row 0 below n0, in
+        // row 1 below n01, else in row 2 (the wave runs its longest list, not its longest row)
+        const int n0 = ns[0], n01 = ns[0] + ns[1], nt = n01 + ns[2];
+        auto test = [&](auto seamTag, auto stagedTag) {
+            constexpr bool kSeam = decltype(seamTag)::value;
+            constexpr bool kStaged = decltype(stagedTag)::value;
+            // position of candidate t: staged row and index, or map position
+            auto cand = [&](int t) -> f2 {
+                if constexpr (kStaged) {
+                    const int r = t < n0 ? 0 : (t < n01 ? 1 : 2);
+                    const int q = t - (r == 0 ? 0 : (r == 1 ? n0 : n01)) + (r == 0 ? lo[0] - u0[0] : (r == 1 ? lo[1] - u0[1] : lo[2] - u0[2]));
+                    return to_f2(S.u.cpos[r][min(q, kCellStage - 1)]);
+                } else {
+                    const int r = t < n0 ? 0 : (t < n01 ? 1 : 2);
+                    const int m = (r == 0 ? lo[0] : (r == 1 ? lo[1] - n0 : lo[2] - n01)) + t;
+                    return to_f2(c.mpos[min(m, mEnd - 1)]);
+                }
+            };
+            for (int t = 0; t < nt; t += 2) {
+                // two candidates per iteration; the second may pass the list's end (trimmed below)
+                const f2 p0 = cand(t), p1 = cand(t + 1);
+                const float d0 = lensq(kSeam ? torus_delta(self.pos, p0, Wf, Hf) : sub(self.pos, p0));
+                const float d1 = lensq(kSeam ? torus_delta(self.pos, p1, Wf, Hf) : sub(self.pos, p1));
+                mask |= d0 <= P.reachSq ? (1ull << t) : 0ull;
+                mask |= d1 <= P.reachSq ? (2ull << t) : 0ull;
+            }
+            mask &= nt >= 64 ? ~0ull : (1ull << nt) - 1ull;
+        };
+        if (staged) {
+            if (seam)
+                test(std::true_type{}, std::true_type{});
+            else
+                test(std::false_type{}, std::true_type{});
+        } else {
+            if (seam)
+                test(std::true_type{}, std::false_type{});
+            else
+                test(std::false_type{}, std::false_type{});
+        }
+        // a barrier cell meets itself as a barrier neighbour; any other cell adds its own density
+        // term at its place in the visiting order, without a pair
+        const int selfFlat = ns[0] + (m - lo[1]);
+        if (!flags_barrier(self.flags)) {
+            mask &= ~(1ull << selfFlat);
+            selfIdx = __builtin_popcountll(mask & ((1ull << selfFlat) - 1ull));
+        }
+        if (flags_num_conn(self.flags) > 0) {
+            unsigned long long mm = mask;
+            while (mm) {
+                const int ci = __builtin_ctzll(mm);
+                mm &= mm - 1ull;
+                const int r = ci < ns[0] ? 0 : (ci < ns[0] + ns[1] ? 1 : 2);
+                const int rest = ci - (r == 0 ? 0 : (r == 1 ? ns[0] : ns[0] + ns[1]));
+                const int jm = (r == 0 ? lo[0] : (r == 1 ? lo[1] : lo[2])) + rest;
+                if (is_bonded(bt, c.sorted[jm] - w.cellOff)) bmask |= 1ull << ci;
+            }
+        }
+    }
+    const int count = __builtin_popcountll(mask);
+    const int inc = int(wave_inclusive_scan(uint32_t(count)));
+    const int numPairs = wave_last(inc);
+    const int pbase = inc - count;
+    __syncwarp();  // (the pair records reuse the staged positions' LDS)
+
+    Acc a;
+    acc_init(a, self);
+    for (int R0 = 0; R0 < numPairs; R0 += kCellRec) {
+        const int rend = min(numPairs, R0 + kCellRec);
+        // owners list their pairs of this window, in visiting order
+        if (count > 0 && pbase < rend && pbase + count > R0) {
+            unsigned long long mm = mask;
+            for (int p = pbase; mm; ++p) {
+                const int ci = __builtin_ctzll(mm);
+                mm &= mm - 1ull;
+                if (p < R0 || p >= rend) continue;
+                const int r = ci < ns[0] ? 0 : (ci < ns[0] + ns[1] ? 1 : 2);
+                const int rest = ci - (r == 0 ? 0 : (r == 1 ? ns[0] : ns[0] + ns[1]));
+                S.u.pr.rnb[p - R0] = (r == 0 ? lo[0] : (r == 1 ? lo[1] : lo[2])) + rest;
+                S.u.pr.rown[p - R0] = uint8_t(lane | int((bmask >> ci) & 1ull) << 7);
+            }
+        }
+        __syncwarp();
+        for (int off = R0; off < rend; off += 64) {
+            // pair terms, one pair per lane
+            const int p = off + lane;
+            float tm[kPairTerms];
+#pragma unroll
+            for (int v = 0; v < kPairTerms; ++v) tm[v] = 0.0f;
+            uint32_t f = 0u;
+            if (p < rend) {
+                const int jm = S.u.pr.rnb[p - R0];
+                const int ow = S.u.pr.rown[p - R0];
+                const int ol = ow & 63;
+                Nb s;
+                s.pos = to_f2(S.pos[ol]);
+                s.vel = to_f2(S.vel[ol]);
+                s.dens = S.dens[ol];
+                s.flags = S.flags[ol];
+                s.aidx = S.aidx[ol];
+                s.lidx = blk.y + wave * 64 + ol;
+                const Nb o = map_nb(c, jm);
+                f = pair_term<true>(c, w, P, wi, s, o, (ow >> 7) != 0, tm, seam,
+                                    [&] { return make_int2(s.lidx, o.lidx); });
+            }
+#pragma unroll
+            for (int v = 0; v < kPairTerms; ++v) S.u.pr.res[v][lane] = tm[v];
+            S.u.pr.res[kPairTerms][lane] = __uint_as_float(f);
+            __syncwarp();
+            // every owner adds its pairs of this round in visiting order
+            const int jlo = max(pbase, off), jhi = min(pbase + count, min(off + 64, rend));
+            for (int j = jlo; j < jhi; ++j) {
+                const int q = j - off;
+                float tt[kPairTerms];
+#pragma unroll
+                for (int v = 0; v < kPairTerms; ++v) tt[v] = S.u.pr.res[v][q];
+                const uint32_t ff = __float_as_uint(S.u.pr.res[kPairTerms][q]);
+                if (j - pbase == selfIdx) a.rho = a.rho + w0;
+                acc_add<true>(a, ff, tt);
+            }
+            __syncwarp();
+        }
+    }
+    if (!active) return;
+    if (direct) {
+        atomicAdd(&c.sc->directCells, 1ull);
+        cell_direct<true>(c, w, P, wi, self, [&](auto&& visit) {
+            for (int dy = -1; dy <= 1; ++dy) {
+                const int rb = wrap_small(by + dy, w.nby) * w.nbx;
+                for (int dx = -1; dx <= 1; ++dx) {
+                    const int b = rb + wrap_small(bx + dx, w.nbx);
+                    const int s1 = S0(b + 1);
+                    for (int j = S0(b); j < s1; ++j) visit(map_nb(c, j));
+                }
+            }
+        });
+        return;
+    }
+    if (selfIdx == count) a.rho = a.rho + w0;  // the own term after all pairs
+    cell_finish<true>(c, w, P, wi, self, c.A.slot[self.aidx], bt, a, w.binOff + by * w.nbx + bx);
 }
 
 int neighbor_blocks_per_cu(bool fluid) {
@@ -1022,9 +1273,12 @@ int neighbor_blocks_per_cu(bool fluid) {
 }
 
 void launch_neighbor(const Ctx& c, const LaunchShape& sh, hipStream_t s, bool fluid, bool collision) {
-    if (!sh.tileBlocks) return;
-    if (fluid) hipLaunchKernelGGL(k_neighbor_rows<true>, dim3(sh.tileBlocks), dim3(kRowThreads), 0, s, c);
-    if (collision) hipLaunchKernelGGL(k_neighbor_rows<false>, dim3(sh.tileBlocks), dim3(kRowThreads), 0, s, c);
+    if (fluid && sh.cellBlocks) {
+        const int grid = (sh.cellBlocks + kXcdCount - 1) / kXcdCount * kXcdCount;
+        hipLaunchKernelGGL(k_neighbor_cells, dim3(grid), dim3(sh.cellThreads), 0, s, c, sh.cellBlocks);
+    }
+    if (collision && sh.tileBlocks)
+        hipLaunchKernelGGL(k_neighbor_rows<false>, dim3(sh.tileBlocks), dim3(kRowThreads), 0, s, c);
 }
 
 }  // namespace alien

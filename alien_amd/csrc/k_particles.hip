@@ -48,7 +48,8 @@ __global__ void k_part_target(Ctx c) {
         int want = -1;
         if (t >= 0 && t != k && lensq(sub(p, to_f2(c.B.ppos[t]))) < 0.5f) want = t;
         c.want[k] = want;
-        // cell hit this step: lowest-index cell of the slot at map-build time (cellMap.getFirst)
+        // cell hit this step: lowest-index cell of the slot at map-build time (cellMap.getFirst),
+        // the first of the bin's cells in map order (a bin holds its cells by previous index)
         int cell = -1;
         if (want < 0) {
             const f2 q = add(p, to_f2(c.B.pvel[k]));
@@ -56,14 +57,14 @@ __global__ void k_part_target(Ctx c) {
             slot_of(q, w.W, w.H, qx, qy);
             int cs, ce;
             cell_bin_range(c, w, cell_bin_local(w, qx, qy), cs, ce);
-            int bestA = 0x7fffffff;
             for (int m = cs; m < ce; ++m) {
-                const int a = c.sorted[m];
                 int ax, ay;
-                slot_of(to_f2(c.A.pos[a]), w.W, w.H, ax, ay);
-                if (ax == qx && ay == qy && a < bestA) bestA = a;
+                slot_of(to_f2(c.mpos[m]), w.W, w.H, ax, ay);
+                if (ax == qx && ay == qy) {
+                    cell = m;
+                    break;
+                }
             }
-            if (bestA != 0x7fffffff) cell = cs + rank_in_bin(c.sorted, cs, ce, bestA);
         }
         c.absorbCell[k] = cell;
     });
@@ -148,7 +149,8 @@ __global__ void k_part_absorb(Ctx c) {
     });
 }
 
-void launch_particles(const Ctx& c, const LaunchShape& sh, hipStream_t s, Recorder* rec) {
+// movement, targets and fusion claims: the map and the particles only
+void launch_particles_pre(const Ctx& c, const LaunchShape& sh, hipStream_t s, Recorder* rec) {
     if (!sh.partBlocks || !sh.hasParticles) return;
     const dim3 g(sh.partGrid), b(sh.partThreads);
     if (rec) rec->mark("part_move");
@@ -157,6 +159,12 @@ void launch_particles(const Ctx& c, const LaunchShape& sh, hipStream_t s, Record
     hipLaunchKernelGGL(k_part_target, g, b, 0, s, c);
     if (rec) rec->mark("part_claim");
     hipLaunchKernelGGL(k_part_claim, g, b, 0, s, c);
+}
+
+// fusions, reflections and absorptions: also the cells' end-of-pass state
+void launch_particles_post(const Ctx& c, const LaunchShape& sh, hipStream_t s, Recorder* rec) {
+    if (!sh.partBlocks || !sh.hasParticles) return;
+    const dim3 g(sh.partGrid), b(sh.partThreads);
     if (rec) rec->mark("part_apply");
     hipLaunchKernelGGL(k_part_apply, g, b, 0, s, c);
     if (rec) rec->mark("part_absorb");

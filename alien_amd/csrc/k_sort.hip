@@ -2,10 +2,12 @@
 //
 // Replaces ALIEN's CellMap / ParticleMap (source/EngineGpuKernels/Map.cuh:71-280), which insert
 // pointers into a dense world-size grid with atomicCAS chains (cell map) or last-writer-wins
-// stores (particle map).  Here every bin of every world is one counter; the bins of the next step
-// are counted by the kernels that produce the end-of-step positions (bin_for_next_step), then one
-// exclusive scan, one scatter and, for particles, an in-bin sort give each bin its entries.  Cell
-// bins are ranked inside the neighbour-pass tiles instead (see k_neighbor.hip).
+// stores (particle map).  Here every bin of every world is one counter and one offset.  Cells:
+// the kernels that produce the end-of-step positions record each cell's next bin; at the start of
+// the next step the map count gives every bin its size from the same bin of the last map (the
+// cells that stay) plus the list of cells that moved in, one exclusive scan gives the offsets, and
+// the scatter places every cell at its rank among the bin's cells by previous index (R1), so the
+// map order is the logical order of the step.  Particles: counted with atomics, scattered, then sorted inside their bins.
 #include "kernels_common.hpp"
 #include "kernels.hpp"
 
@@ -17,13 +19,17 @@ __global__ void k_fill_u32(uint32_t* p, uint32_t v, size_t n) {
     for (; i < n; i += stride) p[i] = v;
 }
 
-// bins of the current state, used after data is set and after introspection
+// bins of the current state, used after data is set and after introspection: the current state
+// is in no map order yet, so every cell is a mover of the coming step
 __global__ void k_cell_bin(Ctx c) {
     int2 blk = c.cellBlocks[blockIdx.x];
     const DevWorld w = c.worlds[blk.x];
     int i = blk.y + int(threadIdx.x);
     if (i >= w.cellOff + w.numCells) return;
-    bin_for_next_step(c, w, i, to_f2(c.A.pos[i]), c.count);
+    const int gb = cell_bin_of(w, to_f2(c.A.pos[i]));
+    c.binOf[i] = int(uint32_t(gb) | kMoverBit);
+    push_mover(c.moverHead, c.moverNext, gb, i);
+    atomicAdd(&c.delta[gb], 1);
 }
 
 __global__ void k_part_bin(Ctx c) {
@@ -41,9 +47,10 @@ constexpr int kScanThreads = 256;
 constexpr int kScanItems = 8;
 constexpr int kScanTile = kScanThreads * kScanItems;
 
-// bin sizes base..base+7: shared count plus, for cell bins, the owner-counted part
-__device__ __forceinline__ void load8(const uint32_t* count, const uint32_t* priv, int nCell, size_t base, int n,
-                                      uint32_t v[8]) {
+// bin sizes base..base+7: a cell bin's size is its size in the last map plus its delta, a particle
+// bin's its count
+__device__ __forceinline__ void load8(const uint32_t* count, const uint32_t* startPrev, const int* delta, int nCell,
+                                      size_t base, int n, uint32_t v[8]) {
     if (base + 8 <= size_t(n)) {
         const uint4 a = *reinterpret_cast<const uint4*>(count + base);
         const uint4 b = *reinterpret_cast<const uint4*>(count + base + 4);
@@ -52,25 +59,38 @@ __device__ __forceinline__ void load8(const uint32_t* count, const uint32_t* pri
 #pragma unroll
         for (int k = 0; k < 8; ++k) v[k] = base + k < size_t(n) ? count[base + k] : 0u;
     }
+    if (base >= size_t(nCell)) return;
+    uint32_t sp[9];
+    int dl[8];
     if (base + 8 <= size_t(nCell)) {
-        const uint4 a = *reinterpret_cast<const uint4*>(priv + base);
-        const uint4 b = *reinterpret_cast<const uint4*>(priv + base + 4);
-        v[0] += a.x, v[1] += a.y, v[2] += a.z, v[3] += a.w, v[4] += b.x, v[5] += b.y, v[6] += b.z, v[7] += b.w;
+        const uint4 a = *reinterpret_cast<const uint4*>(startPrev + base);
+        const uint4 b = *reinterpret_cast<const uint4*>(startPrev + base + 4);
+        sp[0] = a.x, sp[1] = a.y, sp[2] = a.z, sp[3] = a.w, sp[4] = b.x, sp[5] = b.y, sp[6] = b.z, sp[7] = b.w;
+        sp[8] = startPrev[base + 8];
+        const int4 c0 = *reinterpret_cast<const int4*>(delta + base);
+        const int4 c1 = *reinterpret_cast<const int4*>(delta + base + 4);
+        dl[0] = c0.x, dl[1] = c0.y, dl[2] = c0.z, dl[3] = c0.w, dl[4] = c1.x, dl[5] = c1.y, dl[6] = c1.z, dl[7] = c1.w;
     } else {
+        // (the end of the last cell bin is the start of the particle bins)
 #pragma unroll
-        for (int k = 0; k < 8; ++k)
-            if (base + k < size_t(nCell)) v[k] += priv[base + k];
+        for (int k = 0; k < 9; ++k) sp[k] = base + k <= size_t(nCell) ? startPrev[base + k] : 0u;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) dl[k] = base + k < size_t(nCell) ? delta[base + k] : 0;
     }
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+        if (base + k < size_t(nCell)) v[k] = uint32_t(int(sp[k + 1] - sp[k]) + dl[k]);
 }
 
 // tile totals; also opens the step: clears the per-step list counters, and advances the worlds'
 // timesteps if `advance` is set (a step without bond work has no later reader of them)
-__global__ void __launch_bounds__(kScanThreads) k_scan_reduce(const uint32_t* count, const uint32_t* priv, int nCell,
-                                                              uint32_t* blockSums, int n, StepCounters* sc,
+__global__ void __launch_bounds__(kScanThreads) k_scan_reduce(const uint32_t* count, const uint32_t* startPrev,
+                                                              const int* delta, int nCell, uint32_t* blockSums, int n,
+                                                              StepCounters* sc,
                                                               DevCounters* advance, int numWorlds) {
     const size_t base = size_t(blockIdx.x) * kScanTile + size_t(threadIdx.x) * kScanItems;
     uint32_t v[8];
-    load8(count, priv, nCell, base, n, v);
+    load8(count, startPrev, delta, nCell, base, n, v);
     uint32_t s = v[0] + v[1] + v[2] + v[3] + v[4] + v[5] + v[6] + v[7];
     uint32_t total;
     block_exclusive_scan(s, total);
@@ -86,17 +106,35 @@ __global__ void __launch_bounds__(kScanThreads) k_scan_reduce(const uint32_t* co
 }
 
 // writes start[]: every tile adds the totals of the tiles before it (a few thousand words, read
-// from L2) to its own exclusive scan.  `count` stays: the scatters clear it (cells) or count it down (particles) to zero,
-// which leaves it cleared for the step after next (the two histograms alternate)
-__global__ void __launch_bounds__(kScanThreads) k_scan_down(const uint32_t* count, const uint32_t* priv, int nCell,
-                                                            const uint32_t* blockSums, uint32_t* start, int n) {
+// from L2) to its own exclusive scan.  `count` stays: the particle scatter counts its particle
+// bins down to zero, which leaves them cleared for the step after next (the two histograms
+// alternate); the map count writes the cell bins every step
+__global__ void __launch_bounds__(kScanThreads) k_scan_down(const uint32_t* count, const uint32_t* startPrev,
+                                                            const int* delta, int nCell, const uint32_t* blockSums,
+                                                            uint32_t* start, int n, int* deltaNext, int* moverHeadNext) {
     uint32_t before = 0;
     for (int t = int(threadIdx.x); t < int(blockIdx.x); t += kScanThreads) before += blockSums[t];
     uint32_t prefix;
     block_exclusive_scan(before, prefix);
     const size_t base = size_t(blockIdx.x) * kScanTile + size_t(threadIdx.x) * kScanItems;
     uint32_t v[8];
-    load8(count, priv, nCell, base, n, v);
+    load8(count, startPrev, delta, nCell, base, n, v);
+    // the next step's deltas and mover lists start empty (their last readers ran a step ago)
+    if (deltaNext) {
+        if (base + 8 <= size_t(nCell)) {
+            *reinterpret_cast<int4*>(deltaNext + base) = make_int4(0, 0, 0, 0);
+            *reinterpret_cast<int4*>(deltaNext + base + 4) = make_int4(0, 0, 0, 0);
+            *reinterpret_cast<int4*>(moverHeadNext + base) = make_int4(kNone, kNone, kNone, kNone);
+            *reinterpret_cast<int4*>(moverHeadNext + base + 4) = make_int4(kNone, kNone, kNone, kNone);
+        } else {
+#pragma unroll
+            for (int k = 0; k < 8; ++k)
+                if (base + k < size_t(nCell)) {
+                    deltaNext[base + k] = 0;
+                    moverHeadNext[base + k] = kNone;
+                }
+        }
+    }
     uint32_t s = v[0] + v[1] + v[2] + v[3] + v[4] + v[5] + v[6] + v[7];
     uint32_t total;
     uint32_t off = block_exclusive_scan(s, total) + prefix;
@@ -117,27 +155,50 @@ __global__ void __launch_bounds__(kScanThreads) k_scan_down(const uint32_t* coun
     }
 }
 
-// entries land in their bin (the in-bin order is fixed later by ranking), with a copy of the
-// state the neighbour pass reads: owner-counted cells at the rank they got when counted, the
-// shared ones after them at the place their count returned.  A shared cell also clears its bin's
-// shared counter (plain stores of zero), which leaves the histogram empty for the step after next.
+// Map scatter: cell i (A index, i.e. its place in the last map) goes to its bin at its rank by A
+// index among the bin's cells (R1).  The bin's stayers hold their places of the last map's bin in
+// order, so a stayer's rank among them is its place in that bin minus the movers that left the bin
+// before it (marked lanes of the wave, and cells of the bin in earlier waves), and a mover from a
+// lower place precedes them all, one from a higher place follows them all; every cell then adds
+// the bin's movers with a smaller A index.  With the cell goes a copy of the state the neighbour
+// pass reads.
 __global__ void k_cell_scatter(Ctx c) {
     int2 blk = c.cellBlocks[blockIdx.x];
     const DevWorld w = c.worlds[blk.x];
-    int i = blk.y + int(threadIdx.x);
-    if (i >= w.cellOff + w.numCells) return;
+    const int i = blk.y + int(threadIdx.x);
+    const int lane = int(threadIdx.x) & 63;
+    const int i0 = i - lane;  // the wave's first cell
+    const bool active = i < w.cellOff + w.numCells;
+    const int bo = active ? c.binOf[i] : 0;
+    const bool mover = (uint32_t(bo) & kMoverBit) != 0u;
+    const unsigned long long movers = __ballot(active && mover);
+    if (!active) return;
     const float2 p = c.A.pos[i], v = c.A.vel[i];
-    // the neighbour pass of a fluid world reads 1 / density
-    const float d = c.params[blk.x].motion == 0 ? 1.0f / c.A.dens[i] : c.A.dens[i];
+    const float dd = c.A.dens[i];
     const uint32_t f = c.A.flags[i];
-    const int gb = c.binOf[i];
-    const int arr = c.arrival[i];
-    int rank = arr;
-    if (arr & kArrivalShared) {
-        rank = int(c.priv[gb]) + (arr & ~kArrivalShared);
-        c.count[gb] = 0u;
+    const int gb = int(uint32_t(bo) & ~kMoverBit);
+    const uint32_t base = c.startPrev[w.binOff];
+    const int s = w.cellOff + int(c.startPrev[gb] - base), e = w.cellOff + int(c.startPrev[gb + 1] - base);
+    const int head = c.moverHead[gb];
+    const int dest0 = w.cellOff + int(c.start[gb] - c.start[w.binOff]);
+    int rank;
+    if (!mover) {
+        // a stayer: s <= i < e
+        const int from = max(s, i0) - i0;
+        const unsigned long long below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+        const unsigned long long fromMask = from >= 64 ? 0ull : ~((1ull << from) - 1ull);
+        int left = __builtin_popcountll(movers & below & fromMask);
+        for (int j = s; j < i0; ++j) left += (uint32_t(c.binOf[j]) & kMoverBit) ? 1 : 0;
+        rank = (i - s) - left;
+    } else {
+        rank = 0;
+        if (i >= e)
+            for (int j = s; j < e; ++j) rank += c.binOf[j] == gb ? 1 : 0;
     }
-    const int dest = w.cellOff + int(c.start[gb] - c.start[w.binOff]) + rank;
+    for (int j = head; j != kNone; j = c.moverNext[j]) rank += j < i ? 1 : 0;
+    const int dest = dest0 + rank;
+    // the neighbour pass of a fluid world reads 1 / density
+    const float d = c.params[blk.x].motion == 0 ? 1.0f / dd : dd;
     c.sorted[dest] = i;
     c.mpos[dest] = p;
     c.mvel[dest] = v;
@@ -237,18 +298,23 @@ void launch_fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t s) {
 }
 
 void launch_seed_bins(const Ctx& c, const LaunchShape& sh, hipStream_t s) {
-    // every cell is counted in the shared histogram; the owner-counted part starts empty
-    launch_fill_u32(c.priv, 0u, size_t(c.numCellBins), s);
+    // no last map (every bin of it empty), every cell a mover of the coming step
+    launch_fill_u32(const_cast<uint32_t*>(c.startPrev), 0u, size_t(c.numBins) + 1, s);
+    launch_fill_u32(reinterpret_cast<uint32_t*>(c.moverHead), uint32_t(kNone), size_t(c.numCellBins), s);
+    launch_fill_u32(reinterpret_cast<uint32_t*>(c.delta), 0u, size_t(c.numCellBins), s);
     if (sh.cellBlocks) hipLaunchKernelGGL(k_cell_bin, dim3(sh.cellBlocks), dim3(sh.cellThreads), 0, s, c);
     if (sh.partBlocks) hipLaunchKernelGGL(k_part_bin, dim3(sh.partGrid), dim3(sh.partThreads), 0, s, c);
 }
 
-void launch_scan(const Ctx& c, hipStream_t s, StepCounters* sc, bool advanceTimestep) {
+// exclusive scan of the bin sizes into start (c.start); with `resetNext` it also empties the
+// next step's deltas and mover lists
+void launch_scan(const Ctx& c, hipStream_t s, StepCounters* sc, bool advanceTimestep, bool resetNext) {
     const int tiles = (c.numBins + kScanTile - 1) / kScanTile;
-    hipLaunchKernelGGL(k_scan_reduce, dim3(tiles), dim3(kScanThreads), 0, s, c.count, c.priv, c.numCellBins,
-                       c.blockSums, c.numBins, sc, advanceTimestep ? c.ctr : nullptr, c.numWorlds);
-    hipLaunchKernelGGL(k_scan_down, dim3(tiles), dim3(kScanThreads), 0, s, c.count, c.priv, c.numCellBins,
-                       c.blockSums, c.start, c.numBins);
+    hipLaunchKernelGGL(k_scan_reduce, dim3(tiles), dim3(kScanThreads), 0, s, c.count, c.startPrev, c.delta,
+                       c.numCellBins, c.blockSums, c.numBins, sc, advanceTimestep ? c.ctr : nullptr, c.numWorlds);
+    hipLaunchKernelGGL(k_scan_down, dim3(tiles), dim3(kScanThreads), 0, s, c.count, c.startPrev, c.delta,
+                       c.numCellBins, c.blockSums, c.start, c.numBins, resetNext ? c.deltaNext : nullptr,
+                       c.moverHeadNext);
 }
 
 void launch_cell_scatter(const Ctx& c, const LaunchShape& sh, hipStream_t s) {

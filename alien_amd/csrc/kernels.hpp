@@ -39,7 +39,16 @@ void launch_fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t s);
 void launch_seed_bins(const Ctx& c, const LaunchShape& sh, hipStream_t s);
 // neighbour-pass workgroups one CU keeps resident (fluid or collision variant)
 int neighbor_blocks_per_cu(bool fluid);
-void launch_step(const Ctx& c, const LaunchShape& sh, hipStream_t s, Recorder* rec = nullptr);
+// A second stream of the same priority and two events: the particle kernels that read only the
+// map and the particles (sort, movement, targets, fusion claims) run on it beside the neighbour
+// pass, joined before the particle kernels that read the cells' end-of-pass state.
+struct SideStream {
+    hipStream_t stream = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
+};
+// rec != nullptr (per-kernel timing) launches everything on s, one kernel after another
+void launch_step(const Ctx& c, const LaunchShape& sh, hipStream_t s, Recorder* rec = nullptr,
+                 const SideStream* side = nullptr);
 // cell map of the current state in logical order (sorted/newidx), for introspection; consumes
 // the seeded bins and seeds them again afterwards
 void launch_debug_map(const Ctx& c, const LaunchShape& sh, hipStream_t s);

@@ -20,8 +20,12 @@ __device__ __forceinline__ int div_small(int a, int b) {
     while (q > 0 && q * b > a) --q;
     return q;
 }
+// floor(s / binSize) for a slot coordinate (a shift for power-of-two bins)
+__device__ __forceinline__ int bin_coord(const DevWorld& w, int s) {
+    return w.binShift >= 0 ? s >> w.binShift : div_small(s, w.binSize);
+}
 __device__ __forceinline__ int cell_bin_local(const DevWorld& w, int sx, int sy) {
-    int bx = min(div_small(sx, w.binSize), w.nbx - 1), by = min(div_small(sy, w.binSize), w.nby - 1);
+    int bx = min(bin_coord(w, sx), w.nbx - 1), by = min(bin_coord(w, sy), w.nby - 1);
     return by * w.nbx + bx;
 }
 __device__ __forceinline__ int part_bin_local(const DevWorld& w, int sx, int sy) {
@@ -50,48 +54,42 @@ __device__ __forceinline__ int rank_in_bin(const int* sorted, int s, int e, int 
     return r;
 }
 
-// counting-sort bookkeeping: a cell counted in the shared (atomic) histogram records its place
-// among the bin's shared arrivals (the count before its own) with this mark
-constexpr int kArrivalShared = 0x40000000;
-
-// cell bin of a position (global bin index)
-__device__ __forceinline__ int cell_bin_of(const DevWorld& w, f2 p) {
+// cell bin of a position (global bin index), and its bin coordinates
+__device__ __forceinline__ void cell_bin_xy(const DevWorld& w, f2 p, int& bx, int& by) {
     int sx, sy;
     slot_of(p, w.W, w.H, sx, sy);
-    return w.binOff + cell_bin_local(w, sx, sy);
+    bx = min(bin_coord(w, sx), w.nbx - 1);
+    by = min(bin_coord(w, sy), w.nby - 1);
+}
+__device__ __forceinline__ int cell_bin_of(const DevWorld& w, f2 p) {
+    int bx, by;
+    cell_bin_xy(w, p, bx, by);
+    return w.binOff + by * w.nbx + bx;
 }
 
-// counting-sort bookkeeping for the next step: bin of the end-of-step position, counted into the
-// shared histogram `hist`, and the cell's place among the bin's shared arrivals
-__device__ __forceinline__ void bin_for_next_step(const Ctx& c, const DevWorld& w, int k, f2 p, uint32_t* hist) {
-    const int gb = cell_bin_of(w, p);
-    c.binOf[k] = gb;
-    c.arrival[k] = kArrivalShared | int(atomicAdd(&hist[gb], 1u));
-}
-// bin_for_next_step for a whole wave (every lane calls it; lanes with !valid count nothing): one
-// atomic per run of consecutive lanes whose cells land in the same bin (cells in map order mostly
-// stay together), each cell ranked inside its run's share of the bin's shared arrivals
-__device__ __forceinline__ void bin_for_next_step_runs(const Ctx& c, const DevWorld& w, int k, f2 p, uint32_t* hist,
-                                                       bool valid) {
-    const int lane = int(threadIdx.x) & 63;
-    const int gb = valid ? cell_bin_of(w, p) : -1;
-    const int prev = __shfl_up(gb, 1, 64);
-    const bool start = valid && (lane == 0 || prev != gb);
-    const unsigned long long starts = __ballot(start);
-    const unsigned long long bounds = starts | __ballot(!valid);  // a run ends at a start or an invalid lane
-    const unsigned long long upto = lane == 63 ? ~0ull : (2ull << lane) - 1ull;  // lanes 0..lane
-    const unsigned long long sm = starts & upto;
-    const int rs = sm ? 63 - __clzll(sm) : lane;  // the run's first lane
-    const unsigned long long after = bounds & ~upto;
-    const int re = after ? __ffsll(after) - 1 : 64;  // one past the run's last lane
-    int base = 0;
-    if (start) base = int(atomicAdd(&hist[gb], uint32_t(re - lane)));
-    base = __shfl(base, rs, 64);
-    if (valid) {
+// The map of a step orders cells by bin, and inside a bin by their index in the last step's order
+// (R1).  The kernels that produce a cell's end-of-step position record its next bin (binOf).  A
+// cell that stays in its bin keeps its place among the bin's cells of the last map; a cell that
+// changes bins (a "mover"; after seeding every cell) is marked (kMoverBit), counted out of its old
+// bin and into its new one (delta), and entered in the list of its new bin: head[bin] is the last
+// one entered, next[i] the one before (kNone ends a list; ranks count the list's smaller indices,
+// so its order does not matter).
+__device__ __forceinline__ void push_mover(int* head, int* next, int gb, int k) { next[k] = atomicExch(&head[gb], k); }
+
+// next-step bin of the cell at end-of-step index k: end-of-step position pNew; gbOld, its bin in
+// this step's map (the bin of its start-of-step position)
+__device__ __forceinline__ void bin_next(const Ctx& c, const DevWorld& w, int k, f2 pNew, int gbOld) {
+    const int gb = cell_bin_of(w, pNew);
+    if (gb == gbOld) {
         c.binOf[k] = gb;
-        c.arrival[k] = kArrivalShared | (base + lane - rs);
+        return;
     }
+    c.binOf[k] = int(uint32_t(gb) | kMoverBit);
+    push_mover(c.moverHeadNext, c.moverNext, gb, k);
+    atomicAdd(&c.deltaNext[gb], 1);
+    atomicSub(&c.deltaNext[gbOld], 1);
 }
+
 __device__ __forceinline__ void pbin_for_next_step(const Ctx& c, const DevWorld& w, int k, f2 p, uint32_t* hist) {
     int sx, sy;
     slot_of(p, w.W, w.H, sx, sy);

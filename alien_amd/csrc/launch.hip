@@ -11,34 +11,47 @@
 
 namespace alien {
 
-void launch_scan(const Ctx& c, hipStream_t s, StepCounters* sc, bool advanceTimestep);
+void launch_scan(const Ctx& c, hipStream_t s, StepCounters* sc, bool advanceTimestep, bool resetNext);
 void launch_cell_scatter(const Ctx& c, const LaunchShape& sh, hipStream_t s);
 void launch_part_sort(const Ctx& c, const LaunchShape& sh, hipStream_t s);
 void launch_part_scatter(const Ctx& c, const LaunchShape& sh, hipStream_t s);
 void launch_cell_binsort(const Ctx& c, const LaunchShape& sh, hipStream_t s);
 void launch_neighbor(const Ctx& c, const LaunchShape& sh, hipStream_t s, bool fluid, bool collision);
-void launch_particles(const Ctx& c, const LaunchShape& sh, hipStream_t s, Recorder* rec);
+void launch_particles_pre(const Ctx& c, const LaunchShape& sh, hipStream_t s, Recorder* rec);
+void launch_particles_post(const Ctx& c, const LaunchShape& sh, hipStream_t s, Recorder* rec);
 void launch_bonds(const Ctx& c, const LaunchShape& sh, hipStream_t s, Recorder* rec);
 
-void launch_step(const Ctx& c, const LaunchShape& sh, hipStream_t s, Recorder* rec) {
+void launch_step(const Ctx& c, const LaunchShape& sh, hipStream_t s, Recorder* rec, const SideStream* side) {
     if (rec) rec->mark("scan");
     // without bond work the end-of-step kernel would only advance the timesteps: the scan does it
-    launch_scan(c, s, c.sc, !sh.hasBondWork);
+    launch_scan(c, s, c.sc, !sh.hasBondWork, true);
     if (rec) rec->mark("cell_scatter");
     launch_cell_scatter(c, sh, s);
+    // the particle kernels before the fusions and absorptions read the map and the particles
+    // only: with a side stream they run beside the neighbour pass
+    const bool fork = side && !rec && sh.hasParticles && sh.partBlocks;
+    hipStream_t ps = s;
+    if (fork) {
+        (void)hipEventRecord(side->fork, s);
+        (void)hipStreamWaitEvent(side->stream, side->fork, 0);
+        ps = side->stream;
+    }
     if (rec && sh.hasParticles) rec->mark("part_sort");
-    launch_part_sort(c, sh, s);
+    launch_part_sort(c, sh, ps);
+    launch_particles_pre(c, sh, ps, rec);
+    if (fork) (void)hipEventRecord(side->join, ps);
     if (rec) rec->mark("neighbor");
     launch_neighbor(c, sh, s, sh.hasFluid, sh.hasCollision);
-    launch_particles(c, sh, s, rec);
+    if (fork) (void)hipStreamWaitEvent(s, side->join, 0);
+    launch_particles_post(c, sh, s, rec);
     launch_bonds(c, sh, s, rec);
     if (rec) rec->mark(nullptr);
 }
 
-// The scatters leave the histogram at zero; seeding the bins again restores it for the next
-// step.
+// The map of the current state: consumes the seeded bins and mover lists, which seeding restores
+// for the next step.
 void launch_debug_map(const Ctx& c, const LaunchShape& sh, hipStream_t s) {
-    launch_scan(c, s, nullptr, false);
+    launch_scan(c, s, nullptr, false, true);
     launch_cell_scatter(c, sh, s);
     launch_part_scatter(c, sh, s);
     launch_cell_binsort(c, sh, s);
@@ -243,6 +256,5 @@ hipError_t launch_statistics_noalloc(const Ctx& c, int wi, int cellBlocks, int b
     if (e != hipSuccess) return e;
     return hipStreamSynchronize(s);
 }
-
 
 }  // namespace alien

@@ -360,13 +360,24 @@ private:
     size_t used_ = 0;
 };
 
-WorldSet::WorldSet() { check_hip(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate"); }
+// both streams at the default priority
+WorldSet::WorldSet() {
+    check_hip(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
+    check_hip(hipStreamCreateWithFlags(&side_.stream, hipStreamNonBlocking), "hipStreamCreate");
+    check_hip(hipEventCreateWithFlags(&side_.fork, hipEventDisableTiming), "hipEventCreate");
+    check_hip(hipEventCreateWithFlags(&side_.join, hipEventDisableTiming), "hipEventCreate");
+}
 
 WorldSet::~WorldSet() {
     invalidateGraphs();
+    if (stream_) (void)hipStreamSynchronize(stream_);
+    if (side_.stream) (void)hipStreamSynchronize(side_.stream);
     freeAll();
     for (auto& a : tableAllocs_) (void)hipFree(a.p);
     tableAllocs_.clear();
+    if (side_.fork) (void)hipEventDestroy(side_.fork);
+    if (side_.join) (void)hipEventDestroy(side_.join);
+    if (side_.stream) (void)hipStreamDestroy(side_.stream);
     if (stream_) (void)hipStreamDestroy(stream_);
 }
 
@@ -402,7 +413,9 @@ uint64_t WorldSet::estimateBytes(uint64_t cells, uint64_t particles, int W, int 
     const uint64_t perParticle = 2 * (8 + 8 + 4 + 4 + 4 + 1) + 6 * 4;
     const uint64_t bins = uint64_t(std::max(1, W / binSize)) * uint64_t(std::max(1, H / binSize)) +
                           uint64_t(std::max(1, W / kParticleBin)) * uint64_t(std::max(1, H / kParticleBin));
-    return cells * perCell + particles * perParticle + bins * 16 + (1u << 20);
+    // per bin: two histograms, two offset arrays, two mover-list heads and deltas (cell bins), scan
+    // partials
+    return cells * perCell + particles * perParticle + bins * 36 + (1u << 20);
 }
 
 void WorldSet::clearAll() {
@@ -484,6 +497,9 @@ void WorldSet::build(const std::vector<Source>& sources) {
         w.W = h.desc.worldSizeX;
         w.H = h.desc.worldSizeY;
         w.binSize = bin_size_for(h.params);
+        w.binShift = -1;
+        for (int sh = 0; sh < 24; ++sh)
+            if ((1 << sh) == w.binSize) w.binShift = sh;
         w.nbx = w.W / w.binSize;
         w.nby = w.H / w.binSize;
         w.pnbx = std::max(1, w.W / kParticleBin);
@@ -565,10 +581,12 @@ void WorldSet::build(const std::vector<Source>& sources) {
     count_[1] = static_cast<uint32_t*>(A(sizeof(uint32_t) * NB));
     numCellBins_ = cellBins;
     const size_t NCB = size_t(std::max(cellBins, 1));
-    priv_[0] = static_cast<uint32_t*>(A(sizeof(uint32_t) * NCB));
-    priv_[1] = static_cast<uint32_t*>(A(sizeof(uint32_t) * NCB));
-    arrival_ = static_cast<int*>(A(sizeof(int) * NC));
-    start_ = static_cast<uint32_t*>(A(sizeof(uint32_t) * (size_t(NB) + 1)));
+    for (int b = 0; b < 2; ++b) {
+        start_[b] = static_cast<uint32_t*>(A(sizeof(uint32_t) * (size_t(NB) + 1)));
+        moverHead_[b] = static_cast<int*>(A(sizeof(int) * NCB));
+        delta_[b] = static_cast<int*>(A(sizeof(int) * NCB));
+    }
+    moverNext_ = static_cast<int*>(A(sizeof(int) * NC));
     blockSums_ = static_cast<uint32_t*>(A(sizeof(uint32_t) * (size_t(NB) / 2048 + 2)));
     sorted_ = static_cast<int*>(A(sizeof(int) * NC));
     mpos_ = static_cast<float2*>(A(sizeof(float2) * NC));
@@ -696,8 +714,11 @@ void WorldSet::build(const std::vector<Source>& sources) {
     check_hip(hipMemcpy(dSc_, &sc, sizeof(sc), hipMemcpyHostToDevice), "H2D");
     launch_fill_u32(count_[0], 0u, size_t(NB), stream_);
     launch_fill_u32(count_[1], 0u, size_t(NB), stream_);
-    launch_fill_u32(priv_[0], 0u, NCB, stream_);
-    launch_fill_u32(priv_[1], 0u, NCB, stream_);
+    for (int b = 0; b < 2; ++b) {
+        launch_fill_u32(start_[b], 0u, size_t(NB) + 1, stream_);
+        launch_fill_u32(reinterpret_cast<uint32_t*>(moverHead_[b]), uint32_t(kNone), NCB, stream_);
+        launch_fill_u32(reinterpret_cast<uint32_t*>(delta_[b]), 0u, NCB, stream_);
+    }
     launch_fill_u32(reinterpret_cast<uint32_t*>(fclaim_), uint32_t(kClaimFree), size_t(NP), stream_);
     launch_fill_u32(reinterpret_cast<uint32_t*>(cclaim_), uint32_t(kClaimFree), size_t(NC), stream_);
     launch_fill_u32(reinterpret_cast<uint32_t*>(claim_), 0xffffffffu, size_t(NC) * 2, stream_);
@@ -959,9 +980,13 @@ Ctx WorldSet::makeCtx(int parity) const {
     c.partCap = partCap_;
     c.numBins = numBins_;
     c.numCellBins = numCellBins_;
-    c.priv = priv_[parity];
-    c.privNext = priv_[parity ^ 1];
-    c.arrival = arrival_;
+    c.start = start_[parity];
+    c.startPrev = start_[parity ^ 1];
+    c.moverHead = moverHead_[parity];
+    c.moverHeadNext = moverHead_[parity ^ 1];
+    c.moverNext = moverNext_;
+    c.delta = delta_[parity];
+    c.deltaNext = delta_[parity ^ 1];
     c.worlds = dWorlds_;
     c.params = dParams_;
     c.ctr = dCtr_;
@@ -981,7 +1006,6 @@ Ctx WorldSet::makeCtx(int parity) const {
     c.pbinOf = pbinOf_;
     c.count = count_[parity];
     c.countNext = count_[parity ^ 1];
-    c.start = start_;
     c.blockSums = blockSums_;
     c.sorted = sorted_;
     c.mpos = mpos_;
@@ -1114,14 +1138,14 @@ void WorldSet::step(uint64_t n) {
             if (!g) {
                 hipGraph_t graph = nullptr;
                 check_hip(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
-                launch_step(makeCtx(parity_), shape_, stream_);
+                launch_step(makeCtx(parity_), shape_, stream_, nullptr, &side_);
                 check_hip(hipStreamEndCapture(stream_, &graph), "hipStreamEndCapture");
                 check_hip(hipGraphInstantiate(&g, graph, nullptr, nullptr, 0), "hipGraphInstantiate");
                 (void)hipGraphDestroy(graph);
             }
             check_hip(hipGraphLaunch(g, stream_), "hipGraphLaunch");
         } else {
-            launch_step(makeCtx(parity_), shape_, stream_);
+            launch_step(makeCtx(parity_), shape_, stream_, nullptr, &side_);
             check_hip(hipGetLastError(), "kernel launch");
         }
         parity_ ^= 1;
@@ -1323,7 +1347,7 @@ void WorldSet::cellMap(int wi, std::vector<int>& bins, std::vector<int>& slots) 
     bins.resize(n);
     slots.resize(n);
     for (int k = 0; k < n; ++k) {
-        bins[k] = binOfAll[sorted[k]] - w.binOff;
+        bins[k] = int(uint32_t(binOfAll[sorted[k]]) & ~kMoverBit) - w.binOff;
         slots[k] = slotAll[sorted[k]];
     }
 }

@@ -41,7 +41,7 @@ DevParams to_dev_params(const alien_params& p);
 int choose_strip(const std::vector<float2>& pos, int W, int H, int binSize, bool twoRowStages, int& rowsPerStage);
 void strip_shape(DevWorld& w, int strip, int rowsPerStage);
 void chunk_shape(std::vector<DevWorld>& worlds, const std::vector<DevParams>& params, int forcedRows = 0);
-constexpr int kXcds = 8;  // accelerator complex dies of an MI355X
+constexpr int kXcds = kXcdCount;  // accelerator complex dies of an MI355X
 
 // host-side identity and pass-through data of one world (fields the physics path never touches).
 // age, activationTime and livingState hold the values at the last set (stepsSinceSet steps ago):
@@ -173,6 +173,7 @@ private:
     void dropParticlesIfNone();  // stops launching the particle kernels once no particle is left
 
     hipStream_t stream_ = nullptr;
+    SideStream side_;  // particle kernels beside the neighbour pass
     std::vector<HostWorld> hosts_;
     std::vector<DevWorld> worlds_;
     std::vector<DevParams> params_;
@@ -201,10 +202,14 @@ private:
     std::vector<uint8_t> livingWork_;  // per world
     int *binOf_ = nullptr, *pbinOf_ = nullptr;
     uint32_t* count_[2] = {nullptr, nullptr};  // bin histograms, alternating with the state buffers
-    uint32_t* priv_[2] = {nullptr, nullptr};   // owner-counted part of the cell-bin histograms
-    int* arrival_ = nullptr;
+    // map build: offsets of the map of the step of that parity and the mover lists (per cell bin)
+    // read in it, alternating with the state buffers
+    uint32_t* start_[2] = {nullptr, nullptr};
+    int* moverHead_[2] = {nullptr, nullptr};
+    int* delta_[2] = {nullptr, nullptr};
+    int* moverNext_ = nullptr;
     int numCellBins_ = 0;
-    uint32_t *start_ = nullptr, *blockSums_ = nullptr;
+    uint32_t* blockSums_ = nullptr;
     int *sorted_ = nullptr, *newidx_ = nullptr, *psorted_ = nullptr;
     float2 *mpos_ = nullptr, *mvel_ = nullptr;
     float* mdens_ = nullptr;

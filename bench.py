@@ -27,6 +27,7 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E peak
 # algorithmic (compulsory) bytes per unit and step: fluid cell 56 B, energy particle 52 B
 BYTES_PER_FLUID_CELL = 56
 BYTES_PER_PARTICLE = 52
+NEIGHBOR_KERNEL = "k_neighbor_cells"  # the neighbour pass of fluid worlds (the dominant kernel)
 
 
 def parse():
@@ -309,10 +310,13 @@ def main():
             tpath = os.path.join(ROOT, "profiles", "traffic_neighbor_large_fluid.json")
             if os.path.exists(tpath):
                 with open(tpath) as f:
-                    traffic = json.load(f).get("bytes_per_launch")
+                    rec = json.load(f)
+                # (counter readings of the same kernel only)
+                if rec.get("kernel") == NEIGHBOR_KERNEL:
+                    traffic = rec.get("bytes_per_launch")
             roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                         "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                        "kernel": "k_neighbor_rows<true>", "avg_launch_us": avg_s * 1e6,
+                        "kernel": NEIGHBOR_KERNEL, "avg_launch_us": avg_s * 1e6,
                         "bytes_per_launch": per_launch}
 
     step_bytes = BYTES_PER_FLUID_CELL * n + BYTES_PER_PARTICLE * args.particles

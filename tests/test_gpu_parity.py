@@ -81,14 +81,16 @@ def test_mix_dense_fusion_bit_exact():
 
 @pytest.mark.parametrize("motion", [MOTION_FLUID, MOTION_COLLISION])
 def test_crowded_bins_take_the_fallback_paths_bit_exact(motion):
-    """A staged row above 128 entries and three consecutive rows above 256 (whole chunks walk
-    their stencils in global memory) and cells with more than 64 candidates (one cell walks its
-    stencil alone): same results as the oracle."""
+    """Cells with more than 64 candidates walk their stencils alone (both passes), and in the
+    row-staged collision pass a staged row above 128 entries and three consecutive rows above 256
+    make whole chunks walk their stencils in global memory: same results as the oracle."""
     w = worlds.crowded_bins(11 + motion, motion=motion)
     say(f"crowded bins motion={motion} cells={len(w.cells)}")
     sim, _ = run_both(w, [1, 2, 4])
     chunks, cells = sim.path_counts()
-    assert chunks > 0 and cells > 0, (chunks, cells)
+    assert cells > 0, (chunks, cells)
+    if motion == MOTION_COLLISION:
+        assert chunks > 0, (chunks, cells)
 
 
 def test_plain_soft_body_bit_exact():
@@ -415,11 +417,12 @@ def test_clear():
 def test_invalid_parameters_are_rejected():
     w = worlds.random_mix(22)
     sim = engine.Simulation.from_world(w)
-    p = w.params.copy()
-    p.innerFriction = 0.3
-    with pytest.raises(engine.AlienError) as e:
-        sim.set_params(p)
-    assert e.value.code == abi.E_ARG
+    for field, value in (("innerFriction", 1.5), ("cellMaxForceDecayProb", -0.1), ("timestepSize", 0.0)):
+        p = w.params.copy()
+        setattr(p, field, value)
+        with pytest.raises(engine.AlienError) as e:
+            sim.set_params(p)
+        assert e.value.code == abi.E_ARG, field
     sim.step(1)  # still usable
 
 
