@@ -644,12 +644,15 @@ struct alien_server {
                 }
             }
             std::vector<uint64_t> admitted;
-            uint64_t used = 0, cells = 0, parts = 0;
+            uint64_t kept = 0, cells = 0, parts = 0;
             for (auto id : keep) {
-                used += jobs.at(id)->bytes;
+                kept += jobs.at(id)->bytes;
                 cells += jobs.at(id)->numCells;
                 parts += jobs.at(id)->numParticles;
             }
+            // the old set keeps its memory until the new one is built (the budget bounds the peak)
+            const uint64_t oldSet = set ? uint64_t(set->deviceBytes()) : 0;
+            uint64_t used = 0;
             while (!queue.empty()) {
                 uint64_t id = queue.front();
                 auto it = jobs.find(id);
@@ -660,7 +663,8 @@ struct alien_server {
                 }
                 Job& j = *it->second;
                 // the budget, and the size limits of one world set (alien::kMaxSetCells)
-                if (used + j.bytes > budget || cells + j.numCells > uint64_t(alien::kMaxSetCells) ||
+                if (alien_server_plan_admission(budget, oldSet, kept + used, &j.bytes, 1) == 0 ||
+                    cells + j.numCells > uint64_t(alien::kMaxSetCells) ||
                     parts + j.numParticles > uint64_t(alien::kMaxSetParticles))
                     break;
                 used += j.bytes;
@@ -688,7 +692,7 @@ struct alien_server {
             // erases running jobs)
             std::vector<uint64_t> oldRunning = running;
             lk.unlock();
-            std::string err;
+            std::string err, buildErr;
             try {
                 if (changed) {
                     // rebuild: the live worlds (device copies) and the admitted ones, finished
@@ -706,15 +710,43 @@ struct alien_server {
                         }
                         src.push_back(s);
                     }
-                    fresh->build(src);
-                    set = std::move(fresh);
+                    try {
+                        fresh->build(src);
+                    } catch (const std::exception& e) {
+                        buildErr = e.what();
+                    }
+                    if (buildErr.empty()) set = std::move(fresh);
                 }
-                if (chunkSteps > 0) set->step(chunkSteps);
+                if (buildErr.empty() && chunkSteps > 0) set->step(chunkSteps);
             } catch (const std::exception& e) {
                 err = e.what();
             }
             lk.lock();
             const double t = nowSeconds();
+            if (!buildErr.empty()) {
+                // the old set is untouched: only the jobs being admitted fail; the resident ones go
+                // on with the old set (cancelled ones were dropped from it above)
+                for (auto id : admitted) {
+                    Job& j = *jobs.at(id);
+                    j.state = ALIEN_JOB_FAILED;
+                    j.error = buildErr;
+                    j.image.reset();
+                    j.finishTime = t;
+                }
+                for (auto& id : oldRunning)
+                    if (id && !jobs.count(id)) id = 0;  // cancelled and erased: inactive in place
+                running = oldRunning;
+                for (size_t i = 0; i < running.size(); ++i)
+                    if (!running[i] && set && i < size_t(set->numWorlds())) set->setActive(int(i), false);
+                bool anyLive = false;
+                for (auto id : running) anyLive = anyLive || id != 0;
+                if (!anyLive) {
+                    running.clear();
+                    set.reset();
+                }
+                doneCv.notify_all();
+                continue;
+            }
             for (auto id : admitted) {
                 Job& j = *jobs.at(id);
                 j.state = ALIEN_JOB_RUNNING;
@@ -797,6 +829,16 @@ extern "C" {
 
 uint64_t alien_server_estimate_bytes(uint64_t numCells, uint64_t numParticles, int32_t worldSizeX, int32_t worldSizeY) {
     return WorldSet::estimateBytes(numCells, numParticles, worldSizeX, worldSizeY, 2);
+}
+
+uint64_t alien_server_plan_admission(uint64_t budgetBytes, uint64_t oldSetBytes, uint64_t keptBytes,
+                                     const uint64_t* queuedBytes, uint64_t numQueued) {
+    uint64_t need = oldSetBytes + keptBytes, n = 0;
+    for (; n < numQueued && queuedBytes; ++n) {
+        if (need + queuedBytes[n] > budgetBytes || need + queuedBytes[n] < need) break;
+        need += queuedBytes[n];
+    }
+    return n;
 }
 
 int alien_server_create(uint64_t budgetBytes, uint32_t chunkSteps, alien_server** out) {

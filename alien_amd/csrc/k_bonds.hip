@@ -248,8 +248,8 @@ __global__ void k_conn_verlet_vel(Ctx c) {
     if (!st) return;
     const f2 p2 = torus_pos(add(p, sh), w.W, w.H);
     c.B.pos[x] = to_float2(p2);
-    // (the map copy holds the start-of-step position at the cell's end-of-step index)
-    bin_next(c, w, x, p2, cell_bin_of(w, to_f2(c.mpos[x])));
+    // (the map gives the cell's start-of-step index at its end-of-step index)
+    bin_next(c, w, x, p2, cell_bin_of(w, to_f2(c.A.pos[c.sorted[x]])));
     if (flags_barrier(fx)) return;
     const f2 acc = mul(add(s1, s2), 0.5f);
     f2 v = add(v0, mul(acc, dt));

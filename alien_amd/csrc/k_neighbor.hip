@@ -978,9 +978,10 @@ k_neighbor_rows(Ctx c) {
 
 // ---------------------------------------------------------------------------------------------
 // Cell-parallel fluid pass.  The map is in logical order (cells by bin, inside a bin by previous
-// index), so the three stencil bins of a cell in one bin row are one contiguous range of the map
-// copy, and a wave of 64 consecutive map positions reads its candidates straight from it (the
-// ranges of neighbouring lanes overlap and stay in the caches).  Per wave: every lane tests its
+// index), so the three stencil bins of a cell in one bin row are one contiguous range of the map,
+// and a wave of 64 consecutive map positions reads its candidates through it (the ranges of
+// neighbouring lanes overlap and stay in the caches; the start-of-step state is nearly in map
+// order, so the reads through the map are nearly sequential).  Per wave: every lane tests its
 // candidates into a mask in visiting order; the wave's neighbour pairs are listed back to back in
 // LDS and evaluated one pair per lane; every lane then adds its own pairs' terms in visiting order
 // (the float sums of a sequential stencil walk, bit for bit) and finishes its cell.  A cell in the
@@ -1005,7 +1006,7 @@ struct CellShared {
     int aidx[64];
     union {
         // while candidates are tested: positions of the union of the lanes' ranges, per stencil row
-        float2 cpos[3][kCellStage];
+        float2 cpos[3 * kCellStage];
         struct {
             // pair records: neighbour map position, owner lane | bonded << 7
             int rnb[kCellRec];
@@ -1016,25 +1017,14 @@ struct CellShared {
     } u;
 };
 
-__device__ __forceinline__ int wave_min_int(int v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
-    return v;
-}
-__device__ __forceinline__ int wave_max_int(int v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
-    return v;
-}
-
-// the map-copy entry at map position m as a neighbour
+// the cell at map position m (fluid: 1 / density) as a neighbour
 __device__ __forceinline__ Nb map_nb(const Ctx& c, int m) {
     Nb o;
-    o.pos = to_f2(c.mpos[m]);
-    o.vel = to_f2(c.mvel[m]);
-    o.dens = c.mdens[m];
-    o.flags = c.mflags[m];
     o.aidx = c.sorted[m];
+    o.pos = to_f2(c.A.pos[o.aidx]);
+    o.vel = to_f2(c.A.vel[o.aidx]);
+    o.dens = 1.0f / c.A.dens[o.aidx];
+    o.flags = c.A.flags[o.aidx];
     o.lidx = m;
     return o;
 }
@@ -1101,45 +1091,65 @@ __global__ void __launch_bounds__(64 * kCellWaves) k_neighbor_cells(Ctx c, int t
     // The union of the lanes' ranges of each stencil row is staged in LDS (all its loads in flight
     // together) when it fits; a wave whose cells span two bin rows tests from global memory.
     const bool tests = active && !direct;
-    int u0[3], un[3];  // the staged range of each stencil row: [u0, u0 + un)
-    bool staged = __any(tests);
-#pragma unroll
-    for (int r = 0; r < 3; ++r) {
-        u0[r] = wave_min_int(tests ? lo[r] : 0x7fffffff);
-        const int u1 = wave_max_int(tests ? lo[r] + ns[r] : 0);
-        un[r] = staged ? u1 - u0[r] : 0;
-        staged = staged && un[r] <= kCellStage;
-    }
+    // The staged range of each stencil row, [u0, u0 + un): from the first testing lane's range start
+    // to the last one's end (in one bin row both grow with the lane), used when every testing
+    // lane's range lies inside (a wave across two bin rows fails that).
+    int u0[3], un[3];
+    const unsigned long long testers = __ballot(tests);
+    bool staged = testers != 0ull;
     if (staged) {
+        const int first = __builtin_ctzll(testers), last = 63 - __builtin_clzll(testers);
+        bool inside = true;
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+            u0[r] = __builtin_amdgcn_readlane(lo[r], first);
+            un[r] = __builtin_amdgcn_readlane(lo[r] + ns[r], last) - u0[r];
+            inside = inside && (!tests || (lo[r] >= u0[r] && lo[r] + ns[r] <= u0[r] + un[r]));
+            staged = staged && un[r] >= 0 && un[r] <= kCellStage;
+        }
+        staged = staged && __all(inside);
+    }
+    if (!staged) u0[0] = u0[1] = u0[2] = un[0] = un[1] = un[2] = 0;
+    if (staged) {
+        int a[3][3];
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+            for (int q = 0; q < 3; ++q) a[r][q] = lane + 64 * q < un[r] ? c.sorted[u0[r] + lane + 64 * q] : -1;
         float2 v[3][3];
 #pragma unroll
         for (int r = 0; r < 3; ++r)
 #pragma unroll
-            for (int q = 0; q < 3; ++q)
-                v[r][q] = lane + 64 * q < un[r] ? c.mpos[u0[r] + lane + 64 * q] : float2{0.0f, 0.0f};
+            for (int q = 0; q < 3; ++q) v[r][q] = a[r][q] >= 0 ? c.A.pos[a[r][q]] : float2{0.0f, 0.0f};
 #pragma unroll
         for (int r = 0; r < 3; ++r)
 #pragma unroll
-            for (int q = 0; q < 3; ++q) S.u.cpos[r][lane + 64 * q] = v[r][q];
+            for (int q = 0; q < 3; ++q) S.u.cpos[r * kCellStage + lane + 64 * q] = v[r][q];
     }
     __syncwarp();
     if (tests) {
         // the three ranges as one list in visiting order: candidate t lies in row 0 below n0, in
         // row 1 below n01, else in row 2 (the wave runs its longest list, not its longest row)
         const int n0 = ns[0], n01 = ns[0] + ns[1], nt = n01 + ns[2];
+        // candidate t lies at t plus its row's offset: in the staged rows (read as one array of
+        // three rows) or in the map
+        const int so0 = lo[0] - u0[0], so1 = kCellStage + lo[1] - u0[1] - n0, so2 = 2 * kCellStage + lo[2] - u0[2] - n01;
+        const int go0 = lo[0], go1 = lo[1] - n0, go2 = lo[2] - n01;
+        const int sd1 = so1 - so0, sd2 = so2 - so1, gd1 = go1 - go0, gd2 = go2 - go1;
         auto test = [&](auto seamTag, auto stagedTag) {
             constexpr bool kSeam = decltype(seamTag)::value;
             constexpr bool kStaged = decltype(stagedTag)::value;
-            // position of candidate t: staged row and index, or map position
             auto cand = [&](int t) -> f2 {
                 if constexpr (kStaged) {
-                    const int r = t < n0 ? 0 : (t < n01 ? 1 : 2);
-                    const int q = t - (r == 0 ? 0 : (r == 1 ? n0 : n01)) + (r == 0 ? lo[0] - u0[0] : (r == 1 ? lo[1] - u0[1] : lo[2] - u0[2]));
-                    return to_f2(S.u.cpos[r][min(q, kCellStage - 1)]);
+                    int x = t + so0;
+                    x += t >= n0 ? sd1 : 0;
+                    x += t >= n01 ? sd2 : 0;
+                    return to_f2(S.u.cpos[min(x, 3 * kCellStage - 1)]);
                 } else {
-                    const int r = t < n0 ? 0 : (t < n01 ? 1 : 2);
-                    const int m = (r == 0 ? lo[0] : (r == 1 ? lo[1] - n0 : lo[2] - n01)) + t;
-                    return to_f2(c.mpos[min(m, mEnd - 1)]);
+                    int x = t + go0;
+                    x += t >= n0 ? gd1 : 0;
+                    x += t >= n01 ? gd2 : 0;
+                    return to_f2(c.A.pos[c.sorted[min(x, mEnd - 1)]]);
                 }
             };
             for (int t = 0; t < nt; t += 2) {

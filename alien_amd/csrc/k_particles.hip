@@ -59,7 +59,7 @@ __global__ void k_part_target(Ctx c) {
             cell_bin_range(c, w, cell_bin_local(w, qx, qy), cs, ce);
             for (int m = cs; m < ce; ++m) {
                 int ax, ay;
-                slot_of(to_f2(c.mpos[m]), w.W, w.H, ax, ay);
+                slot_of(to_f2(c.A.pos[c.sorted[m]]), w.W, w.H, ax, ay);
                 if (ax == qx && ay == qy) {
                     cell = m;
                     break;

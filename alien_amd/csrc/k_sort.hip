@@ -160,8 +160,7 @@ __global__ void __launch_bounds__(kScanThreads) k_scan_down(const uint32_t* coun
 // order, so a stayer's rank among them is its place in that bin minus the movers that left the bin
 // before it (marked lanes of the wave, and cells of the bin in earlier waves), and a mover from a
 // lower place precedes them all, one from a higher place follows them all; every cell then adds
-// the bin's movers with a smaller A index.  With the cell goes a copy of the state the neighbour
-// pass reads.
+// the bin's movers with a smaller A index.
 __global__ void k_cell_scatter(Ctx c) {
     int2 blk = c.cellBlocks[blockIdx.x];
     const DevWorld w = c.worlds[blk.x];
@@ -173,9 +172,6 @@ __global__ void k_cell_scatter(Ctx c) {
     const bool mover = (uint32_t(bo) & kMoverBit) != 0u;
     const unsigned long long movers = __ballot(active && mover);
     if (!active) return;
-    const float2 p = c.A.pos[i], v = c.A.vel[i];
-    const float dd = c.A.dens[i];
-    const uint32_t f = c.A.flags[i];
     const int gb = int(uint32_t(bo) & ~kMoverBit);
     const uint32_t base = c.startPrev[w.binOff];
     const int s = w.cellOff + int(c.startPrev[gb] - base), e = w.cellOff + int(c.startPrev[gb + 1] - base);
@@ -197,13 +193,15 @@ __global__ void k_cell_scatter(Ctx c) {
     }
     for (int j = head; j != kNone; j = c.moverNext[j]) rank += j < i ? 1 : 0;
     const int dest = dest0 + rank;
-    // the neighbour pass of a fluid world reads 1 / density
-    const float d = c.params[blk.x].motion == 0 ? 1.0f / dd : dd;
     c.sorted[dest] = i;
-    c.mpos[dest] = p;
-    c.mvel[dest] = v;
-    c.mdens[dest] = d;
-    c.mflags[dest] = f;
+    // the row-staged collision pass reads a copy of the state in map order; the fluid pass reads
+    // the state through `sorted`
+    if (c.params[blk.x].motion != 0) {
+        c.mpos[dest] = c.A.pos[i];
+        c.mvel[dest] = c.A.vel[i];
+        c.mdens[dest] = c.A.dens[i];
+        c.mflags[dest] = c.A.flags[i];
+    }
 }
 
 __global__ void k_part_scatter(Ctx c) {

@@ -282,7 +282,10 @@ typedef struct alien_job_info {
 
 /* Continuous-batching server: one controller thread keeps all admitted worlds resident on the
  * device, advances them together in chunks of `chunkSteps`, admits queued jobs whenever their
- * estimated device bytes fit `budgetBytes`, and copies a snapshot out when a job completes. */
+ * estimated device bytes fit `budgetBytes`, and copies a snapshot out when a job completes.
+ * `budgetBytes` bounds the peak: admitting rebuilds the resident set into fresh memory while the
+ * old set still holds its own (see alien_server_plan_admission).  A failed rebuild fails only the
+ * jobs it was admitting (the resident jobs go on); a failed step fails the jobs of that set. */
 int alien_server_create(uint64_t budgetBytes, uint32_t chunkSteps, alien_server** out);
 int alien_server_destroy(alien_server* s);
 int alien_server_submit(alien_server* s, const alien_world_desc* desc, const alien_params* params,
@@ -300,6 +303,14 @@ int alien_server_release(alien_server* s, uint64_t jobId);
 /* Estimated device bytes a job of this size needs (the admission figure). */
 uint64_t alien_server_estimate_bytes(uint64_t numCells, uint64_t numParticles, int32_t worldSizeX,
                                      int32_t worldSizeY);
+/* The admission rule of the server (host arithmetic only): how many of the queued jobs, in order,
+ * are admitted now.  oldSetBytes: device bytes of the current resident set (0 if none), which stay
+ * allocated while the new set is built; keptBytes: estimated bytes of the resident jobs that go on;
+ * queuedBytes[numQueued]: estimates of the queued jobs.  A job is admitted while
+ * oldSetBytes + keptBytes + (bytes admitted so far) + its bytes <= budgetBytes; the first job that
+ * does not fit stops admission (FIFO). */
+uint64_t alien_server_plan_admission(uint64_t budgetBytes, uint64_t oldSetBytes, uint64_t keptBytes,
+                                     const uint64_t* queuedBytes, uint64_t numQueued);
 
 /* ---- settings files ---------------------------------------------------------------------- */
 

@@ -193,6 +193,32 @@ def test_estimator_at_the_set_limit():
     assert e4 > 160e9
 
 
+def test_server_admission_counts_the_rebuild_peak():
+    """The serving controller's admission rule: while a rebuild builds the new resident set, the old
+    one still holds its memory, so old + kept + admitted must fit the budget; admission is FIFO and
+    stops at the first job that does not fit."""
+    L = ctypes.CDLL(engine.LIB_PATH)
+    f = L.alien_server_plan_admission
+    f.restype = ctypes.c_uint64
+    f.argtypes = [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64, ctypes.POINTER(ctypes.c_uint64), ctypes.c_uint64]
+
+    def plan(budget, old, kept, queued):
+        arr = (ctypes.c_uint64 * max(1, len(queued)))(*queued)
+        return f(budget, old, kept, arr, len(queued))
+
+    GB = 1 << 30
+    assert plan(200 * GB, 0, 0, [50 * GB, 60 * GB, 100 * GB]) == 2      # 50 + 60 fit, + 100 does not
+    assert plan(200 * GB, 0, 0, [50 * GB, 60 * GB, 90 * GB]) == 3       # exactly the budget
+    # a 120 GB resident set: rebuilding it with one more job needs 120 (old) + 120 (kept) + the job
+    assert plan(200 * GB, 120 * GB, 120 * GB, [10 * GB]) == 0
+    assert plan(300 * GB, 120 * GB, 120 * GB, [10 * GB, 60 * GB]) == 1
+    # FIFO: a small job behind one that does not fit waits
+    assert plan(200 * GB, 0, 0, [250 * GB, 1 * GB]) == 0
+    assert plan(200 * GB, 0, 0, []) == 0
+    # no overflow past 2^64
+    assert plan(2**64 - 1, 2**63, 2**62, [2**63]) == 0
+
+
 @pytest.mark.parametrize("what", ["cell", "particle"])
 def test_colour_above_the_palette_is_rejected(what):
     """ALIEN has colours 0..6 (MAX_COLORS = 7); a transfer object with colour 7 or more is refused
