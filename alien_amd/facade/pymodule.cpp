@@ -33,6 +33,97 @@ alien_params paramsFromBytes(py::bytes b) {
 
 PYBIND11_MODULE(_facade, m) {
     m.doc() = "C++ SimulationFacade (physics subset of ALIEN's _SimulationFacade) over the HIP C ABI";
+    using namespace alien;
+    // data descriptions (Descriptions.h) and their conversion (host only)
+    py::class_<RealVector2D>(m, "RealVector2D")
+        .def(py::init([](float x, float y) { return RealVector2D{x, y}; }), py::arg("x") = 0.0f, py::arg("y") = 0.0f)
+        .def_readwrite("x", &RealVector2D::x)
+        .def_readwrite("y", &RealVector2D::y);
+    py::class_<ConnectionDescription>(m, "ConnectionDescription")
+        .def(py::init<>())
+        .def_readwrite("cellId", &ConnectionDescription::cellId)
+        .def_readwrite("distance", &ConnectionDescription::distance)
+        .def_readwrite("angleFromPrevious", &ConnectionDescription::angleFromPrevious);
+    py::class_<CellMetadataDescription>(m, "CellMetadataDescription")
+        .def(py::init<>())
+        .def_readwrite("name", &CellMetadataDescription::name)
+        .def_readwrite("description", &CellMetadataDescription::description);
+    py::class_<CellFunctionDescription>(m, "CellFunctionDescription")
+        .def(py::init<>())
+        .def_readwrite("type", &CellFunctionDescription::type)
+        .def_readwrite("state", &CellFunctionDescription::state)
+        .def_readwrite("genome", &CellFunctionDescription::genome);
+    py::class_<CellDescription>(m, "CellDescription")
+        .def(py::init<>())
+        .def_readwrite("id", &CellDescription::id)
+        .def_readwrite("connections", &CellDescription::connections)
+        .def_readwrite("pos", &CellDescription::pos)
+        .def_readwrite("vel", &CellDescription::vel)
+        .def_readwrite("energy", &CellDescription::energy)
+        .def_readwrite("stiffness", &CellDescription::stiffness)
+        .def_readwrite("color", &CellDescription::color)
+        .def_readwrite("maxConnections", &CellDescription::maxConnections)
+        .def_readwrite("barrier", &CellDescription::barrier)
+        .def_readwrite("age", &CellDescription::age)
+        .def_readwrite("livingState", &CellDescription::livingState)
+        .def_readwrite("creatureId", &CellDescription::creatureId)
+        .def_readwrite("mutationId", &CellDescription::mutationId)
+        .def_readwrite("ancestorMutationId", &CellDescription::ancestorMutationId)
+        .def_readwrite("genomeComplexity", &CellDescription::genomeComplexity)
+        .def_readwrite("activationTime", &CellDescription::activationTime)
+        .def_readwrite("cellFunction", &CellDescription::cellFunction)
+        .def_readwrite("metadata", &CellDescription::metadata);
+    py::class_<ClusterDescription>(m, "ClusterDescription")
+        .def(py::init<>())
+        .def_readwrite("cells", &ClusterDescription::cells);
+    py::class_<ParticleDescription>(m, "ParticleDescription")
+        .def(py::init<>())
+        .def_readwrite("id", &ParticleDescription::id)
+        .def_readwrite("pos", &ParticleDescription::pos)
+        .def_readwrite("vel", &ParticleDescription::vel)
+        .def_readwrite("energy", &ParticleDescription::energy)
+        .def_readwrite("color", &ParticleDescription::color);
+    py::class_<ClusteredDataDescription>(m, "ClusteredDataDescription")
+        .def(py::init<>())
+        .def_readwrite("clusters", &ClusteredDataDescription::clusters)
+        .def_readwrite("particles", &ClusteredDataDescription::particles);
+    py::class_<DataDescription>(m, "DataDescription")
+        .def(py::init<>())
+        .def(py::init<ClusteredDataDescription const&>())
+        .def_readwrite("cells", &DataDescription::cells)
+        .def_readwrite("particles", &DataDescription::particles);
+    // conversions with a fresh store of pass-through data (TO = transfer-object bytes)
+    m.def("dataToTO", [](DataDescription const& d) {
+        SimulationData to;
+        AuxStore aux;
+        DescriptionConverter::toTO(d, to, aux);
+        return py::make_tuple(toBytes(to.cells), toBytes(to.particles));
+    });
+    m.def("clusteredToTO", [](ClusteredDataDescription const& d) {
+        SimulationData to;
+        AuxStore aux;
+        DescriptionConverter::toTO(d, to, aux);
+        return py::make_tuple(toBytes(to.cells), toBytes(to.particles));
+    });
+    m.def("toDataDescription", [](py::array cells, py::array particles) {
+        SimulationData to;
+        to.cells = fromArray<alien_cell_to>(cells);
+        to.particles = fromArray<alien_particle_to>(particles);
+        return DescriptionConverter::toDataDescription(to, AuxStore{});
+    });
+    m.def("toClusteredDataDescription", [](py::array cells, py::array particles) {
+        SimulationData to;
+        to.cells = fromArray<alien_cell_to>(cells);
+        to.particles = fromArray<alien_particle_to>(particles);
+        return DescriptionConverter::toClusteredDataDescription(to, AuxStore{});
+    });
+    // description -> transfer objects -> clustered description, pass-through data kept by id
+    m.def("roundTripClustered", [](DataDescription const& d) {
+        SimulationData to;
+        AuxStore aux;
+        DescriptionConverter::toTO(d, to, aux);
+        return DescriptionConverter::toClusteredDataDescription(to, aux);
+    });
     py::register_exception<alien::MemoryAllocationException>(m, "MemoryAllocationException", PyExc_MemoryError);
     py::class_<SimulationFacade>(m, "SimulationFacade")
         .def(py::init<>())
@@ -51,14 +142,14 @@ PYBIND11_MODULE(_facade, m) {
                  d.cells = fromArray<alien_cell_to>(cells);
                  d.particles = fromArray<alien_particle_to>(particles);
                  py::gil_scoped_release rel;
-                 f.setSimulationData(d);
+                 f.setSimulationTO(d);
              })
         .def("getSimulationData",
              [](SimulationFacade& f) {
                  alien::SimulationData d;
                  {
                      py::gil_scoped_release rel;
-                     d = f.getSimulationData();
+                     d = f.getSimulationTO();
                  }
                  return py::make_tuple(toBytes(d.cells), toBytes(d.particles));
              })
@@ -127,6 +218,31 @@ PYBIND11_MODULE(_facade, m) {
                  d["numCells"] = t.numCells;
                  d["numParticles"] = t.numParticles;
                  return d;
+             })
+        .def("setDataDescription",
+             [](SimulationFacade& f, DataDescription const& d) {
+                 py::gil_scoped_release rel;
+                 f.setSimulationData(d);
+             })
+        .def("getDataDescription",
+             [](SimulationFacade& f) {
+                 py::gil_scoped_release rel;
+                 return f.getSimulationData();
+             })
+        .def("setClusteredSimulationData",
+             [](SimulationFacade& f, ClusteredDataDescription const& d) {
+                 py::gil_scoped_release rel;
+                 f.setClusteredSimulationData(d);
+             })
+        .def("getClusteredSimulationData",
+             [](SimulationFacade& f) {
+                 py::gil_scoped_release rel;
+                 return f.getClusteredSimulationData();
+             })
+        .def("addAndSelectSimulationData",
+             [](SimulationFacade& f, DataDescription const& d) {
+                 py::gil_scoped_release rel;
+                 f.addAndSelectSimulationData(d);
              })
         .def("testOnly_failNextWorkerStep", &SimulationFacade::testOnly_failNextWorkerStep)
         .def("testOnly_setGuardTimeoutMs",

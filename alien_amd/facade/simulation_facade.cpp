@@ -135,9 +135,7 @@ std::string SimulationFacade::getGpuName() const {
     return buf;
 }
 
-SimulationData SimulationFacade::getSimulationData() {
-    Guard g(*this);
-    throwIfWorkerFailed();
+SimulationData SimulationFacade::readTO() {
     SimulationData d;
     uint64_t n = 0, p = 0;
     check(alien_get_counts(_world, &n, &p));
@@ -149,11 +147,69 @@ SimulationData SimulationFacade::getSimulationData() {
     return d;
 }
 
-void SimulationFacade::setSimulationData(SimulationData const& data) {
-    Guard g(*this);
+void SimulationFacade::writeTO(SimulationData const& data) {
     check(alien_set_data(_world, data.cells.data(), data.cells.size(), data.particles.data(), data.particles.size()));
     std::lock_guard<std::mutex> lk(_stateMutex);
     _workerError = nullptr;  // fresh data makes the engine usable again
+}
+
+SimulationData SimulationFacade::getSimulationTO() {
+    Guard g(*this);
+    throwIfWorkerFailed();
+    return readTO();
+}
+
+void SimulationFacade::setSimulationTO(SimulationData const& data) {
+    Guard g(*this);
+    writeTO(data);
+}
+
+DataDescription SimulationFacade::getSimulationData() {
+    Guard g(*this);
+    throwIfWorkerFailed();
+    return DescriptionConverter::toDataDescription(readTO(), _aux);
+}
+
+ClusteredDataDescription SimulationFacade::getClusteredSimulationData() {
+    Guard g(*this);
+    throwIfWorkerFailed();
+    return DescriptionConverter::toClusteredDataDescription(readTO(), _aux);
+}
+
+void SimulationFacade::setSimulationData(DataDescription const& data) {
+    SimulationData to;
+    AuxStore aux;
+    DescriptionConverter::toTO(data, to, aux);  // (checked before anything changes)
+    Guard g(*this);
+    writeTO(to);
+    _aux = std::move(aux);
+}
+
+void SimulationFacade::setClusteredSimulationData(ClusteredDataDescription const& data) {
+    SimulationData to;
+    AuxStore aux;
+    DescriptionConverter::toTO(data, to, aux);
+    Guard g(*this);
+    writeTO(to);
+    _aux = std::move(aux);
+}
+
+// the current data plus the new (whose bonds may refer only to its own cells), set as a whole
+void SimulationFacade::addAndSelectSimulationData(DataDescription const& data) {
+    SimulationData added;
+    AuxStore aux;
+    DescriptionConverter::toTO(data, added, aux);
+    Guard g(*this);
+    throwIfWorkerFailed();
+    SimulationData to = readTO();
+    const int32_t offset = int32_t(to.cells.size());
+    for (auto c : added.cells) {
+        for (int b = 0; b < c.numConnections; ++b) c.connections[b].cellIndex += offset;
+        to.cells.push_back(c);
+    }
+    to.particles.insert(to.particles.end(), added.particles.begin(), added.particles.end());
+    writeTO(to);
+    for (auto& kv : aux) _aux[kv.first] = std::move(kv.second);
 }
 
 void SimulationFacade::calcTimesteps(uint64_t timesteps) {

@@ -22,6 +22,7 @@
 #include <vector>
 
 #include "../../include/alien_hip.h"
+#include "descriptions.hpp"
 
 namespace alien {
 
@@ -33,12 +34,6 @@ struct GeneralSettings {
 
 struct IntVector2D {
     int x = 0, y = 0;
-};
-
-// DataTO-shaped host data (TOs.cuh:195-236) in vectors
-struct SimulationData {
-    std::vector<alien_cell_to> cells;
-    std::vector<alien_particle_to> particles;
 };
 
 // CudaMemoryAllocationException analog (Base/Exceptions.h:15-20)
@@ -57,8 +52,18 @@ public:
     void clear();
     std::string getGpuName() const;
 
-    SimulationData getSimulationData();
-    void setSimulationData(SimulationData const& data);
+    // getSimulationData / getClusteredSimulationData / setSimulationData /
+    // setClusteredSimulationData / addAndSelectSimulationData (SimulationFacade.h:39-47), the
+    // whole world; adding selects nothing (selection is not part of the physics path).  The
+    // pass-through data of the cells (cell-function state, genome, metadata) is kept here by id.
+    DataDescription getSimulationData();
+    ClusteredDataDescription getClusteredSimulationData();
+    void setSimulationData(DataDescription const& data);
+    void setClusteredSimulationData(ClusteredDataDescription const& data);
+    void addAndSelectSimulationData(DataDescription const& data);
+    // the same data as transfer objects (DataTO of the device seam)
+    SimulationData getSimulationTO();
+    void setSimulationTO(SimulationData const& data);
 
     void calcTimesteps(uint64_t timesteps);
     void runSimulation();
@@ -91,7 +96,11 @@ private:
     std::optional<std::string> takeInjectedFailure();
     static void check(int rc);
 
+    SimulationData readTO();
+    void writeTO(SimulationData const& data);
+
     alien_world* _world = nullptr;
+    AuxStore _aux;  // pass-through data of the cells, by id
     GeneralSettings _settings;
     alien_params _params{};
 
