@@ -106,7 +106,7 @@ struct DevCounters {
 // global counters of one step (reset at the start of every step)
 struct StepCounters {
     int candidates;  // fusion candidate pairs
-    int errors;      // bit 0: candidate list overflow
+    int errors;      // bit 0: candidate list overflow; bit 1: map-build look-back gave up
     int liveParticles;  // energy particles alive after this step's particle sort, all worlds
     // neighbour-pass paths since the data was set (not reset per step): chunks that walked their
     // stencils in global memory, and cells of staged rows that walked theirs alone
@@ -173,10 +173,15 @@ struct Ctx {
     // per cell bin: movers in minus movers out (cell-bin size = last size + delta)
     int* delta;
     int* deltaNext;
-    uint32_t* blockSums;
+    // one-pass map scan: per tile of bins a status word (flag, launch epoch, sum), the launch epoch
+    // and the tile ticket
+    unsigned long long* mapStatus;
+    uint32_t* mapEpoch;
+    uint32_t* mapTicket;
     int* sorted;   // [cellCap] map order (bin, then previous index) -> A index
-    // [cellCap] the start-of-step state of the cells in map order, written next to `sorted`, so
-    // that the neighbour pass reads each row of bins as one contiguous range
+    // [cellCap] the start-of-step state of the cells of collision worlds in map order, written next
+    // to `sorted`, so that the row-staged neighbour pass reads each row of bins as one contiguous
+    // range (the fluid pass reads the state through `sorted`)
     float2* mpos;
     float2* mvel;
     float* mdens;  // fluid: 1 / density

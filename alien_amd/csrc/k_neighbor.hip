@@ -1029,7 +1029,9 @@ __device__ __forceinline__ Nb map_nb(const Ctx& c, int m) {
     return o;
 }
 
-__global__ void __launch_bounds__(64 * kCellWaves) k_neighbor_cells(Ctx c, int tableBlocks) {
+// (at most 80 VGPRs: six waves per SIMD)
+__global__ void __launch_bounds__(64 * kCellWaves) __attribute__((amdgpu_waves_per_eu(6)))
+k_neighbor_cells(Ctx c, int tableBlocks) {
     __shared__ CellShared SS[kCellWaves];
     const int per = (tableBlocks + kXcdCount - 1) / kXcdCount;
     const int e = int(blockIdx.x % kXcdCount) * per + int(blockIdx.x / kXcdCount);
@@ -1086,8 +1088,6 @@ __global__ void __launch_bounds__(64 * kCellWaves) k_neighbor_cells(Ctx c, int t
     const bool seam = __any(active && (by == 0 || by == w.nby - 1)) || w.nbx < 8 || w.nby < 8;
     unsigned long long mask = 0ull, bmask = 0ull;
     int selfIdx = -1;  // pairs before the cell's own density term
-    int bt[kMaxBonds];
-    load_bonds(c, self, bt);
     // The union of the lanes' ranges of each stencil row is staged in LDS (all its loads in flight
     // together) when it fits; a wave whose cells span two bin rows tests from global memory.
     const bool tests = active && !direct;
@@ -1181,6 +1181,10 @@ __global__ void __launch_bounds__(64 * kCellWaves) k_neighbor_cells(Ctx c, int t
             selfIdx = __builtin_popcountll(mask & ((1ull << selfFlat) - 1ull));
         }
         if (flags_num_conn(self.flags) > 0) {
+            // (the bond targets are loaded here and again at the end rather than held through the
+            // pair work: few cells of a fluid world have bonds, and the loads hit the cache)
+            int bt[kMaxBonds];
+            load_bonds(c, self, bt);
             unsigned long long mm = mask;
             while (mm) {
                 const int ci = __builtin_ctzll(mm);
@@ -1272,6 +1276,8 @@ __global__ void __launch_bounds__(64 * kCellWaves) k_neighbor_cells(Ctx c, int t
         return;
     }
     if (selfIdx == count) a.rho = a.rho + w0;  // the own term after all pairs
+    int bt[kMaxBonds];
+    load_bonds(c, self, bt);
     cell_finish<true>(c, w, P, wi, self, c.A.slot[self.aidx], bt, a, w.binOff + by * w.nbx + bx);
 }
 

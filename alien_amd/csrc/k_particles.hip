@@ -100,19 +100,23 @@ __global__ void k_part_apply(Ctx c) {
         if (c.want[k] >= 0) return;
         const int cm = c.absorbCell[k];
         if (cm < 0) return;
-        const uint32_t fc = c.B.flags[cm];
+        // the cell as the neighbour pass found it (through the map: a barrier cell does not move
+        // or change speed in that pass, and no cell changes its flags or slot there)
+        const int ca = c.sorted[cm];
+        const uint32_t fc = c.A.flags[ca];
         if (flags_barrier(fc)) {  // reflection (:99-106)
-            const f2 vr = sub(to_f2(c.B.pvel[k]), to_f2(c.B.vel[cm]));
-            const f2 rr = torus_delta(to_f2(c.B.ppos[k]), to_f2(c.B.pos[cm]), float(w.W), float(w.H));
+            const f2 vb = to_f2(c.A.vel[ca]);
+            const f2 vr = sub(to_f2(c.B.pvel[k]), vb);
+            const f2 rr = torus_delta(to_f2(c.B.ppos[k]), to_f2(c.A.pos[ca]), float(w.W), float(w.H));
             const float d = dot2(vr, rr);
             if (d < 0.0f) {
                 const float trs = fmaxf(0.1f, lensq(rr));
-                c.B.pvel[k] = to_float2(add(sub(vr, dvd(mul(mul(rr, 2.0f), d), trs)), to_f2(c.B.vel[cm])));
+                c.B.pvel[k] = to_float2(add(sub(vr, dvd(mul(mul(rr, 2.0f), d), trs)), vb));
             }
             c.absorbCell[k] = -1;
             return;
         }
-        if (c.B.plast[k] == c.B.slot[cm] || P.absorption[flags_color(fc) % kColors] < kNearZero) {
+        if (c.B.plast[k] == c.A.slot[ca] || P.absorption[flags_color(fc) % kColors] < kNearZero) {
             c.absorbCell[k] = -1;
             return;
         }
@@ -127,12 +131,13 @@ __global__ void k_part_absorb(Ctx c) {
         if (cm >= 0 && c.cclaim[cm] == k) {
             c.cclaim[cm] = kClaimFree;
             const DevParams& P = c.params[wi];
-            const uint32_t fc = c.B.flags[cm];
+            const int ca = c.sorted[cm];
+            const uint32_t fc = c.A.flags[ca];
             const float ra = P.absorption[flags_color(fc) % kColors];
             float e = c.B.pen[k];
             float transfer = e * ra;
             if (e < 0.01f) transfer = e;
-            const int cslot = c.B.slot[cm];
+            const int cslot = c.A.slot[ca];
             float* ce = &c.energy[w.cellOff + cslot];
             *ce = *ce + transfer;
             e = e - transfer;

@@ -41,11 +41,7 @@ __global__ void k_part_bin(Ctx c) {
     });
 }
 
-// exclusive scan of the bin sizes into start[0..n]: tile totals, then tile scans that each add
-// the totals of the tiles before them
-constexpr int kScanThreads = 256;
-constexpr int kScanItems = 8;
-constexpr int kScanTile = kScanThreads * kScanItems;
+constexpr int kScanItems = 8;  // bins per thread of the map scan
 
 // bin sizes base..base+7: a cell bin's size is its size in the last map plus its delta, a particle
 // bin's its count
@@ -82,76 +78,137 @@ __device__ __forceinline__ void load8(const uint32_t* count, const uint32_t* sta
         if (base + k < size_t(nCell)) v[k] = uint32_t(int(sp[k + 1] - sp[k]) + dl[k]);
 }
 
-// tile totals; also opens the step: clears the per-step list counters, and advances the worlds'
-// timesteps if `advance` is set (a step without bond work has no later reader of them)
-__global__ void __launch_bounds__(kScanThreads) k_scan_reduce(const uint32_t* count, const uint32_t* startPrev,
-                                                              const int* delta, int nCell, uint32_t* blockSums, int n,
-                                                              StepCounters* sc,
-                                                              DevCounters* advance, int numWorlds) {
-    const size_t base = size_t(blockIdx.x) * kScanTile + size_t(threadIdx.x) * kScanItems;
-    uint32_t v[8];
-    load8(count, startPrev, delta, nCell, base, n, v);
-    uint32_t s = v[0] + v[1] + v[2] + v[3] + v[4] + v[5] + v[6] + v[7];
-    uint32_t total;
-    block_exclusive_scan(s, total);
-    if (threadIdx.x == 0) {
-        blockSums[blockIdx.x] = total;
-        if (sc && blockIdx.x == 0) {
-            sc->candidates = 0;
-            sc->liveParticles = 0;
-        }
-    }
-    if (advance && blockIdx.x == 0)
-        for (int wi = threadIdx.x; wi < numWorlds; wi += kScanThreads) advance[wi].timestep += 1ull;
+// ---------------------------------------------------------------------------------------------
+// Offsets of a step's map in one pass over the bins.  Every workgroup takes the next tile of 8192
+// bins (tickets in the order the workgroups start), sums its bin sizes and publishes the sum, then
+// finds the sum of all bins before it by looking back over what the tiles before it published
+// (decoupled look-back: a tile publishes its own sum before it looks back, and every tile before
+// it holds an earlier ticket and has started, so each wait ends).  With that it writes the offsets
+// of its bins.  It also empties the next step's deltas and mover lists of its cell bins (their
+// last readers ran a step ago), and the first tile opens the step: clears the per-step list
+// counters and advances the worlds' timesteps if asked (a step without bond work has no later
+// reader of them).  `count` stays: the particle scatter counts its particle bins down to zero,
+// which leaves them cleared for the step after next (the two histograms alternate).
+// ---------------------------------------------------------------------------------------------
+constexpr int kMapScanThreads = 1024;
+constexpr int kMapScanTile = kMapScanThreads * kScanItems;
+// tile status word: flag (bits 62-63: 1 the tile's sum, 2 the sum up to and including the tile),
+// launch epoch (bits 32-61), value (bits 0-31)
+constexpr unsigned long long kTileSum = 1ull << 62, kTileInclusive = 2ull << 62;
+constexpr uint32_t kEpochMask = 0x3fffffffu;
+constexpr int kLookBackSpins = 1 << 22;  // far beyond any wait for a running tile
+
+__device__ __forceinline__ bool tile_status_ready(unsigned long long w, uint32_t epoch) {
+    return (w >> 62) != 0ull && uint32_t((w >> 32) & kEpochMask) == epoch;
 }
 
-// writes start[]: every tile adds the totals of the tiles before it (a few thousand words, read
-// from L2) to its own exclusive scan.  `count` stays: the particle scatter counts its particle
-// bins down to zero, which leaves them cleared for the step after next (the two histograms
-// alternate); the map count writes the cell bins every step
-__global__ void __launch_bounds__(kScanThreads) k_scan_down(const uint32_t* count, const uint32_t* startPrev,
-                                                            const int* delta, int nCell, const uint32_t* blockSums,
-                                                            uint32_t* start, int n, int* deltaNext, int* moverHeadNext) {
-    uint32_t before = 0;
-    for (int t = int(threadIdx.x); t < int(blockIdx.x); t += kScanThreads) before += blockSums[t];
-    uint32_t prefix;
-    block_exclusive_scan(before, prefix);
-    const size_t base = size_t(blockIdx.x) * kScanTile + size_t(threadIdx.x) * kScanItems;
+__global__ void __launch_bounds__(kMapScanThreads) k_map_scan(Ctx c, int tiles, StepCounters* open, int advance) {
+    __shared__ int sTile;
+    __shared__ uint32_t sEpoch, sPrefix;
+    const int tid = int(threadIdx.x), lane = tid & 63;
+    // All atomics here are relaxed (agent scope): a status word carries its whole message (flag,
+    // epoch, value), and the epoch and ticket a launch reads were written by the launches before
+    // it.  Ordered ones would write back or invalidate the XCD's L2 at every store and load.
+    if (tid == 0) {
+        const uint32_t e = __hip_atomic_load(c.mapEpoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        // the epoch has been read when the ticket is taken; the last ticket's holder advances it
+        __asm__ volatile("s_waitcnt vmcnt(0)" ::: "memory");
+        sEpoch = e & kEpochMask;
+        sTile = int(__hip_atomic_fetch_add(c.mapTicket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    }
+    __syncthreads();
+    const int tile = sTile;
+    const uint32_t epoch = sEpoch;
+    const unsigned long long ep = (unsigned long long)epoch << 32;
+    const int n = c.numBins, nCell = c.numCellBins;
+    const size_t base = size_t(tile) * kMapScanTile + size_t(tid) * kScanItems;
     uint32_t v[8];
-    load8(count, startPrev, delta, nCell, base, n, v);
-    // the next step's deltas and mover lists start empty (their last readers ran a step ago)
-    if (deltaNext) {
-        if (base + 8 <= size_t(nCell)) {
-            *reinterpret_cast<int4*>(deltaNext + base) = make_int4(0, 0, 0, 0);
-            *reinterpret_cast<int4*>(deltaNext + base + 4) = make_int4(0, 0, 0, 0);
-            *reinterpret_cast<int4*>(moverHeadNext + base) = make_int4(kNone, kNone, kNone, kNone);
-            *reinterpret_cast<int4*>(moverHeadNext + base + 4) = make_int4(kNone, kNone, kNone, kNone);
-        } else {
-#pragma unroll
-            for (int k = 0; k < 8; ++k)
-                if (base + k < size_t(nCell)) {
-                    deltaNext[base + k] = 0;
-                    moverHeadNext[base + k] = kNone;
-                }
-        }
-    }
-    uint32_t s = v[0] + v[1] + v[2] + v[3] + v[4] + v[5] + v[6] + v[7];
+    load8(c.count, c.startPrev, c.delta, nCell, base, n, v);
+    const uint32_t s = v[0] + v[1] + v[2] + v[3] + v[4] + v[5] + v[6] + v[7];
     uint32_t total;
-    uint32_t off = block_exclusive_scan(s, total) + prefix;
-    if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) start[n] = prefix + total;
-    uint32_t o[8];
+    const uint32_t off = block_exclusive_scan(s, total);
+    if (tid == 0)
+        __hip_atomic_store(&c.mapStatus[tile], (tile == 0 ? kTileInclusive : kTileSum) | ep | total, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+    if (tid < 64) {
+        // the first wave looks back, 64 tiles at a time, until a tile with its inclusive sum
+        uint32_t prefix = 0;
+        if (tile > 0) {
+            for (int j = tile - 1;; j -= 64) {
+                const int idx = j - lane;
+                unsigned long long w = kTileInclusive | ep;  // before the first tile: sum 0
+                if (idx >= 0) {
+                    // (bounded: a tile that never published would be a bug; the step then reports
+                    // an error instead of hanging)
+                    for (int spin = 0;; ++spin) {
+                        w = __hip_atomic_load(&c.mapStatus[idx], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        if (tile_status_ready(w, epoch)) break;
+                        if (spin >= kLookBackSpins) {
+                            atomicOr(&c.sc->errors, 2);
+                            w = kTileInclusive | ep;
+                            break;
+                        }
+                        __builtin_amdgcn_s_sleep(1);
+                    }
+                }
+                const unsigned long long incl = __ballot((w >> 62) == 2ull);
+                const int stop = incl ? __builtin_ctzll(incl) : 64;
+                uint32_t val = lane <= stop ? uint32_t(w) : 0u;
 #pragma unroll
-    for (int k = 0; k < 8; ++k) {
-        o[k] = off;
-        off += v[k];
+                for (int o = 32; o > 0; o >>= 1) val += __shfl_xor(val, o, 64);
+                prefix += val;
+                if (incl) break;
+            }
+            if (lane == 0)
+                __hip_atomic_store(&c.mapStatus[tile], kTileInclusive | ep | (prefix + total), __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+        }
+        if (lane == 0) sPrefix = prefix;
     }
-    if (base + 8 <= size_t(n)) {
-        *reinterpret_cast<uint4*>(start + base) = make_uint4(o[0], o[1], o[2], o[3]);
-        *reinterpret_cast<uint4*>(start + base + 4) = make_uint4(o[4], o[5], o[6], o[7]);
+    // the next step's deltas and mover lists start empty
+    if (base + 8 <= size_t(nCell)) {
+        *reinterpret_cast<int4*>(c.deltaNext + base) = make_int4(0, 0, 0, 0);
+        *reinterpret_cast<int4*>(c.deltaNext + base + 4) = make_int4(0, 0, 0, 0);
+        *reinterpret_cast<int4*>(c.moverHeadNext + base) = make_int4(kNone, kNone, kNone, kNone);
+        *reinterpret_cast<int4*>(c.moverHeadNext + base + 4) = make_int4(kNone, kNone, kNone, kNone);
     } else {
 #pragma unroll
         for (int k = 0; k < 8; ++k)
-            if (base + k < size_t(n)) start[base + k] = o[k];
+            if (base + k < size_t(nCell)) {
+                c.deltaNext[base + k] = 0;
+                c.moverHeadNext[base + k] = kNone;
+            }
+    }
+    if (tile == 0) {
+        if (open && tid == 0) {
+            open->candidates = 0;
+            open->liveParticles = 0;
+        }
+        if (advance)
+            for (int wi = tid; wi < c.numWorlds; wi += kMapScanThreads) c.ctr[wi].timestep += 1ull;
+    }
+    __syncthreads();
+    uint32_t o = sPrefix + off;
+    uint32_t st8[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        st8[k] = o;
+        o += v[k];
+    }
+    if (base + 8 <= size_t(n)) {
+        *reinterpret_cast<uint4*>(c.start + base) = make_uint4(st8[0], st8[1], st8[2], st8[3]);
+        *reinterpret_cast<uint4*>(c.start + base + 4) = make_uint4(st8[4], st8[5], st8[6], st8[7]);
+    } else {
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+            if (base + k < size_t(n)) c.start[base + k] = st8[k];
+    }
+    if (base <= size_t(n - 1) && size_t(n - 1) < base + 8) c.start[n] = o;
+    // the last ticket: every ticket is gone; reset the ticket and advance the epoch for the next
+    // launch
+    if (tile == tiles - 1 && tid == 0) {
+        __hip_atomic_store(c.mapTicket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_fetch_add(c.mapEpoch, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
 }
 
@@ -304,16 +361,15 @@ void launch_seed_bins(const Ctx& c, const LaunchShape& sh, hipStream_t s) {
     if (sh.partBlocks) hipLaunchKernelGGL(k_part_bin, dim3(sh.partGrid), dim3(sh.partThreads), 0, s, c);
 }
 
-// exclusive scan of the bin sizes into start (c.start); with `resetNext` it also empties the
-// next step's deltas and mover lists
-void launch_scan(const Ctx& c, hipStream_t s, StepCounters* sc, bool advanceTimestep, bool resetNext) {
-    const int tiles = (c.numBins + kScanTile - 1) / kScanTile;
-    hipLaunchKernelGGL(k_scan_reduce, dim3(tiles), dim3(kScanThreads), 0, s, c.count, c.startPrev, c.delta,
-                       c.numCellBins, c.blockSums, c.numBins, sc, advanceTimestep ? c.ctr : nullptr, c.numWorlds);
-    hipLaunchKernelGGL(k_scan_down, dim3(tiles), dim3(kScanThreads), 0, s, c.count, c.startPrev, c.delta,
-                       c.numCellBins, c.blockSums, c.start, c.numBins, resetNext ? c.deltaNext : nullptr,
-                       c.moverHeadNext);
+// exclusive scan of the bin sizes into start (c.start); with `open` it also opens the step (its
+// list counters cleared) and with `advanceTimestep` advances the worlds' timesteps
+void launch_map_scan(const Ctx& c, hipStream_t s, StepCounters* open, bool advanceTimestep) {
+    const int tiles = map_scan_tiles(c.numBins);
+    if (tiles == 0) return;
+    hipLaunchKernelGGL(k_map_scan, dim3(tiles), dim3(kMapScanThreads), 0, s, c, tiles, open, advanceTimestep ? 1 : 0);
 }
+
+int map_scan_tiles(int numBins) { return (numBins + kMapScanTile - 1) / kMapScanTile; }
 
 void launch_cell_scatter(const Ctx& c, const LaunchShape& sh, hipStream_t s) {
     if (sh.cellBlocks) hipLaunchKernelGGL(k_cell_scatter, dim3(sh.cellBlocks), dim3(sh.cellThreads), 0, s, c);

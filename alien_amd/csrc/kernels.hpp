@@ -35,6 +35,8 @@ struct Recorder {
 };
 
 void launch_fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t s);
+// bin tiles of the one-pass map scan (one status word each)
+int map_scan_tiles(int numBins);
 // bins of the current state (A) for the next sort; the step keeps them up to date itself
 void launch_seed_bins(const Ctx& c, const LaunchShape& sh, hipStream_t s);
 // neighbour-pass workgroups one CU keeps resident (fluid or collision variant)

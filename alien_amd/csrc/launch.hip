@@ -11,7 +11,7 @@
 
 namespace alien {
 
-void launch_scan(const Ctx& c, hipStream_t s, StepCounters* sc, bool advanceTimestep, bool resetNext);
+void launch_map_scan(const Ctx& c, hipStream_t s, StepCounters* open, bool advanceTimestep);
 void launch_cell_scatter(const Ctx& c, const LaunchShape& sh, hipStream_t s);
 void launch_part_sort(const Ctx& c, const LaunchShape& sh, hipStream_t s);
 void launch_part_scatter(const Ctx& c, const LaunchShape& sh, hipStream_t s);
@@ -22,9 +22,10 @@ void launch_particles_post(const Ctx& c, const LaunchShape& sh, hipStream_t s, R
 void launch_bonds(const Ctx& c, const LaunchShape& sh, hipStream_t s, Recorder* rec);
 
 void launch_step(const Ctx& c, const LaunchShape& sh, hipStream_t s, Recorder* rec, const SideStream* side) {
-    if (rec) rec->mark("scan");
-    // without bond work the end-of-step kernel would only advance the timesteps: the scan does it
-    launch_scan(c, s, c.sc, !sh.hasBondWork, true);
+    // without bond work the end-of-step kernel would only advance the timesteps: the map scan does
+    // it
+    if (rec) rec->mark("map_scan");
+    launch_map_scan(c, s, c.sc, !sh.hasBondWork);
     if (rec) rec->mark("cell_scatter");
     launch_cell_scatter(c, sh, s);
     // the particle kernels before the fusions and absorptions read the map and the particles
@@ -39,11 +40,16 @@ void launch_step(const Ctx& c, const LaunchShape& sh, hipStream_t s, Recorder* r
     if (rec && sh.hasParticles) rec->mark("part_sort");
     launch_part_sort(c, sh, ps);
     launch_particles_pre(c, sh, ps, rec);
+    // Absorption adds to cell energies, which the neighbour pass reads for fusion and the bond
+    // kernels for bond decay: without bond work (no bonded, barrier or bondable cell) neither
+    // reads them, and the fusions and absorptions run beside the neighbour pass as well.
+    const bool allBeside = fork && !sh.hasBondWork;
+    if (allBeside) launch_particles_post(c, sh, ps, rec);
     if (fork) (void)hipEventRecord(side->join, ps);
     if (rec) rec->mark("neighbor");
     launch_neighbor(c, sh, s, sh.hasFluid, sh.hasCollision);
     if (fork) (void)hipStreamWaitEvent(s, side->join, 0);
-    launch_particles_post(c, sh, s, rec);
+    if (!allBeside) launch_particles_post(c, sh, s, rec);
     launch_bonds(c, sh, s, rec);
     if (rec) rec->mark(nullptr);
 }
@@ -51,7 +57,7 @@ void launch_step(const Ctx& c, const LaunchShape& sh, hipStream_t s, Recorder* r
 // The map of the current state: consumes the seeded bins and mover lists, which seeding restores
 // for the next step.
 void launch_debug_map(const Ctx& c, const LaunchShape& sh, hipStream_t s) {
-    launch_scan(c, s, nullptr, false, true);
+    launch_map_scan(c, s, nullptr, false);
     launch_cell_scatter(c, sh, s);
     launch_part_scatter(c, sh, s);
     launch_cell_binsort(c, sh, s);

@@ -587,7 +587,8 @@ void WorldSet::build(const std::vector<Source>& sources) {
         delta_[b] = static_cast<int*>(A(sizeof(int) * NCB));
     }
     moverNext_ = static_cast<int*>(A(sizeof(int) * NC));
-    blockSums_ = static_cast<uint32_t*>(A(sizeof(uint32_t) * (size_t(NB) / 2048 + 2)));
+    mapStatus_ = static_cast<unsigned long long*>(A(sizeof(unsigned long long) * size_t(map_scan_tiles(NB))));
+    mapEpochTicket_ = static_cast<uint32_t*>(A(sizeof(uint32_t) * 2));
     sorted_ = static_cast<int*>(A(sizeof(int) * NC));
     mpos_ = static_cast<float2*>(A(sizeof(float2) * NC));
     mvel_ = static_cast<float2*>(A(sizeof(float2) * NC));
@@ -721,6 +722,8 @@ void WorldSet::build(const std::vector<Source>& sources) {
     }
     launch_fill_u32(reinterpret_cast<uint32_t*>(fclaim_), uint32_t(kClaimFree), size_t(NP), stream_);
     launch_fill_u32(reinterpret_cast<uint32_t*>(cclaim_), uint32_t(kClaimFree), size_t(NC), stream_);
+    launch_fill_u32(reinterpret_cast<uint32_t*>(mapStatus_), 0u, 2 * size_t(map_scan_tiles(NB)), stream_);
+    launch_fill_u32(mapEpochTicket_, 0u, 2, stream_);
     launch_fill_u32(reinterpret_cast<uint32_t*>(claim_), 0xffffffffu, size_t(NC) * 2, stream_);
     launch_fill_u32(reinterpret_cast<uint32_t*>(delFlag_), 0u, (size_t(NC) + 3) / 4, stream_);
     launch_fill_u32(reinterpret_cast<uint32_t*>(mark_), 0u, (size_t(NC) + 3) / 4, stream_);
@@ -1006,7 +1009,9 @@ Ctx WorldSet::makeCtx(int parity) const {
     c.pbinOf = pbinOf_;
     c.count = count_[parity];
     c.countNext = count_[parity ^ 1];
-    c.blockSums = blockSums_;
+    c.mapStatus = mapStatus_;
+    c.mapEpoch = mapEpochTicket_;
+    c.mapTicket = mapEpochTicket_ + 1;
     c.sorted = sorted_;
     c.mpos = mpos_;
     c.mvel = mvel_;
@@ -1109,6 +1114,11 @@ void WorldSet::checkErrors() {
         sc.errors = 0;
         check_hip(hipMemcpy(dSc_, &sc, sizeof(sc), hipMemcpyHostToDevice), "H2D");
         throw Error(ALIEN_E_CAPACITY, "fusion candidate list overflow");
+    }
+    if (sc.errors & 2) {
+        sc.errors = 0;
+        check_hip(hipMemcpy(dSc_, &sc, sizeof(sc), hipMemcpyHostToDevice), "H2D");
+        throw Error(ALIEN_E_STATE, "map build: a bin tile's offset was not published in time");
     }
     const int wantCand = std::min(2048, (std::min(sc.candidates, candCap_) + 255) / 256);
     if (wantCand > 2 * shape_.candGrid) {
@@ -1333,7 +1343,7 @@ void WorldSet::statisticsNoAlloc(int wi, alien_statistics* out) {
 void WorldSet::cellMap(int wi, std::vector<int>& bins, std::vector<int>& slots) {
     synchronize();
     const Ctx c = makeCtx(parity_);
-    launch_debug_map(c, shape_, stream_);
+    launch_debug_map(c, shape_, stream_);  // (which seeds the bins again)
     check_hip(hipGetLastError(), "sort");
     synchronize();
     const DevWorld& w = worlds_.at(wi);
@@ -1355,7 +1365,7 @@ void WorldSet::cellMap(int wi, std::vector<int>& bins, std::vector<int>& slots) 
 void WorldSet::neighbors(int wi, std::vector<std::vector<int>>& bySlot) {
     synchronize();
     const Ctx c = makeCtx(parity_);
-    launch_debug_map(c, shape_, stream_);
+    launch_debug_map(c, shape_, stream_);  // (which seeds the bins again)
     int* counts = nullptr;
     int* out = nullptr;
     uint64_t* offs = nullptr;

@@ -209,7 +209,8 @@ private:
     int* delta_[2] = {nullptr, nullptr};
     int* moverNext_ = nullptr;
     int numCellBins_ = 0;
-    uint32_t* blockSums_ = nullptr;
+    unsigned long long* mapStatus_ = nullptr;
+    uint32_t* mapEpochTicket_ = nullptr;  // [2]: epoch, ticket
     int *sorted_ = nullptr, *newidx_ = nullptr, *psorted_ = nullptr;
     float2 *mpos_ = nullptr, *mvel_ = nullptr;
     float* mdens_ = nullptr;

@@ -484,6 +484,62 @@ def test_cpp_facade_end_to_end():
     f.closeSimulation()
 
 
+def test_cpp_facade_clustered_descriptions():
+    """setClusteredSimulationData / getClusteredSimulationData / addAndSelectSimulationData of the
+    facade (SimulationFacade.h:39-47): the same step results as the oracle on the same transfer
+    objects, clusters equal to the bonded bodies, cell metadata and genomes passed through."""
+    from alien_amd import _facade as F
+    from oracle.pyoracle import OracleWorld
+
+    w = worlds.random_mix(31, motion=MOTION_COLLISION, size=(96, 96), num_free=200, patches=8, num_particles=60)
+    cl = F.toClusteredDataDescription(w.cells, w.particles)
+    assert len(cl.clusters) < len(w.cells)  # bonded bodies form clusters
+    first = cl.clusters[0].cells[0]
+    first.metadata.name = "first"
+    fn = F.CellFunctionDescription()
+    fn.type, fn.genome = 4, list(range(20))
+    first.cellFunction = fn
+    k0 = cl.clusters[0]
+    k0.cells = [first] + list(k0.cells[1:])
+    cl.clusters = [k0] + list(cl.clusters[1:])
+    f = F.SimulationFacade()
+    f.newSimulation(0, w.size[0], w.size[1], bytes(w.params))
+    f.setClusteredSimulationData(cl)
+    f.calcTimesteps(7)
+    tc, tp = F.clusteredToTO(cl)
+    ref = OracleWorld(w.size, w.params, precision=32, threads=CPU_THREADS)
+    ref.set_data(np.frombuffer(tc, dtype=abi.CELL_DTYPE), np.frombuffer(tp, dtype=abi.PARTICLE_DTYPE))
+    ref.step(7)
+    oc, op = ref.get_data()
+    cb, pb = f.getSimulationData()
+    fc = np.frombuffer(cb, dtype=abi.CELL_DTYPE)
+    fp = np.frombuffer(pb, dtype=abi.PARTICLE_DTYPE)
+    assert H.packed_bytes(fc) == H.packed_bytes(oc) and H.packed_bytes(fp) == H.packed_bytes(op)
+    got = f.getClusteredSimulationData()
+    exp = F.toClusteredDataDescription(oc, op)
+    assert sorted(sorted(c.id for c in k.cells) for k in got.clusters) == \
+        sorted(sorted(c.id for c in k.cells) for k in exp.clusters)
+    back = [c for k in got.clusters for c in k.cells if c.id == first.id][0]
+    assert back.metadata.name == "first" and list(back.cellFunction.genome) == list(range(20))
+    # adding a body keeps the running world and appends the new cells with their own bonds
+    extra = F.DataDescription()
+    a, b = F.CellDescription(), F.CellDescription()
+    a.pos, b.pos = F.RealVector2D(10.0, 10.0), F.RealVector2D(11.0, 10.0)
+    a.id, b.id = 900001, 900002
+    ca, cbd = F.ConnectionDescription(), F.ConnectionDescription()
+    ca.cellId, ca.distance, ca.angleFromPrevious = 900002, 1.0, 360.0
+    cbd.cellId, cbd.distance, cbd.angleFromPrevious = 900001, 1.0, 360.0
+    a.connections, b.connections = [ca], [cbd]
+    a.maxConnections = b.maxConnections = 6
+    extra.cells = [a, b]
+    f.addAndSelectSimulationData(extra)
+    d = f.getDataDescription()
+    ids = {c.id for c in d.cells}
+    assert len(d.cells) == len(fc) + 2 and {900001, 900002} <= ids
+    assert [x.cellId for c in d.cells if c.id == 900001 for x in c.connections] == [900002]
+    f.closeSimulation()
+
+
 # --------------------------------------------------------------------------------------------
 # batch and server: each world's result is the same alone and in the mix
 
