@@ -1005,8 +1005,10 @@ struct CellShared {
     uint32_t flags[64];
     int aidx[64];
     union {
-        // while candidates are tested: positions of the union of the lanes' ranges, per stencil row
-        float2 cpos[3 * kCellStage];
+        // while candidates are tested: positions of the union of the lanes' ranges, per stencil
+        // row, and one entry past the end (read by the second test of an iteration that passes a
+        // list's end; its result is trimmed)
+        float2 cpos[3 * kCellStage + 1];
         struct {
             // pair records: neighbour map position, owner lane | bonded << 7
             int rnb[kCellRec];
@@ -1141,10 +1143,12 @@ k_neighbor_cells(Ctx c, int tableBlocks) {
             constexpr bool kStaged = decltype(stagedTag)::value;
             auto cand = [&](int t) -> f2 {
                 if constexpr (kStaged) {
-                    int x = t + so0;
-                    x += t >= n0 ? sd1 : 0;
-                    x += t >= n01 ? sd2 : 0;
-                    return to_f2(S.u.cpos[min(x, 3 * kCellStage - 1)]);
+                    // (t <= nt: at most one entry past the last row's range, inside cpos; the
+                    // offsets in bytes, so that a row select is not scaled per candidate)
+                    int xb = 8 * t + 8 * so0;
+                    xb += t >= n0 ? 8 * sd1 : 0;
+                    xb += t >= n01 ? 8 * sd2 : 0;
+                    return to_f2(*reinterpret_cast<const float2*>(reinterpret_cast<const char*>(S.u.cpos) + xb));
                 } else {
                     int x = t + go0;
                     x += t >= n0 ? gd1 : 0;
@@ -1152,14 +1156,22 @@ k_neighbor_cells(Ctx c, int tableBlocks) {
                     return to_f2(c.A.pos[c.sorted[min(x, mEnd - 1)]]);
                 }
             };
-            for (int t = 0; t < nt; t += 2) {
-                // two candidates per iteration; the second may pass the list's end (trimmed below)
-                const f2 p0 = cand(t), p1 = cand(t + 1);
-                const float d0 = lensq(kSeam ? torus_delta(self.pos, p0, Wf, Hf) : sub(self.pos, p0));
-                const float d1 = lensq(kSeam ? torus_delta(self.pos, p1, Wf, Hf) : sub(self.pos, p1));
-                mask |= d0 <= P.reachSq ? (1ull << t) : 0ull;
-                mask |= d1 <= P.reachSq ? (2ull << t) : 0ull;
-            }
+            // two candidates per iteration (the second may pass the list's end: trimmed below),
+            // their bits set in the half of the mask that holds t (a uniform loop bound per half:
+            // a wave's lists rarely pass 32 candidates)
+            auto half = [&](int t0, int t1, uint32_t& m32) {
+                for (int t = t0; t < t1; t += 2) {
+                    const f2 p0 = cand(t), p1 = cand(t + 1);
+                    const float d0 = lensq(kSeam ? torus_delta(self.pos, p0, Wf, Hf) : sub(self.pos, p0));
+                    const float d1 = lensq(kSeam ? torus_delta(self.pos, p1, Wf, Hf) : sub(self.pos, p1));
+                    const uint32_t bit = 1u << (t & 31);
+                    m32 |= (d0 <= P.reachSq ? bit : 0u) | (d1 <= P.reachSq ? bit << 1 : 0u);
+                }
+            };
+            uint32_t lo32 = 0u, hi32 = 0u;
+            half(0, min(nt, 32), lo32);
+            half(32, nt, hi32);
+            mask = (static_cast<unsigned long long>(hi32) << 32) | lo32;
             mask &= nt >= 64 ? ~0ull : (1ull << nt) - 1ull;
         };
         if (staged) {
@@ -1281,10 +1293,9 @@ k_neighbor_cells(Ctx c, int tableBlocks) {
     cell_finish<true>(c, w, P, wi, self, c.A.slot[self.aidx], bt, a, w.binOff + by * w.nbx + bx);
 }
 
-int neighbor_blocks_per_cu(bool fluid) {
+int neighbor_blocks_per_cu() {
     int n = 0;
-    const hipError_t e = fluid ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_neighbor_rows<true>, kRowThreads, 0)
-                               : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_neighbor_rows<false>, kRowThreads, 0);
+    const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_neighbor_rows<false>, kRowThreads, 0);
     return e == hipSuccess ? std::max(1, n) : 1;
 }
 

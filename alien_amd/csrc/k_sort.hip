@@ -41,19 +41,19 @@ __global__ void k_part_bin(Ctx c) {
     });
 }
 
-constexpr int kScanItems = 8;  // bins per thread of the map scan
+constexpr int kScanItems = 8;  // consecutive bins per group of the map scan
 
 // bin sizes base..base+7: a cell bin's size is its size in the last map plus its delta, a particle
-// bin's its count
+// bin's its count (the cell bins come first; `count` is read for particle bins only)
 __device__ __forceinline__ void load8(const uint32_t* count, const uint32_t* startPrev, const int* delta, int nCell,
                                       size_t base, int n, uint32_t v[8]) {
-    if (base + 8 <= size_t(n)) {
+    if (base >= size_t(nCell) && base + 8 <= size_t(n)) {
         const uint4 a = *reinterpret_cast<const uint4*>(count + base);
         const uint4 b = *reinterpret_cast<const uint4*>(count + base + 4);
         v[0] = a.x, v[1] = a.y, v[2] = a.z, v[3] = a.w, v[4] = b.x, v[5] = b.y, v[6] = b.z, v[7] = b.w;
-    } else {
+    } else if (base + 8 > size_t(nCell)) {
 #pragma unroll
-        for (int k = 0; k < 8; ++k) v[k] = base + k < size_t(n) ? count[base + k] : 0u;
+        for (int k = 0; k < 8; ++k) v[k] = base + k >= size_t(nCell) && base + k < size_t(n) ? count[base + k] : 0u;
     }
     if (base >= size_t(nCell)) return;
     uint32_t sp[9];
@@ -91,7 +91,8 @@ __device__ __forceinline__ void load8(const uint32_t* count, const uint32_t* sta
 // which leaves them cleared for the step after next (the two histograms alternate).
 // ---------------------------------------------------------------------------------------------
 constexpr int kMapScanThreads = 1024;
-constexpr int kMapScanTile = kMapScanThreads * kScanItems;
+constexpr int kScanGroups = 1;  // groups of kScanItems consecutive bins per thread (2 measured slower)
+constexpr int kMapScanTile = kMapScanThreads * kScanGroups * kScanItems;
 // tile status word: flag (bits 62-63: 1 the tile's sum, 2 the sum up to and including the tile),
 // launch epoch (bits 32-61), value (bits 0-31)
 constexpr unsigned long long kTileSum = 1ull << 62, kTileInclusive = 2ull << 62;
@@ -121,10 +122,15 @@ __global__ void __launch_bounds__(kMapScanThreads) k_map_scan(Ctx c, int tiles, 
     const uint32_t epoch = sEpoch;
     const unsigned long long ep = (unsigned long long)epoch << 32;
     const int n = c.numBins, nCell = c.numCellBins;
-    const size_t base = size_t(tile) * kMapScanTile + size_t(tid) * kScanItems;
-    uint32_t v[8];
-    load8(c.count, c.startPrev, c.delta, nCell, base, n, v);
-    const uint32_t s = v[0] + v[1] + v[2] + v[3] + v[4] + v[5] + v[6] + v[7];
+    const size_t base0 = size_t(tile) * kMapScanTile + size_t(tid) * (kScanGroups * kScanItems);
+    uint32_t v[kScanGroups][kScanItems];
+#pragma unroll
+    for (int g = 0; g < kScanGroups; ++g) load8(c.count, c.startPrev, c.delta, nCell, base0 + g * kScanItems, n, v[g]);
+    uint32_t s = 0;
+#pragma unroll
+    for (int g = 0; g < kScanGroups; ++g)
+#pragma unroll
+        for (int k = 0; k < kScanItems; ++k) s += v[g][k];
     uint32_t total;
     const uint32_t off = block_exclusive_scan(s, total);
     if (tid == 0)
@@ -166,18 +172,22 @@ __global__ void __launch_bounds__(kMapScanThreads) k_map_scan(Ctx c, int tiles, 
         if (lane == 0) sPrefix = prefix;
     }
     // the next step's deltas and mover lists start empty
-    if (base + 8 <= size_t(nCell)) {
-        *reinterpret_cast<int4*>(c.deltaNext + base) = make_int4(0, 0, 0, 0);
-        *reinterpret_cast<int4*>(c.deltaNext + base + 4) = make_int4(0, 0, 0, 0);
-        *reinterpret_cast<int4*>(c.moverHeadNext + base) = make_int4(kNone, kNone, kNone, kNone);
-        *reinterpret_cast<int4*>(c.moverHeadNext + base + 4) = make_int4(kNone, kNone, kNone, kNone);
-    } else {
 #pragma unroll
-        for (int k = 0; k < 8; ++k)
-            if (base + k < size_t(nCell)) {
-                c.deltaNext[base + k] = 0;
-                c.moverHeadNext[base + k] = kNone;
-            }
+    for (int g = 0; g < kScanGroups; ++g) {
+        const size_t base = base0 + g * kScanItems;
+        if (base + 8 <= size_t(nCell)) {
+            *reinterpret_cast<int4*>(c.deltaNext + base) = make_int4(0, 0, 0, 0);
+            *reinterpret_cast<int4*>(c.deltaNext + base + 4) = make_int4(0, 0, 0, 0);
+            *reinterpret_cast<int4*>(c.moverHeadNext + base) = make_int4(kNone, kNone, kNone, kNone);
+            *reinterpret_cast<int4*>(c.moverHeadNext + base + 4) = make_int4(kNone, kNone, kNone, kNone);
+        } else {
+#pragma unroll
+            for (int k = 0; k < 8; ++k)
+                if (base + k < size_t(nCell)) {
+                    c.deltaNext[base + k] = 0;
+                    c.moverHeadNext[base + k] = kNone;
+                }
+        }
     }
     if (tile == 0) {
         if (open && tid == 0) {
@@ -189,21 +199,25 @@ __global__ void __launch_bounds__(kMapScanThreads) k_map_scan(Ctx c, int tiles, 
     }
     __syncthreads();
     uint32_t o = sPrefix + off;
-    uint32_t st8[8];
 #pragma unroll
-    for (int k = 0; k < 8; ++k) {
-        st8[k] = o;
-        o += v[k];
-    }
-    if (base + 8 <= size_t(n)) {
-        *reinterpret_cast<uint4*>(c.start + base) = make_uint4(st8[0], st8[1], st8[2], st8[3]);
-        *reinterpret_cast<uint4*>(c.start + base + 4) = make_uint4(st8[4], st8[5], st8[6], st8[7]);
-    } else {
+    for (int g = 0; g < kScanGroups; ++g) {
+        const size_t base = base0 + g * kScanItems;
+        uint32_t st8[8];
 #pragma unroll
-        for (int k = 0; k < 8; ++k)
-            if (base + k < size_t(n)) c.start[base + k] = st8[k];
+        for (int k = 0; k < 8; ++k) {
+            st8[k] = o;
+            o += v[g][k];
+        }
+        if (base + 8 <= size_t(n)) {
+            *reinterpret_cast<uint4*>(c.start + base) = make_uint4(st8[0], st8[1], st8[2], st8[3]);
+            *reinterpret_cast<uint4*>(c.start + base + 4) = make_uint4(st8[4], st8[5], st8[6], st8[7]);
+        } else {
+#pragma unroll
+            for (int k = 0; k < 8; ++k)
+                if (base + k < size_t(n)) c.start[base + k] = st8[k];
+        }
+        if (base <= size_t(n - 1) && size_t(n - 1) < base + 8) c.start[n] = o;
     }
-    if (base <= size_t(n - 1) && size_t(n - 1) < base + 8) c.start[n] = o;
     // the last ticket: every ticket is gone; reset the ticket and advance the epoch for the next
     // launch
     if (tile == tiles - 1 && tid == 0) {
@@ -217,47 +231,90 @@ __global__ void __launch_bounds__(kMapScanThreads) k_map_scan(Ctx c, int tiles, 
 // order, so a stayer's rank among them is its place in that bin minus the movers that left the bin
 // before it (marked lanes of the wave, and cells of the bin in earlier waves), and a mover from a
 // lower place precedes them all, one from a higher place follows them all; every cell then adds
-// the bin's movers with a smaller A index.
-__global__ void k_cell_scatter(Ctx c) {
-    int2 blk = c.cellBlocks[blockIdx.x];
-    const DevWorld w = c.worlds[blk.x];
-    const int i = blk.y + int(threadIdx.x);
+// the bin's movers with a smaller A index.  A workgroup places the cells of kScatterRuns entries of
+// the cell block table, one cell of each per thread, with the loads of each phase issued for all
+// of them together (the pass is a chain of dependent loads: bin, then the bin's offsets and mover
+// list, then the store).
+constexpr int kScatterRuns = 4;
+
+__global__ void k_cell_scatter(Ctx c, int tableBlocks) {
     const int lane = int(threadIdx.x) & 63;
-    const int i0 = i - lane;  // the wave's first cell
-    const bool active = i < w.cellOff + w.numCells;
-    const int bo = active ? c.binOf[i] : 0;
-    const bool mover = (uint32_t(bo) & kMoverBit) != 0u;
-    const unsigned long long movers = __ballot(active && mover);
-    if (!active) return;
-    const int gb = int(uint32_t(bo) & ~kMoverBit);
-    const uint32_t base = c.startPrev[w.binOff];
-    const int s = w.cellOff + int(c.startPrev[gb] - base), e = w.cellOff + int(c.startPrev[gb + 1] - base);
-    const int head = c.moverHead[gb];
-    const int dest0 = w.cellOff + int(c.start[gb] - c.start[w.binOff]);
-    int rank;
-    if (!mover) {
-        // a stayer: s <= i < e
-        const int from = max(s, i0) - i0;
-        const unsigned long long below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
-        const unsigned long long fromMask = from >= 64 ? 0ull : ~((1ull << from) - 1ull);
-        int left = __builtin_popcountll(movers & below & fromMask);
-        for (int j = s; j < i0; ++j) left += (uint32_t(c.binOf[j]) & kMoverBit) ? 1 : 0;
-        rank = (i - s) - left;
-    } else {
-        rank = 0;
-        if (i >= e)
-            for (int j = s; j < e; ++j) rank += c.binOf[j] == gb ? 1 : 0;
+    const unsigned long long below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    int i[kScatterRuns], bo[kScatterRuns], wi[kScatterRuns], cellOff[kScatterRuns], binOff[kScatterRuns];
+    bool active[kScatterRuns];
+#pragma unroll
+    for (int q = 0; q < kScatterRuns; ++q) {
+        const int e = int(blockIdx.x) * kScatterRuns + q;
+        const int2 blk = e < tableBlocks ? c.cellBlocks[e] : make_int2(0, 0);
+        const DevWorld& w = c.worlds[blk.x];
+        wi[q] = blk.x;
+        cellOff[q] = w.cellOff;
+        binOff[q] = w.binOff;
+        i[q] = blk.y + int(threadIdx.x);
+        active[q] = e < tableBlocks && i[q] < w.cellOff + w.numCells;
+        bo[q] = active[q] ? c.binOf[i[q]] : 0;
     }
-    for (int j = head; j != kNone; j = c.moverNext[j]) rank += j < i ? 1 : 0;
-    const int dest = dest0 + rank;
-    c.sorted[dest] = i;
-    // the row-staged collision pass reads a copy of the state in map order; the fluid pass reads
-    // the state through `sorted`
-    if (c.params[blk.x].motion != 0) {
-        c.mpos[dest] = c.A.pos[i];
-        c.mvel[dest] = c.A.vel[i];
-        c.mdens[dest] = c.A.dens[i];
-        c.mflags[dest] = c.A.flags[i];
+    int gb[kScatterRuns], s[kScatterRuns], e[kScatterRuns], head[kScatterRuns], dest0[kScatterRuns];
+    unsigned long long movers[kScatterRuns];
+#pragma unroll
+    for (int q = 0; q < kScatterRuns; ++q) {
+        const bool mover = (uint32_t(bo[q]) & kMoverBit) != 0u;
+        movers[q] = __ballot(active[q] && mover);
+        gb[q] = int(uint32_t(bo[q]) & ~kMoverBit);
+        const uint32_t spBase = c.startPrev[binOff[q]], stBase = c.start[binOff[q]];
+        const int g = active[q] ? gb[q] : binOff[q];
+        s[q] = cellOff[q] + int(c.startPrev[g] - spBase);
+        e[q] = cellOff[q] + int(c.startPrev[g + 1] - spBase);
+        head[q] = active[q] ? c.moverHead[g] : kNone;
+        dest0[q] = cellOff[q] + int(c.start[g] - stBase);
+    }
+    int rank[kScatterRuns];
+#pragma unroll
+    for (int q = 0; q < kScatterRuns; ++q) {
+        rank[q] = 0;
+        if (!active[q]) continue;
+        const int iq = i[q], i0 = iq - lane;  // the wave's first cell
+        if ((uint32_t(bo[q]) & kMoverBit) == 0u) {
+            // a stayer: s <= i < e
+            const int from = max(s[q], i0) - i0;
+            const unsigned long long fromMask = from >= 64 ? 0ull : ~((1ull << from) - 1ull);
+            int left = __builtin_popcountll(movers[q] & below & fromMask);
+            for (int j = s[q]; j < i0; ++j) left += (uint32_t(c.binOf[j]) & kMoverBit) ? 1 : 0;
+            rank[q] = (iq - s[q]) - left;
+        } else if (iq >= e[q]) {
+            for (int j = s[q]; j < e[q]; ++j) rank[q] += c.binOf[j] == gb[q] ? 1 : 0;
+        }
+    }
+    // the bins' mover lists, walked side by side (one load per list in flight per round)
+    int jw[kScatterRuns];
+#pragma unroll
+    for (int q = 0; q < kScatterRuns; ++q) jw[q] = active[q] ? head[q] : kNone;
+    for (;;) {
+        bool more = false;
+#pragma unroll
+        for (int q = 0; q < kScatterRuns; ++q)
+            if (jw[q] != kNone) {
+                rank[q] += jw[q] < i[q] ? 1 : 0;
+                more = true;
+            }
+        if (!more) break;
+#pragma unroll
+        for (int q = 0; q < kScatterRuns; ++q) jw[q] = jw[q] != kNone ? c.moverNext[jw[q]] : kNone;
+    }
+#pragma unroll
+    for (int q = 0; q < kScatterRuns; ++q) {
+        if (!active[q]) continue;
+        const int iq = i[q];
+        const int dest = dest0[q] + rank[q];
+        c.sorted[dest] = iq;
+        // the row-staged collision pass reads a copy of the state in map order; the fluid pass
+        // reads the state through `sorted`
+        if (c.params[wi[q]].motion != 0) {
+            c.mpos[dest] = c.A.pos[iq];
+            c.mvel[dest] = c.A.vel[iq];
+            c.mdens[dest] = c.A.dens[iq];
+            c.mflags[dest] = c.A.flags[iq];
+        }
     }
 }
 
@@ -372,7 +429,9 @@ void launch_map_scan(const Ctx& c, hipStream_t s, StepCounters* open, bool advan
 int map_scan_tiles(int numBins) { return (numBins + kMapScanTile - 1) / kMapScanTile; }
 
 void launch_cell_scatter(const Ctx& c, const LaunchShape& sh, hipStream_t s) {
-    if (sh.cellBlocks) hipLaunchKernelGGL(k_cell_scatter, dim3(sh.cellBlocks), dim3(sh.cellThreads), 0, s, c);
+    if (sh.cellBlocks)
+        hipLaunchKernelGGL(k_cell_scatter, dim3((sh.cellBlocks + kScatterRuns - 1) / kScatterRuns), dim3(sh.cellThreads),
+                           0, s, c, sh.cellBlocks);
 }
 
 void launch_part_scatter(const Ctx& c, const LaunchShape& sh, hipStream_t s) {

@@ -39,11 +39,12 @@ void launch_fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t s);
 int map_scan_tiles(int numBins);
 // bins of the current state (A) for the next sort; the step keeps them up to date itself
 void launch_seed_bins(const Ctx& c, const LaunchShape& sh, hipStream_t s);
-// neighbour-pass workgroups one CU keeps resident (fluid or collision variant)
-int neighbor_blocks_per_cu(bool fluid);
-// A second stream of the same priority and two events: the particle kernels that read only the
-// map and the particles (sort, movement, targets, fusion claims) run on it beside the neighbour
-// pass, joined before the particle kernels that read the cells' end-of-pass state.
+// workgroups of the row-streaming (collision) neighbour pass one CU keeps resident
+int neighbor_blocks_per_cu();
+// A second stream of the same priority and two events: the particle kernels run on it beside the
+// neighbour pass (sort, movement, targets and fusion claims always; fusions and absorptions too
+// when no world has bond work, so that nothing in the pass reads the energies they change),
+// joined before the kernels after the pass.
 struct SideStream {
     hipStream_t stream = nullptr;
     hipEvent_t fork = nullptr, join = nullptr;

@@ -790,13 +790,12 @@ void strip_shape(DevWorld& w, int strip, int rowsPerStage) {
     w.nty = w.nby / rowsPerStage;
 }
 
-// Rows per neighbour-pass chunk, one value for all worlds of a set (forcedRows if nonzero).  Tall
-// chunks stage fewer halo rows per computed row; short ones even out the work of the workgroups,
-// whose cell counts vary.  Measured on MI355X, the pass ran fastest with the tallest chunks that
-// still give at least minRounds workgroups per resident slot: 3.5 for the 4M-cell fluid world
-// (6 rows, 3 % faster than at 2 rounds), 5 for the 3.4M-cell soft-body world, whose bodies make
-// the workgroups' loads unequal (2 % faster than at 3.5); sets too small for that take the
-// shortest chunks.
+// Rows per chunk of the row-streaming (collision) pass, one value for all worlds of a set
+// (forcedRows if nonzero).  Tall chunks stage fewer halo rows per computed row; short ones even
+// out the work of the workgroups, whose cell counts vary.  Measured on MI355X, the pass ran fastest
+// with the tallest chunks that still give at least 5 workgroups per resident slot on the
+// 3.4M-cell soft-body world, whose bodies make the workgroups' loads unequal (2 % faster than at
+// 3.5); sets too small for that take the shortest chunks.
 void chunk_shape(std::vector<DevWorld>& worlds, const std::vector<DevParams>& params, int forcedRows) {
     static const int numCUs = [] {
         int dev = 0, n = 0;
@@ -804,14 +803,12 @@ void chunk_shape(std::vector<DevWorld>& worlds, const std::vector<DevParams>& pa
             return 256;
         return std::max(1, n);
     }();
-    bool fluid = false, collision = false;
-    for (size_t i = 0; i < worlds.size(); ++i) (params[i].motion == 0 ? fluid : collision) = true;
-    int perCU = 1 << 30;
-    if (fluid) perCU = std::min(perCU, neighbor_blocks_per_cu(true));
-    if (collision) perCU = std::min(perCU, neighbor_blocks_per_cu(false));
-    if (!fluid && !collision) perCU = 1;
+    // (only collision worlds run the row-streaming pass; the fluid pass is cell-parallel)
+    bool collision = false;
+    for (size_t i = 0; i < worlds.size(); ++i) collision = collision || params[i].motion != 0;
+    const int perCU = collision ? neighbor_blocks_per_cu() : 1;
     const int64_t slots = int64_t(perCU) * numCUs;
-    const double minRounds = collision ? 5.0 : 3.5;
+    const double minRounds = 5.0;
     int best = 2;
     // staged rows per chunk of a world for `rows` bin rows (a chunk covers at most kChunkMax bin
     // rows: the LDS histogram of its region)
@@ -820,8 +817,9 @@ void chunk_shape(std::vector<DevWorld>& worlds, const std::vector<DevParams>& pa
     };
     for (int rows = 2; rows <= kChunkMax; ++rows) {
         int64_t tiles = 0;
-        for (const DevWorld& w : worlds) {
-            if (w.numCells == 0) continue;
+        for (size_t i = 0; i < worlds.size(); ++i) {
+            const DevWorld& w = worlds[i];
+            if (w.numCells == 0 || params[i].motion == 0) continue;
             const int ty = stagedRows(w, rows), nyS = w.nby / w.rowsPerStage;
             tiles += int64_t(w.ntx) * ((nyS + ty - 1) / ty);
         }
@@ -865,7 +863,7 @@ void WorldSet::buildTables() {
     std::vector<int2> tiles;
     for (int wi = 0; wi < int(worlds_.size()); ++wi) {
         const DevWorld& w = worlds_[wi];
-        if (w.numCells == 0 || !active_[wi]) continue;
+        if (w.numCells == 0 || !active_[wi] || params_[wi].motion == 0) continue;
         for (int t = 0; t < w.ntx * w.nty; ++t) tiles.push_back(make_int2(wi, t));
     }
     std::vector<int2> tileTable;
@@ -1084,6 +1082,7 @@ void WorldSet::setParams(int w, const alien_params& p) {
         throw Error(ALIEN_E_STATE, "the bin grid changes with these parameters: build a fresh set around them");
     if (!livingWork_.at(w) && needs_fresh_set(hosts_.at(w).params, p))
         throw Error(ALIEN_E_STATE, "cellMinEnergy changes: build a fresh set around the new parameters");
+    const bool wasFluid = params_.at(w).motion == 0;
     hosts_.at(w).params = p;
     params_.at(w) = to_dev_params(p);
     check_hip(hipMemcpy(dParams_ + w, &params_[w], sizeof(DevParams), hipMemcpyHostToDevice), "H2D");
@@ -1092,7 +1091,15 @@ void WorldSet::setParams(int w, const alien_params& p) {
     const bool fluid = shape_.hasFluid, collision = shape_.hasCollision;
     shape_.hasFluid = shape_.hasCollision = false;
     for (const DevParams& q : params_) (q.motion == 0 ? shape_.hasFluid : shape_.hasCollision) = true;
-    if (fluid != shape_.hasFluid || collision != shape_.hasCollision) invalidateGraphs();
+    if (wasFluid != (params_[w].motion == 0)) {
+        // only collision worlds run the row-streaming pass: its chunks and tiles follow the switch
+        chunk_shape(worlds_, params_, chunkRows_);
+        check_hip(hipMemcpy(dWorlds_, worlds_.data(), sizeof(DevWorld) * worlds_.size(), hipMemcpyHostToDevice), "H2D");
+        buildTables();
+        invalidateGraphs();
+    } else if (fluid != shape_.hasFluid || collision != shape_.hasCollision) {
+        invalidateGraphs();
+    }
     updateInnerFriction();
 }
 
