@@ -425,9 +425,8 @@ struct RowShared {
     // the row in flight
     int nrs[kStageBins + 1];
     int nrm[kMaxSub][3];
-    uint8_t nebin[kRowCap];  // arrival index -> bin slot
+    uint8_t nebin[kRowCap];  // entry of the row -> bin slot
     union {
-        int naidx[kRowCap];  // while a row is committed: arrival index -> previous index
         struct {
             // while a row is computed, the pair records of the current window: the cell's pool
             // position, and the neighbour as bonded << 15 | ring row << 8 | pool position
@@ -438,7 +437,7 @@ struct RowShared {
 };
 
 // registers of a row in flight: bin counters of the bin slots lane, lane + 64, lane + 128, then
-// the entries lane and lane + 64 in arrival order
+// the entries lane and lane + 64 in map order
 template <int kSub>
 struct RowFlight {
     static constexpr int kSlots = kSub == 2 ? 3 : 2;  // bin slots per lane
@@ -571,8 +570,9 @@ __device__ __forceinline__ void row_issue_entries(const Ctx& c, const StripGeom&
     }
 }
 
-// C: the row into ring row r at pool position head, every entry at its rank by previous index
-// inside its bin (R1)
+// C: the row into ring row r at pool position head.  The row was read from the map copy bin by
+// bin, and the map holds every bin's cells by previous index (R1): an entry's place in the row is
+// its rank in its bin.
 template <int kSub, class RS>
 __device__ __forceinline__ void row_commit(const StripGeom& g, RS& S, const RowFlight<kSub>& F, int r, int head) {
     const int lane = threadIdx.x;
@@ -582,19 +582,10 @@ __device__ __forceinline__ void row_commit(const StripGeom& g, RS& S, const RowF
 #pragma unroll
     for (int q = 0; q < 2; ++q) {
         const int e = lane + q * kRowThreads;
-        if (e < F.total) S.u.naidx[e] = F.aidx[q];
-    }
-    __syncthreads();
-#pragma unroll
-    for (int q = 0; q < 2; ++q) {
-        const int e = lane + q * kRowThreads;
         if (e < F.total) {
             const int j = S.nebin[e];
-            const int s0 = S.nrs[j], s1 = S.nrs[j + 1];
             const int a = F.aidx[q];
-            int rank = 0;
-            for (int u = s0; u < s1; ++u) rank += S.u.naidx[u] < a ? 1 : 0;
-            const int pp = (head + s0 + rank) & kPoolMask;
+            const int pp = (head + e) & kPoolMask;
             S.pos[pp] = F.pos[q];
             if (pp < kPosMirror) S.pos[kPool + pp] = F.pos[q];
             S.vel[pp] = F.vel[q];
@@ -840,25 +831,16 @@ __device__ __forceinline__ void rows_direct(const Ctx& c, const DevWorld& w, con
                     hi = mid;
             }
             const int hx = lo + 1;
-            int bs, bn;
-            binRange(strip_bin(g, y, hx), bs, bn);
+            // (the map holds every bin's cells by previous index: a cell's logical index is its map
+            // position)
             const int i = c.sorted[m];
-            const Nb self = global_nb<kFluid>(c, i, bs + rank_in_bin(c.sorted, bs, bs + bn, i));
+            const Nb self = global_nb<kFluid>(c, i, m);
             cell_direct<kFluid>(c, w, P, wi, self, [&](auto&& visit) {
                 for (int dy = -1; dy <= 1; ++dy)
                     for (int dx = -1; dx <= 1; ++dx) {
                         int s0, n;
                         binRange(strip_bin(g, y + dy, hx + dx), s0, n);
-                        int last = -1;
-                        for (int t = 0; t < n; ++t) {
-                            int best = 0x7fffffff;
-                            for (int u = 0; u < n; ++u) {
-                                const int a = c.sorted[s0 + u];
-                                if (a > last && a < best) best = a;
-                            }
-                            last = best;
-                            visit(global_nb<kFluid>(c, best, s0 + t));
-                        }
+                        for (int t = 0; t < n; ++t) visit(global_nb<kFluid>(c, c.sorted[s0 + t], s0 + t));
                     }
             });
         }
@@ -891,7 +873,6 @@ __device__ __forceinline__ bool stage_fits(const Ctx& c, const StripGeom& g, int
 template <bool kFluid, int kSub, class RS>
 __device__ __forceinline__ bool chunk_rows(const Ctx& c, const DevWorld& w, const DevParams& P, int wi,
                                            StripGeom& g, RS& S, int y0, int IY) {
-    const int lane = threadIdx.x;
     const int nbyS = g.nby / kSub;
     // Away from the seam no staged bin wraps, and two stencil neighbours lie less than three bin
     // edges apart (the last column and row absorb less than one edge of remainder) while half the

@@ -46,14 +46,6 @@ __device__ __forceinline__ void part_bin_range(const Ctx& c, const DevWorld& w, 
     e = w.partOff + int(c.start[w.pbinOff + b + 1] - base);
 }
 
-// rank of entry value v among the entries of sorted[s, e): the cell's position in its bin under
-// the ascending-previous-index order (R1)
-__device__ __forceinline__ int rank_in_bin(const int* sorted, int s, int e, int v) {
-    int r = 0;
-    for (int t = s; t < e; ++t) r += sorted[t] < v ? 1 : 0;
-    return r;
-}
-
 // cell bin of a position (global bin index), and its bin coordinates
 __device__ __forceinline__ void cell_bin_xy(const DevWorld& w, f2 p, int& bx, int& by) {
     int sx, sy;
