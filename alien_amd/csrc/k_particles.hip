@@ -12,42 +12,35 @@
 
 namespace alien {
 
-__global__ void k_part_move(Ctx c) {
-    for_live_particles(c, [&](int wi, const DevWorld& w, int k) {
-        const DevParams& P = c.params[wi];
-        const int src = c.psorted[k];
-        f2 p = to_f2(c.A.ppos[src]);
-        const f2 v = to_f2(c.A.pvel[src]);
-        p = torus_pos(add(p, mul(v, P.dt)), w.W, w.H);
-        c.B.ppos[k] = to_float2(p);
-        c.B.pvel[k] = to_float2(v);
-        c.B.pen[k] = c.A.pen[src];
-        c.B.plast[k] = c.A.plast[src];
-        c.B.pslot[k] = c.A.pslot[src];
-        c.B.pflags[k] = c.A.pflags[src];
-    });
+// The particle map winner of the slot of position p (R2): the first particle of the bin whose
+// pre-move slot matches; particle k fuses into it when it is another particle closer than sqrt(0.5)
+// after the move.  Returns the target or -1.
+__device__ __forceinline__ int particle_target(const Ctx& c, const DevWorld& w, int k, f2 p) {
+    int sx, sy;
+    slot_of(p, w.W, w.H, sx, sy);
+    int s, e;
+    part_bin_range(c, w, part_bin_local(w, sx, sy), s, e);
+    int t = -1;
+    for (int m = s; m < e; ++m) {
+        int qx, qy;
+        slot_of(to_f2(c.A.ppos[c.psorted[m]]), w.W, w.H, qx, qy);
+        if (qx == sx && qy == sy) {
+            t = m;
+            break;
+        }
+    }
+    return t >= 0 && t != k && lensq(sub(p, to_f2(c.B.ppos[t]))) < 0.5f ? t : -1;
 }
 
+// Targets and fusion claims in one pass: a particle claims its target when the target does not
+// fuse itself, which it finds by computing the target's own target (the same function of the same
+// moved positions; fusing particles are rare).
 __global__ void k_part_target(Ctx c) {
     for_live_particles(c, [&](int, const DevWorld& w, int k) {
         const f2 p = to_f2(c.B.ppos[k]);
-        int sx, sy;
-        slot_of(p, w.W, w.H, sx, sy);
-        // particle map winner of the slot (R2): first particle of the bin whose pre-move slot matches
-        int s, e;
-        part_bin_range(c, w, part_bin_local(w, sx, sy), s, e);
-        int t = -1;
-        for (int m = s; m < e; ++m) {
-            int qx, qy;
-            slot_of(to_f2(c.A.ppos[c.psorted[m]]), w.W, w.H, qx, qy);
-            if (qx == sx && qy == sy) {
-                t = m;
-                break;
-            }
-        }
-        int want = -1;
-        if (t >= 0 && t != k && lensq(sub(p, to_f2(c.B.ppos[t]))) < 0.5f) want = t;
+        const int want = particle_target(c, w, k, p);
         c.want[k] = want;
+        if (want >= 0 && particle_target(c, w, want, to_f2(c.B.ppos[want])) < 0) atomicMin(&c.fclaim[want], k);
         // cell hit this step: lowest-index cell of the slot at map-build time (cellMap.getFirst),
         // the first of the bin's cells in map order (a bin holds its cells by previous index)
         int cell = -1;
@@ -67,13 +60,6 @@ __global__ void k_part_target(Ctx c) {
             }
         }
         c.absorbCell[k] = cell;
-    });
-}
-
-__global__ void k_part_claim(Ctx c) {
-    for_live_particles(c, [&](int, const DevWorld&, int k) {
-        const int t = c.want[k];
-        if (t >= 0 && c.want[t] < 0) atomicMin(&c.fclaim[t], k);
     });
 }
 
@@ -154,16 +140,11 @@ __global__ void k_part_absorb(Ctx c) {
     });
 }
 
-// movement, targets and fusion claims: the map and the particles only
+// targets and fusion claims: the moved particles and the map
 void launch_particles_pre(const Ctx& c, const LaunchShape& sh, hipStream_t s, Recorder* rec) {
     if (!sh.partBlocks || !sh.hasParticles) return;
-    const dim3 g(sh.partGrid), b(sh.partThreads);
-    if (rec) rec->mark("part_move");
-    hipLaunchKernelGGL(k_part_move, g, b, 0, s, c);
     if (rec) rec->mark("part_target");
-    hipLaunchKernelGGL(k_part_target, g, b, 0, s, c);
-    if (rec) rec->mark("part_claim");
-    hipLaunchKernelGGL(k_part_claim, g, b, 0, s, c);
+    hipLaunchKernelGGL(k_part_target, dim3(sh.partGrid), dim3(sh.partThreads), 0, s, c);
 }
 
 // fusions, reflections and absorptions: also the cells' end-of-pass state

@@ -308,7 +308,7 @@ __global__ void k_cell_scatter(Ctx c, int tableBlocks) {
         const int dest = dest0[q] + rank[q];
         c.sorted[dest] = iq;
         // the row-staged collision pass reads a copy of the state in map order; the fluid pass
-        // reads the state through `sorted`
+        // reads the state through `sorted` (a copy for it costs the scatter more than it saves)
         if (c.params[wi[q]].motion != 0) {
             c.mpos[dest] = c.A.pos[iq];
             c.mvel[dest] = c.A.vel[iq];
@@ -382,7 +382,9 @@ __global__ void k_cell_binsort(Ctx c) {
     sort_bin<true>(c.sorted, s, e, c.newidx);
 }
 
-// particle map in logical order; also publishes the new number of live particles per world
+// Particle map in logical order, and the movement of the bin's particles (RadiationProcessor.cuh:
+// 61-70: position += velocity * dt on the torus) into the end-of-step arrays at their map
+// positions; also publishes the new number of live particles per world.
 __global__ void k_part_binsort(Ctx c) {
     if (blockIdx.x == 0) {
         for (int wi = threadIdx.x; wi < c.numWorlds; wi += blockDim.x) {
@@ -399,6 +401,17 @@ __global__ void k_part_binsort(Ctx c) {
     int s, e;
     part_bin_range(c, w, gb - w.pbinOff, s, e);
     sort_bin<false>(c.psorted, s, e, nullptr);
+    const float dt = c.params[blk.x].dt;
+    for (int k = s; k < e; ++k) {
+        const int src = c.psorted[k];
+        const f2 v = to_f2(c.A.pvel[src]);
+        c.B.ppos[k] = to_float2(torus_pos(add(to_f2(c.A.ppos[src]), mul(v, dt)), w.W, w.H));
+        c.B.pvel[k] = to_float2(v);
+        c.B.pen[k] = c.A.pen[src];
+        c.B.plast[k] = c.A.plast[src];
+        c.B.pslot[k] = c.A.pslot[src];
+        c.B.pflags[k] = c.A.pflags[src];
+    }
 }
 
 // ---------------------------------------------------------------------------------------------
