@@ -4,6 +4,7 @@
 #include <pybind11/stl.h>
 
 #include <cstring>
+#include <sstream>
 
 #include "simulation_facade.hpp"
 
@@ -124,6 +125,68 @@ PYBIND11_MODULE(_facade, m) {
         DescriptionConverter::toTO(d, to, aux);
         return DescriptionConverter::toClusteredDataDescription(to, aux);
     });
+    // statistics history (StatisticsHistory.h, DataPointCollection.h) and its .statistics.csv form
+    py::class_<DataPoint>(m, "DataPoint")
+        .def(py::init<>())
+        .def_property(
+            "values", [](DataPoint const& d) { return std::vector<double>(d.values, d.values + kStatColors); },
+            [](DataPoint& d, std::vector<double> const& v) {
+                if (v.size() != size_t(kStatColors)) throw std::invalid_argument("seven values, one per colour");
+                for (int i = 0; i < kStatColors; ++i) d.values[i] = v[size_t(i)];
+            })
+        .def_readwrite("summedValues", &DataPoint::summedValues);
+    py::class_<DataPointCollection>(m, "DataPointCollection")
+        .def(py::init<>())
+        .def_readwrite("time", &DataPointCollection::time)
+        .def_readwrite("systemClock", &DataPointCollection::systemClock)
+        .def_readwrite("numCells", &DataPointCollection::numCells)
+        .def_readwrite("numSelfReplicators", &DataPointCollection::numSelfReplicators)
+        .def_readwrite("numColonies", &DataPointCollection::numColonies)
+        .def_readwrite("numViruses", &DataPointCollection::numViruses)
+        .def_readwrite("numFreeCells", &DataPointCollection::numFreeCells)
+        .def_readwrite("numParticles", &DataPointCollection::numParticles)
+        .def_readwrite("averageGenomeCells", &DataPointCollection::averageGenomeCells)
+        .def_readwrite("averageGenomeComplexity", &DataPointCollection::averageGenomeComplexity)
+        .def_readwrite("varianceGenomeComplexity", &DataPointCollection::varianceGenomeComplexity)
+        .def_readwrite("maxGenomeComplexityOfColonies", &DataPointCollection::maxGenomeComplexityOfColonies)
+        .def_readwrite("totalEnergy", &DataPointCollection::totalEnergy)
+        .def_readwrite("numCreatedCells", &DataPointCollection::numCreatedCells)
+        .def_readwrite("numAttacks", &DataPointCollection::numAttacks)
+        .def_readwrite("numMuscleActivities", &DataPointCollection::numMuscleActivities)
+        .def_readwrite("numDefenderActivities", &DataPointCollection::numDefenderActivities)
+        .def_readwrite("numTransmitterActivities", &DataPointCollection::numTransmitterActivities)
+        .def_readwrite("numDetonations", &DataPointCollection::numDetonations);
+    m.def("serializeStatistics", [](StatisticsHistoryData const& h) {
+        std::ostringstream out;
+        serializeStatistics(h, out);
+        return out.str();
+    });
+    m.def("deserializeStatistics", [](std::string const& text) {
+        std::istringstream in(text);
+        return deserializeStatistics(in);
+    });
+    // the history sampling alone, fed with given per-colour statistics (test hook of the host logic)
+    py::class_<StatisticsHistory>(m, "StatisticsHistory")
+        .def(py::init<>())
+        .def("addDataPoint",
+             [](StatisticsHistory& h, std::vector<int> cells, std::vector<int> freeCells, std::vector<int> particles,
+                std::vector<double> energy, uint64_t timestep, double clock) {
+                 if (cells.size() != 7 || freeCells.size() != 7 || particles.size() != 7 || energy.size() != 7)
+                     throw std::invalid_argument("seven values per colour");
+                 TimestepStatistics raw;
+                 for (int i = 0; i < 7; ++i) {
+                     raw.device.numCells[i] = cells[size_t(i)];
+                     raw.numFreeCells[i] = freeCells[size_t(i)];
+                     raw.device.numParticles[i] = particles[size_t(i)];
+                     raw.device.totalEnergy[i] = energy[size_t(i)];
+                 }
+                 raw.device.timestep = timestep;
+                 h.addDataPoint(raw, timestep, clock);
+             })
+        .def("resetTime", &StatisticsHistory::resetTime)
+        .def("rewrite", &StatisticsHistory::rewrite)
+        .def("data", &StatisticsHistory::data)
+        .def("longtermTimestepDelta", &StatisticsHistory::longtermTimestepDelta);
     py::register_exception<alien::MemoryAllocationException>(m, "MemoryAllocationException", PyExc_MemoryError);
     py::class_<SimulationFacade>(m, "SimulationFacade")
         .def(py::init<>())
@@ -183,6 +246,8 @@ PYBIND11_MODULE(_facade, m) {
              })
         .def("getGpuName", &SimulationFacade::getGpuName)
         .def("getTps", &SimulationFacade::getTps)
+        .def("getStatisticsHistory", &SimulationFacade::getStatisticsHistory)
+        .def("setStatisticsHistory", &SimulationFacade::setStatisticsHistory, py::call_guard<py::gil_scoped_release>())
         .def("getStatistics",
              [](SimulationFacade& f) {
                  alien_statistics s;

@@ -3,6 +3,8 @@
 
 #include <cstring>
 
+#include "../../include/alien_hip.h"
+
 namespace alien {
 
 void SimulationFacade::check(int rc) {
@@ -63,6 +65,7 @@ void SimulationFacade::workerLoop() {
             check(alien_step(_world, 1));
             ++stepsSince;
             auto now = std::chrono::steady_clock::now();
+            if (!_lastStatisticsUpdate || now - *_lastStatisticsUpdate > std::chrono::milliseconds(30)) updateStatistics();
             double dt = std::chrono::duration<double>(now - last).count();
             if (dt > 0.5) {
                 _tps = float(double(stepsSince) / dt);
@@ -101,6 +104,12 @@ void SimulationFacade::newSimulation(uint64_t timestep, GeneralSettings const& g
     check(alien_set_timestep(_world, timestep));
     _settings = generalSettings;
     _params = parameters;
+    {
+        std::lock_guard<std::mutex> lk(_historyMutex);
+        _history = StatisticsHistory{};
+    }
+    _lastStatisticsUpdate.reset();
+    for (int k = 0; k < kStatColors; ++k) _numFreeCells[k] = 0;
     _stop = false;
     _running = false;
     _workerError = nullptr;
@@ -149,8 +158,40 @@ SimulationData SimulationFacade::readTO() {
 
 void SimulationFacade::writeTO(SimulationData const& data) {
     check(alien_set_data(_world, data.cells.data(), data.cells.size(), data.particles.data(), data.particles.size()));
-    std::lock_guard<std::mutex> lk(_stateMutex);
-    _workerError = nullptr;  // fresh data makes the engine usable again
+    {
+        std::lock_guard<std::mutex> lk(_stateMutex);
+        _workerError = nullptr;  // fresh data makes the engine usable again
+    }
+    // free cells (StatisticsKernels.cu:19-21: mutation id MutationIdForFreeCell = 1); no kernel of
+    // this path changes a cell's colour or mutation id, so the counts hold until the next set
+    for (int k = 0; k < kStatColors; ++k) _numFreeCells[k] = 0;
+    for (auto const& c : data.cells)
+        if (c.mutationId == 1 && c.color < kStatColors) ++_numFreeCells[c.color];
+    updateStatistics();
+}
+
+void SimulationFacade::updateStatistics() {
+    TimestepStatistics raw;
+    check(alien_get_statistics(_world, &raw.device));
+    for (int k = 0; k < kStatColors; ++k) raw.numFreeCells[k] = _numFreeCells[k];
+    const double clock = double(std::chrono::duration_cast<std::chrono::seconds>(
+                                    std::chrono::system_clock::now().time_since_epoch())
+                                    .count());
+    _lastStatisticsUpdate = std::chrono::steady_clock::now();
+    std::lock_guard<std::mutex> lk(_historyMutex);
+    _history.addDataPoint(raw, raw.device.timestep, clock);
+}
+
+StatisticsHistoryData SimulationFacade::getStatisticsHistory() const {
+    std::lock_guard<std::mutex> lk(_historyMutex);
+    return _history.data();
+}
+
+void SimulationFacade::setStatisticsHistory(StatisticsHistoryData const& data) {
+    Guard g(*this);
+    const uint64_t t = alien_get_timestep(_world);
+    std::lock_guard<std::mutex> lk(_historyMutex);
+    _history.rewrite(data, t);
 }
 
 SimulationData SimulationFacade::getSimulationTO() {
@@ -218,6 +259,7 @@ void SimulationFacade::calcTimesteps(uint64_t timesteps) {
     try {
         if (auto m = takeInjectedFailure()) throw std::runtime_error(*m);
         check(alien_step(_world, timesteps));
+        updateStatistics();  // (forceUpdateStatistics of calcTimesteps)
     } catch (...) {
         // a failed step leaves the engine unusable until new data is set (EngineWorker.cpp:472-475)
         std::lock_guard<std::mutex> lk(_stateMutex);

@@ -23,6 +23,7 @@
 
 #include "../../include/alien_hip.h"
 #include "descriptions.hpp"
+#include "statistics_history.hpp"
 
 namespace alien {
 
@@ -83,6 +84,11 @@ public:
     // per-colour counts and energies (the TimestepStatistics colour vectors of getRawStatistics)
     alien_statistics getStatistics();
     float getTps() const;
+    // getStatisticsHistory / setStatisticsHistory (SimulationFacade.h:99-100): the history is
+    // sampled like _SimulationCudaFacade::updateStatistics (SimulationCudaFacade.cu:129-160,
+    // 250-280): after data is set, after calcTimesteps, and every 30 ms while running
+    StatisticsHistoryData getStatisticsHistory() const;
+    void setStatisticsHistory(StatisticsHistoryData const& data);
 
     // test hooks: make the worker thread fail on its next step (failure detection path), and set
     // the guard timeout
@@ -98,6 +104,8 @@ private:
 
     SimulationData readTO();
     void writeTO(SimulationData const& data);
+    // one data point of the statistics history from the device's counts (caller owns the world)
+    void updateStatistics();
 
     alien_world* _world = nullptr;
     AuxStore _aux;  // pass-through data of the cells, by id
@@ -115,6 +123,11 @@ private:
     std::optional<std::string> _injectedFailure;
     std::chrono::milliseconds _guardTimeout{7000};
     std::atomic<float> _tps{0.0f};
+
+    mutable std::mutex _historyMutex;
+    StatisticsHistory _history;
+    int32_t _numFreeCells[kStatColors] = {0, 0, 0, 0, 0, 0, 0};  // per colour, constant on this path
+    std::optional<std::chrono::steady_clock::time_point> _lastStatisticsUpdate;
 };
 
 }  // namespace alien
