@@ -29,26 +29,29 @@ void launch_step(const Ctx& c, const LaunchShape& sh, hipStream_t s, Recorder* r
     if (rec) rec->mark("cell_scatter");
     launch_cell_scatter(c, sh, s);
     // the particle kernels before the fusions and absorptions read the map and the particles
-    // only: with a side stream they run beside the neighbour pass
+    // only: with a side stream they run beside the neighbour pass.  The pass is enqueued first, so
+    // that it starts right after the scatter.
     const bool fork = side && !rec && sh.hasParticles && sh.partBlocks;
-    hipStream_t ps = s;
-    if (fork) {
-        (void)hipEventRecord(side->fork, s);
-        (void)hipStreamWaitEvent(side->stream, side->fork, 0);
-        ps = side->stream;
-    }
-    if (rec && sh.hasParticles) rec->mark("part_sort");
-    launch_part_sort(c, sh, ps);
-    launch_particles_pre(c, sh, ps, rec);
     // Absorption adds to cell energies, which the neighbour pass reads for fusion and the bond
     // kernels for bond decay: without bond work (no bonded, barrier or bondable cell) neither
     // reads them, and the fusions and absorptions run beside the neighbour pass as well.
     const bool allBeside = fork && !sh.hasBondWork;
-    if (allBeside) launch_particles_post(c, sh, ps, rec);
-    if (fork) (void)hipEventRecord(side->join, ps);
-    if (rec) rec->mark("neighbor");
-    launch_neighbor(c, sh, s, sh.hasFluid, sh.hasCollision);
-    if (fork) (void)hipStreamWaitEvent(s, side->join, 0);
+    if (fork) {
+        (void)hipEventRecord(side->fork, s);
+        launch_neighbor(c, sh, s, sh.hasFluid, sh.hasCollision);
+        (void)hipStreamWaitEvent(side->stream, side->fork, 0);
+        launch_part_sort(c, sh, side->stream);
+        launch_particles_pre(c, sh, side->stream, rec);
+        if (allBeside) launch_particles_post(c, sh, side->stream, rec);
+        (void)hipEventRecord(side->join, side->stream);
+        (void)hipStreamWaitEvent(s, side->join, 0);
+    } else {
+        if (rec && sh.hasParticles) rec->mark("part_sort");
+        launch_part_sort(c, sh, s);
+        launch_particles_pre(c, sh, s, rec);
+        if (rec) rec->mark("neighbor");
+        launch_neighbor(c, sh, s, sh.hasFluid, sh.hasCollision);
+    }
     if (!allBeside) launch_particles_post(c, sh, s, rec);
     launch_bonds(c, sh, s, rec);
     if (rec) rec->mark(nullptr);

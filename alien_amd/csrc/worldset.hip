@@ -398,11 +398,12 @@ void WorldSet::freeAll() {
 }
 
 void WorldSet::invalidateGraphs() {
-    for (auto& g : graph_)
-        if (g) {
-            (void)hipGraphExecDestroy(g);
-            g = nullptr;
-        }
+    for (auto* set : {graph_, graphMulti_})
+        for (int p = 0; p < 2; ++p)
+            if (set[p]) {
+                (void)hipGraphExecDestroy(set[p]);
+                set[p] = nullptr;
+            }
 }
 
 void WorldSet::synchronize() { check_hip(hipStreamSynchronize(stream_), "hipStreamSynchronize"); }
@@ -1144,28 +1145,42 @@ void WorldSet::step(uint64_t n) {
                         stream_);
         densityPending_[w] = 0;
     }
-    for (uint64_t s = 0; s < n; ++s) {
+    // the captured graph of `count` steps from parity p; all four are built together on first
+    // use, so that no later call pays for a capture
+    auto build = [&](hipGraphExec_t& g, int p, uint64_t count) {
+        if (g) return;
+        hipGraph_t graph = nullptr;
+        check_hip(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+        for (uint64_t k = 0; k < count; ++k) launch_step(makeCtx(p ^ int(k & 1)), shape_, stream_, nullptr, &side_);
+        check_hip(hipStreamEndCapture(stream_, &graph), "hipStreamEndCapture");
+        check_hip(hipGraphInstantiate(&g, graph, nullptr, nullptr, 0), "hipGraphInstantiate");
+        (void)hipGraphDestroy(graph);
+    };
+    auto graphOf = [&](uint64_t count) -> hipGraphExec_t {
+        for (int p = 0; p < 2; ++p) {
+            build(graph_[p], p, 1);
+            build(graphMulti_[p], p, kStepsPerGraph);
+        }
+        return (count == 1 ? graph_ : graphMulti_)[parity_];
+    };
+    for (uint64_t s = 0; s < n;) {
         // while particles live, every 32 steps: once none is left, the particle kernels go
         if (shape_.hasParticles && s > 0 && s % 32 == 0) dropParticlesIfNone();
+        uint64_t done = 1;
         if (timing_) {
             launch_step(makeCtx(parity_), shape_, stream_, recorder_.get());
             check_hip(hipGetLastError(), "kernel launch");
         } else if (useGraph_) {
-            hipGraphExec_t& g = graph_[parity_];
-            if (!g) {
-                hipGraph_t graph = nullptr;
-                check_hip(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
-                launch_step(makeCtx(parity_), shape_, stream_, nullptr, &side_);
-                check_hip(hipStreamEndCapture(stream_, &graph), "hipStreamEndCapture");
-                check_hip(hipGraphInstantiate(&g, graph, nullptr, nullptr, 0), "hipGraphInstantiate");
-                (void)hipGraphDestroy(graph);
-            }
-            check_hip(hipGraphLaunch(g, stream_), "hipGraphLaunch");
+            // several steps per graph while they fit before the end and the next particle check
+            const uint64_t toCheck = shape_.hasParticles ? 32 - s % 32 : n - s;
+            if (n - s >= kStepsPerGraph && toCheck >= kStepsPerGraph) done = kStepsPerGraph;
+            check_hip(hipGraphLaunch(graphOf(done), stream_), "hipGraphLaunch");
         } else {
             launch_step(makeCtx(parity_), shape_, stream_, nullptr, &side_);
             check_hip(hipGetLastError(), "kernel launch");
         }
-        parity_ ^= 1;
+        if (done & 1) parity_ ^= 1;
+        s += done;
     }
     for (size_t i = 0; i < hosts_.size(); ++i)
         if (active_[i]) hosts_[i].stepsSinceSet += n;  // (an inactive world is frozen)

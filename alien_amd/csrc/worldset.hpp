@@ -237,7 +237,11 @@ private:
     void updateInnerFriction();
     std::vector<Arr> tableAllocs_;
 
-    hipGraphExec_t graph_[2] = {nullptr, nullptr};
+    hipGraphExec_t graph_[2] = {nullptr, nullptr};  // one step from parity 0 / 1
+    // kStepsPerGraph steps from parity 0 / 1 (an even count: the parity is the same after them);
+    // consecutive graph launches leave the device idle for ~20 us between them
+    static constexpr uint64_t kStepsPerGraph = 4;
+    hipGraphExec_t graphMulti_[2] = {nullptr, nullptr};
 
     class EventRecorder;
     std::unique_ptr<EventRecorder> recorder_;

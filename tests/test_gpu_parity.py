@@ -484,6 +484,42 @@ def test_cpp_facade_end_to_end():
     f.closeSimulation()
 
 
+def test_cpp_facade_statistics_history():
+    """getStatisticsHistory / setStatisticsHistory of the facade (SimulationFacade.h:99-100): the
+    history holds a sample after the data is set and after calcTimesteps, whose per-colour cells,
+    energy particles and total energy are the device statistics and whose free cells are the cells
+    of mutation id 1; it survives the .statistics.csv round trip and a rewrite."""
+    from alien_amd import _facade as F
+
+    w = worlds.random_mix(41, motion=MOTION_FLUID, size=(96, 96), num_free=300, num_particles=80)
+    cells = w.cells.copy()
+    cells["mutationId"][::3] = 1
+    f = F.SimulationFacade()
+    f.newSimulation(0, w.size[0], w.size[1], bytes(w.params))
+    f.setSimulationData(cells, w.particles)
+    h0 = f.getStatisticsHistory()
+    assert len(h0) == 1 and h0[0].time == 0.0
+    f.calcTimesteps(25)
+    h = f.getStatisticsHistory()
+    assert len(h) == 2 and h[-1].time == 25.0
+    s = f.getStatistics()
+    last = h[-1]
+    assert last.numCells.values == [float(v) for v in s["numCells"]]
+    assert last.numParticles.values == [float(v) for v in s["numParticles"]]
+    assert last.totalEnergy.values == [float(v) for v in s["totalEnergy"]]
+    free = [0] * 7
+    for c in cells[cells["mutationId"] == 1]:
+        free[int(c["color"])] += 1
+    assert last.numFreeCells.values == [float(v) for v in free]
+    assert last.numFreeCells.summedValues == float(sum(free))
+    assert last.numSelfReplicators.summedValues == 0.0 and last.systemClock > 1.6e9
+    text = F.serializeStatistics(h)
+    assert F.serializeStatistics(F.deserializeStatistics(text)) == text
+    f.setStatisticsHistory(F.deserializeStatistics(text)[:1])
+    assert [p.time for p in f.getStatisticsHistory()] == [0.0]
+    f.closeSimulation()
+
+
 def test_cpp_facade_clustered_descriptions():
     """setClusteredSimulationData / getClusteredSimulationData / addAndSelectSimulationData of the
     facade (SimulationFacade.h:39-47): the same step results as the oracle on the same transfer
