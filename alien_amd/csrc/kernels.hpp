@@ -47,7 +47,7 @@ int neighbor_blocks_per_cu();
 // joined before the kernels after the pass.
 struct SideStream {
     hipStream_t stream = nullptr;
-    hipEvent_t fork = nullptr, join = nullptr;
+    hipEvent_t fork = nullptr, mapped = nullptr, join = nullptr;  // after the scan, the scatter, the particles
 };
 // rec != nullptr (per-kernel timing) launches everything on s, one kernel after another
 void launch_step(const Ctx& c, const LaunchShape& sh, hipStream_t s, Recorder* rec = nullptr,

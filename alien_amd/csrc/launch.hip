@@ -26,11 +26,9 @@ void launch_step(const Ctx& c, const LaunchShape& sh, hipStream_t s, Recorder* r
     // it
     if (rec) rec->mark("map_scan");
     launch_map_scan(c, s, c.sc, !sh.hasBondWork);
-    if (rec) rec->mark("cell_scatter");
-    launch_cell_scatter(c, sh, s);
-    // the particle kernels before the fusions and absorptions read the map and the particles
-    // only: with a side stream they run beside the neighbour pass.  The pass is enqueued first, so
-    // that it starts right after the scatter.
+    // The particle kernels run beside the cell scatter and the neighbour pass: their sort and
+    // movement need only the scan (the particle bins' offsets), their targets and fusion claims
+    // also the cell map.
     const bool fork = side && !rec && sh.hasParticles && sh.partBlocks;
     // Absorption adds to cell energies, which the neighbour pass reads for fusion and the bond
     // kernels for bond decay: without bond work (no bonded, barrier or bondable cell) neither
@@ -38,14 +36,20 @@ void launch_step(const Ctx& c, const LaunchShape& sh, hipStream_t s, Recorder* r
     const bool allBeside = fork && !sh.hasBondWork;
     if (fork) {
         (void)hipEventRecord(side->fork, s);
+        if (rec) rec->mark("cell_scatter");
+        launch_cell_scatter(c, sh, s);
+        (void)hipEventRecord(side->mapped, s);
         launch_neighbor(c, sh, s, sh.hasFluid, sh.hasCollision);
         (void)hipStreamWaitEvent(side->stream, side->fork, 0);
         launch_part_sort(c, sh, side->stream);
+        (void)hipStreamWaitEvent(side->stream, side->mapped, 0);
         launch_particles_pre(c, sh, side->stream, rec);
         if (allBeside) launch_particles_post(c, sh, side->stream, rec);
         (void)hipEventRecord(side->join, side->stream);
         (void)hipStreamWaitEvent(s, side->join, 0);
     } else {
+        if (rec) rec->mark("cell_scatter");
+        launch_cell_scatter(c, sh, s);
         if (rec && sh.hasParticles) rec->mark("part_sort");
         launch_part_sort(c, sh, s);
         launch_particles_pre(c, sh, s, rec);

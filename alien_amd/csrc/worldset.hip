@@ -365,7 +365,10 @@ WorldSet::WorldSet() {
     check_hip(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
     check_hip(hipStreamCreateWithFlags(&side_.stream, hipStreamNonBlocking), "hipStreamCreate");
     check_hip(hipEventCreateWithFlags(&side_.fork, hipEventDisableTiming), "hipEventCreate");
+    check_hip(hipEventCreateWithFlags(&side_.mapped, hipEventDisableTiming), "hipEventCreate");
     check_hip(hipEventCreateWithFlags(&side_.join, hipEventDisableTiming), "hipEventCreate");
+    check_hip(hipEventCreateWithFlags(&liveReady_, hipEventDisableTiming), "hipEventCreate");
+    check_hip(hipHostMalloc(reinterpret_cast<void**>(&liveHost_), sizeof(int), hipHostMallocDefault), "hipHostMalloc");
 }
 
 WorldSet::~WorldSet() {
@@ -375,7 +378,10 @@ WorldSet::~WorldSet() {
     freeAll();
     for (auto& a : tableAllocs_) (void)hipFree(a.p);
     tableAllocs_.clear();
+    if (liveReady_) (void)hipEventDestroy(liveReady_);
+    if (liveHost_) (void)hipHostFree(liveHost_);
     if (side_.fork) (void)hipEventDestroy(side_.fork);
+    if (side_.mapped) (void)hipEventDestroy(side_.mapped);
     if (side_.join) (void)hipEventDestroy(side_.join);
     if (side_.stream) (void)hipStreamDestroy(side_.stream);
     if (stream_) (void)hipStreamDestroy(stream_);
@@ -1164,8 +1170,9 @@ void WorldSet::step(uint64_t n) {
         return (count == 1 ? graph_ : graphMulti_)[parity_];
     };
     for (uint64_t s = 0; s < n;) {
-        // while particles live, every 32 steps: once none is left, the particle kernels go
-        if (shape_.hasParticles && s > 0 && s % 32 == 0) dropParticlesIfNone();
+        // while particles live, every 32 steps their count is read without waiting for the device:
+        // once none is left (seen a check later), the particle kernels go
+        if (shape_.hasParticles && s > 0 && s % 32 == 0) checkParticlesAsync();
         uint64_t done = 1;
         if (timing_) {
             launch_step(makeCtx(parity_), shape_, stream_, recorder_.get());
@@ -1190,8 +1197,33 @@ void WorldSet::step(uint64_t n) {
     if (shape_.hasParticles) dropParticlesIfNone();
 }
 
+void WorldSet::checkParticlesAsync() {
+    if (livePending_) {
+        const hipError_t q = hipEventQuery(liveReady_);
+        if (q == hipSuccess) {
+            livePending_ = false;
+            if (*liveHost_ == 0) {
+                shape_.hasParticles = false;
+                invalidateGraphs();
+                return;
+            }
+        } else if (q == hipErrorNotReady) {
+            (void)hipGetLastError();  // (not ready yet: not an error of the launches)
+        } else {
+            check_hip(q, "hipEventQuery");
+        }
+    }
+    if (!livePending_) {
+        check_hip(hipMemcpyAsync(liveHost_, &dSc_->liveParticles, sizeof(int), hipMemcpyDeviceToHost, stream_),
+                  "hipMemcpyAsync");
+        check_hip(hipEventRecord(liveReady_, stream_), "hipEventRecord");
+        livePending_ = true;
+    }
+}
+
 void WorldSet::dropParticlesIfNone() {
     synchronize();
+    livePending_ = false;  // (a read in flight has landed; the count below is newer)
     StepCounters sc{};
     check_hip(hipMemcpy(&sc, dSc_, sizeof(sc), hipMemcpyDeviceToHost), "D2H");
     if (sc.liveParticles == 0) {

@@ -171,9 +171,15 @@ private:
     void ensureSorted();
     void checkErrors();
     void dropParticlesIfNone();  // stops launching the particle kernels once no particle is left
+    // the same without a host synchronisation: acts on the count read at the previous call (if it
+    // has arrived) and starts reading the current one
+    void checkParticlesAsync();
 
     hipStream_t stream_ = nullptr;
     SideStream side_;  // particle kernels beside the neighbour pass
+    int* liveHost_ = nullptr;        // pinned: live particles as read by checkParticlesAsync
+    hipEvent_t liveReady_ = nullptr;  // recorded after that read
+    bool livePending_ = false;
     std::vector<HostWorld> hosts_;
     std::vector<DevWorld> worlds_;
     std::vector<DevParams> params_;
