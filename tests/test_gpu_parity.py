@@ -297,6 +297,27 @@ def test_particle_kernels_stop_when_none_is_left_and_return_with_new_data():
     assert sim.counts()[1] < 30  # the new particles were absorbed or moved: they are simulated
 
 
+def test_particles_dying_within_one_long_call():
+    """Particles absorbed in the middle of one long call: the in-call checks (every 32 steps, read
+    without a host synchronisation and acted on a check later) stop the particle kernels between
+    graphs of four steps, and the run stays identical to the oracle."""
+    rng = np.random.default_rng(23)
+    c = H.cells(1500)
+    c["pos"] = rng.uniform(0, 64, size=(1500, 2)).astype(np.float32)
+    c["vel"] = rng.normal(0, 0.05, size=(1500, 2)).astype(np.float32)
+    c["maxConnections"] = 0
+    pa = H.particles(40)
+    pa["pos"] = c["pos"][:40]
+    pa["energy"] = 2.0
+    w = tiny_world(c, pa, motion=MOTION_FLUID)
+    sim = engine.Simulation.from_world(w)
+    ref = from_world(w, threads=CPU_THREADS)
+    sim.step(301)  # (an odd count: four-step graphs and single steps)
+    ref.step(301)
+    assert sim.counts()[1] == 0, "particles were expected to be absorbed"
+    assert_identical(sim.get_data(), ref.get_data(), "after one long call")
+
+
 @pytest.mark.parametrize("motion", [MOTION_FLUID, MOTION_COLLISION])
 def test_seam_and_odd_sizes(motion):
     """Cells on and across the torus seam, a world whose side is not a multiple of the bin edge
