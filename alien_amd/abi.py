@@ -99,6 +99,13 @@ class Params(ctypes.Structure):
         ("radiationAbsorption", ctypes.c_float * MAX_COLORS),
         ("cellMaxForce", ctypes.c_float * MAX_COLORS),
         ("cellMinEnergy", ctypes.c_float * MAX_COLORS),
+        ("radiationProb", ctypes.c_float),
+        ("radiationVelocityMultiplier", ctypes.c_float),
+        ("radiationVelocityPerturbation", ctypes.c_float),
+        ("radiationMinCellAge", ctypes.c_int32 * MAX_COLORS),
+        ("highRadiationFactor", ctypes.c_float * MAX_COLORS),
+        ("highRadiationMinCellEnergy", ctypes.c_float * MAX_COLORS),
+        ("radiationCellAgeStrength", ctypes.c_float * MAX_COLORS),
     ]
 
     def copy(self):
@@ -110,8 +117,11 @@ class Params(ctypes.Structure):
 def alien_default_params(**overrides):
     """ALIEN's defaults for the physics subset (Motion.h, SimulationParameters.h,
     SimulationParametersZoneValues.h), as alien_params_default() fills them: inner friction 0.3,
-    force decay probability 0.2 above cellMaxForce 0.8, cellMinEnergy 50."""
+    force decay probability 0.2 above cellMaxForce 0.8, cellMinEnergy 50, cell radiation with
+    probability 0.03 and age strength 0.00002."""
     p = default_params(innerFriction=0.3, cellMaxForceDecayProb=0.2)
+    for c in range(MAX_COLORS):
+        p.radiationCellAgeStrength[c] = 0.00002
     for k, v in overrides.items():
         if not hasattr(p, k):
             raise AttributeError(k)
@@ -121,8 +131,9 @@ def alien_default_params(**overrides):
 
 def default_params(**overrides):
     """The parameter set of the benchmark workloads and most tests: ALIEN's defaults for the
-    physics subset except inner friction 0 and force decay probability 0 (the workloads are
-    specified without them; the tests of those two processes set them explicitly)."""
+    physics subset except inner friction 0, force decay probability 0 and no cell radiation (age
+    strength and high-radiation factor 0; the workloads are specified without these processes, and
+    the tests of them set them explicitly)."""
     p = Params()
     p.timestepSize = 1.0
     p.motionType = MOTION_FLUID
@@ -143,6 +154,13 @@ def default_params(**overrides):
         p.radiationAbsorption[c] = 1.0
         p.cellMaxForce[c] = 0.8
         p.cellMinEnergy[c] = 50.0
+        p.radiationMinCellAge[c] = 0
+        p.highRadiationFactor[c] = 0.0
+        p.highRadiationMinCellEnergy[c] = 500.0
+        p.radiationCellAgeStrength[c] = 0.0
+    p.radiationProb = 0.03
+    p.radiationVelocityMultiplier = 1.0
+    p.radiationVelocityPerturbation = 0.5
     for k, v in overrides.items():
         if not hasattr(p, k):
             raise AttributeError(k)

@@ -117,7 +117,14 @@ void alien_params_default(alien_params* p) {
         p->radiationAbsorption[c] = 1.0f;
         p->cellMaxForce[c] = 0.8f;
         p->cellMinEnergy[c] = 50.0f;
+        p->radiationMinCellAge[c] = 0;
+        p->highRadiationFactor[c] = 0.0f;
+        p->highRadiationMinCellEnergy[c] = 500.0f;
+        p->radiationCellAgeStrength[c] = 0.00002f;
     }
+    p->radiationProb = 0.03f;
+    p->radiationVelocityMultiplier = 1.0f;
+    p->radiationVelocityPerturbation = 0.5f;
 }
 
 int alien_world_create(const alien_world_desc* desc, const alien_params* params, alien_world** out) {

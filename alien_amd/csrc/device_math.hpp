@@ -168,6 +168,9 @@ __device__ __forceinline__ void rot_ccw(f2& v) {  // Math.cuh:164-169
 // {id lo, id hi, timestep lo, purpose} and key {seed lo, seed hi}; the first output word's upper
 // 24 bits give a float in [0, 1).  A draw is a pure function of its arguments.
 constexpr uint32_t kStreamForceDecay = 1;  // checkForces, CellProcessor.cuh:367-386
+// CellProcessor::radiation (CellProcessor.cuh:725-773): whether the cell radiates this step, the
+// fraction of the energy loss drawn, the direction of the velocity perturbation
+constexpr uint32_t kStreamRadiation = 2, kStreamRadiationEnergy = 3, kStreamRadiationAngle = 4;
 __host__ __device__ inline float uniform_draw(unsigned long long seed, unsigned long long timestep,
                                               unsigned long long id, uint32_t purpose) {
     uint32_t c0 = uint32_t(id), c1 = uint32_t(id >> 32), c2 = uint32_t(timestep), c3 = purpose;
@@ -186,6 +189,38 @@ __host__ __device__ inline float uniform_draw(unsigned long long seed, unsigned 
         k1 += 0xBB67AE85u;
     }
     return float(c0 >> 8) * (1.0f / 16777216.0f);
+}
+
+// Math::unitVectorOfAngle (Math.cuh:134-138), {sin a, -cos a} for a in degrees, [0, 360) here.
+// Written with single-precision operations in a fixed order instead of sinf/cosf, whose last bit
+// is not the same on every platform: the angle is reduced to a quarter turn r in [0, 90) degrees,
+// x = r pi / 180, and sin x and cos x are their Taylor polynomials to x^13 and x^14 in Horner form
+// (truncation below 7e-10 on [0, pi/2]); the quarter turns then swap and negate them.
+__host__ __device__ inline float2 unit_vector_of_angle(float deg) {
+    int q = int(deg / 90.0f);
+    q = q < 0 ? 0 : (q > 3 ? 3 : q);
+    const float x = (deg - 90.0f * float(q)) * 0.017453292f;
+    const float x2 = x * x;
+    const float sn =
+        x * (1.0f + x2 * (-0.16666667f +
+                          x2 * (0.008333334f +
+                                x2 * (-0.0001984127f + x2 * (2.7557319e-06f + x2 * (-2.5052108e-08f + x2 * 1.6059044e-10f))))));
+    const float cs =
+        1.0f + x2 * (-0.5f + x2 * (0.041666668f +
+                                   x2 * (-0.0013888889f +
+                                         x2 * (2.4801588e-05f + x2 * (-2.7557319e-07f + x2 * (2.0876757e-09f + x2 * -1.1470746e-11f))))));
+    float s = sn, co = cs;  // sin and cos of the angle
+    if (q == 1) {
+        s = cs;
+        co = -sn;
+    } else if (q == 2) {
+        s = -sn;
+        co = -cs;
+    } else if (q == 3) {
+        s = -cs;
+        co = sn;
+    }
+    return make_float2(s, -co);
 }
 
 // SPH cubic spline and derivative (CellProcessor.cuh:92-119)

@@ -44,6 +44,9 @@ constexpr uint32_t kDetachedBit = 1u << 17;
 // particle flags byte
 constexpr uint8_t kParticleColorMask = 0x7F;
 constexpr uint8_t kParticleDead = 0x80;
+// ids of the particles cell radiation creates: this bit, the timestep (bits 32-62) and the local
+// index of the emitting cell (bits 0-31); ids given to set_data must not have it
+constexpr unsigned long long kCreatedParticleId = 1ull << 63;
 
 __host__ __device__ inline int flags_num_conn(uint32_t f) { return int(f & kNumConnMask); }
 __host__ __device__ inline int flags_max_conn(uint32_t f) { return int((f >> kMaxConnShift) & 0xFu); }
@@ -76,6 +79,12 @@ struct DevParams {
     float decayProb;           // cellMaxForceDecayProb (0: off)
     float maxForceSq[kColors];  // |F| > cellMaxForce  <=>  |F|^2 > maxForceSq (as reachSq)
     float minEnergy[kColors];   // cellMinEnergy
+    // cell radiation (CellProcessor.cuh:725-773); radiates: radiationProb > 0 and some colour has
+    // a positive high-radiation factor or age strength
+    int radiates;
+    float radProb, radVelMul, radVelPert;
+    int radMinAge[kColors];
+    float highRadFactor[kColors], highRadMinEnergy[kColors], radAgeStrength[kColors];
 };
 
 // per-world geometry and placement in the set
@@ -94,6 +103,9 @@ struct DevWorld {
     int bondWork;           // 1 if a bonded or barrier cell exists or can appear by fusion
     int livingWork;         // 1 if living states are tracked on the device (see WorldSet::build)
     unsigned long long seed;  // key of the world's random draws
+    // timestep at which the cells' base ages (ageBase) were taken: a non-barrier cell ages by one
+    // every step, so its age at the start of step t is ageBase + (t - ageTs)
+    unsigned long long ageTs;
 };
 
 // per-world device counters
@@ -106,7 +118,8 @@ struct DevCounters {
 // global counters of one step (reset at the start of every step)
 struct StepCounters {
     int candidates;  // fusion candidate pairs
-    int errors;      // bit 0: candidate list overflow; bit 1: map-build look-back gave up
+    int errors;      // bit 0: candidate list overflow; bit 1: map-build look-back gave up;
+                     // bit 2: a world's particles exceeded its particle capacity
     int liveParticles;  // energy particles alive after this step's particle sort, all worlds
     // neighbour-pass paths since the data was set (not reset per step): chunks that walked their
     // stencils in global memory, and cells of staged rows that walked theirs alone
@@ -128,8 +141,9 @@ struct HotBuffers {
     float2* pvel;
     float* pen;
     int* plast;   // local cold slot of the last absorbing cell, kNone if none
-    int* pslot;   // local particle slot (stable identity)
+    int* pslot;   // local particle slot (host-side selection state), kNone for created particles
     uint8_t* pflags;
+    unsigned long long* pid;  // particle id
 };
 
 struct Ctx {
@@ -146,6 +160,11 @@ struct Ctx {
     float* energy;  // cold [cellCap], indexed by cellOff + slot
     float* stiff;   // cold [cellCap]
     unsigned long long* cid;  // cold [cellCap]: cell id (key of the cell's random draws)
+    uint32_t* ageBase;        // cold [cellCap]: cell age at the world's ageTs
+    // Particles emitted by cell radiation during a step are written at partStage + (cell index),
+    // behind the worlds' particle ranges (partStage = 0: no staging area), and enter the step's
+    // particle map like the live particles.
+    int partStage;
     // living states of the worlds with livingWork, cold by slot: `living` holds the state at the
     // start of the step, `livingNext` the other buffer (the step's phases alternate between the
     // two and leave the end-of-step state in `livingNext`, the next step's `living`)
@@ -214,6 +233,7 @@ struct Ctx {
     const int2* tileBlocks;  // per block: (world, tile index), -1 for padding blocks
     const int2* livingBlocks;  // per block: (world, first global cell index) over livingWork worlds
     int numPartBlocks;
+    int numCellBlocks, cellBlockCells;  // entries of cellBlocks, cells per entry
 };
 
 }  // namespace alien

@@ -140,6 +140,67 @@ __global__ void k_part_absorb(Ctx c) {
     });
 }
 
+// Cell radiation at the start of a step (CellProcessor::radiation, CellProcessor.cuh:725-773, run
+// by the reference after the cell map update and before the forces): a non-barrier cell radiates
+// with probability radiationProb (R9 draw); its factor is highRadiationFactor above
+// highRadiationMinCellEnergy plus radiationCellAgeStrength above radiationMinCellAge; it loses
+// 2 u E factor / radiationProb (at most E - 1, only if E > 1), emitted as an energy particle with
+// the cell's velocity times radiationVelocityMultiplier plus a unit vector of random angle times
+// radiationVelocityPerturbation, 1.5 ahead of the cell along it and one velocity behind (the move
+// of this step brings it forward), unless the energy is below NEAR_ZERO
+// (RadiationProcessor::createEnergyParticle, RadiationProcessor.cuh:248-364, without radiation
+// sources and external energy).  The particle of cell i goes to the staging slot partStage + i and
+// into this step's particle histogram, so that the map sorts it behind the bin's live particles,
+// in cell order (R18); its id is kCreatedParticleId | timestep << 32 | local cell index (R18).
+__global__ void k_radiation(Ctx c) {
+    const int2 blk = c.cellBlocks[blockIdx.x];
+    const int wi = blk.x;
+    const DevWorld& w = c.worlds[wi];
+    const int i = blk.y + int(threadIdx.x);
+    if (i >= w.cellOff + w.numCells) return;
+    const DevParams& P = c.params[wi];
+    if (!P.radiates) return;
+    const uint32_t f = c.A.flags[i];
+    if (flags_barrier(f)) return;
+    const int sl = w.cellOff + c.A.slot[i];
+    const unsigned long long id = c.cid[sl];
+    const unsigned long long ts = c.ctr[wi].timestep;
+    if (!(uniform_draw(w.seed, ts, id, kStreamRadiation) < P.radProb)) return;
+    const int col = flags_color(f) % kColors;
+    const float e = c.energy[sl];
+    float factor = 0.0f;
+    if (e > P.highRadMinEnergy[col]) factor = factor + P.highRadFactor[col];
+    const long long age = (long long)(c.ageBase[sl] + uint32_t(ts - w.ageTs));
+    if (age > (long long)P.radMinAge[col]) factor = factor + P.radAgeStrength[col];
+    if (!(factor > 0.0f)) return;
+    float loss = e * factor;
+    loss = loss / P.radProb;
+    loss = 2.0f * loss * uniform_draw(w.seed, ts, id, kStreamRadiationEnergy);
+    if (!(e > 1.0f)) return;
+    const f2 u = to_f2(unit_vector_of_angle(uniform_draw(w.seed, ts, id, kStreamRadiationAngle) * 360.0f));
+    const f2 v = add(mul(to_f2(c.A.vel[i]), P.radVelMul), mul(u, P.radVelPert));
+    const f2 p = torus_pos(sub(add(to_f2(c.A.pos[i]), mul(unit(v), 1.5f)), v), w.W, w.H);
+    if (loss > e - 1.0f) loss = e - 1.0f;
+    if (loss > kNearZero) {
+        const int st = c.partStage + i;
+        c.A.ppos[st] = to_float2(p);
+        c.A.pvel[st] = to_float2(v);
+        c.A.pen[st] = loss;
+        c.A.plast[st] = kNone;
+        c.A.pslot[st] = kNone;
+        c.A.pflags[st] = uint8_t(col & kParticleColorMask);
+        c.A.pid[st] = kCreatedParticleId | ((ts & 0x7fffffffull) << 32) | (unsigned long long)uint32_t(i - w.cellOff);
+        pbin_for_next_step(c, w, st, p, c.count);
+    }
+    c.energy[sl] = e - loss;
+}
+
+void launch_radiation(const Ctx& c, const LaunchShape& sh, hipStream_t s, Recorder* rec) {
+    if (!sh.hasRadiation || !sh.cellBlocks) return;
+    if (rec) rec->mark("radiation");
+    hipLaunchKernelGGL(k_radiation, dim3(sh.cellBlocks), dim3(sh.cellThreads), 0, s, c);
+}
+
 // targets and fusion claims: the moved particles and the map
 void launch_particles_pre(const Ctx& c, const LaunchShape& sh, hipStream_t s, Recorder* rec) {
     if (!sh.partBlocks || !sh.hasParticles) return;

@@ -318,13 +318,37 @@ __global__ void k_cell_scatter(Ctx c, int tableBlocks) {
     }
 }
 
+// particle i (A index or staging slot) of bin gb to its place in the map (the order inside the bin
+// is fixed by the bin sort)
+__device__ __forceinline__ void scatter_particle(const Ctx& c, const DevWorld& w, int gb, int i) {
+    const int rel = int(c.start[gb] - c.start[w.pbinOff]) + int(atomicSub(&c.count[gb], 1u)) - 1;
+    if (rel < w.partCap)
+        c.psorted[w.partOff + rel] = i;
+    else
+        atomicOr(&c.sc->errors, 4);  // (the host keeps room for a step's emissions: not reached)
+}
+
 __global__ void k_part_scatter(Ctx c) {
     for_live_particles(c, [&](int, const DevWorld& w, int i) {
         int gb = c.pbinOf[i];
         if (gb < 0) return;
-        int dest = w.partOff + int(c.start[gb] - c.start[w.pbinOff]) + int(atomicSub(&c.count[gb], 1u)) - 1;
-        c.psorted[dest] = i;
+        scatter_particle(c, w, gb, i);
     });
+    // the particles cell radiation emitted this step, from their staging slots (cleared once read)
+    if (c.partStage > 0)
+        for (int e = blockIdx.x; e < c.numCellBlocks; e += gridDim.x) {
+            const int2 blk = c.cellBlocks[e];
+            const DevWorld& w = c.worlds[blk.x];
+            for (int t = int(threadIdx.x); t < c.cellBlockCells; t += int(blockDim.x)) {
+                const int i = blk.y + t;
+                if (i >= w.cellOff + w.numCells) break;
+                const int st = c.partStage + i;
+                const int gb = c.pbinOf[st];
+                if (gb < 0) continue;
+                c.pbinOf[st] = -1;
+                scatter_particle(c, w, gb, st);
+            }
+        }
 }
 
 // sort the entries of one bin ascending (R1 stable order), small bins in registers
@@ -384,12 +408,13 @@ __global__ void k_cell_binsort(Ctx c) {
 
 // Particle map in logical order, and the movement of the bin's particles (RadiationProcessor.cuh:
 // 61-70: position += velocity * dt on the torus) into the end-of-step arrays at their map
-// positions; also publishes the new number of live particles per world.
+// positions; also publishes the new number of live particles per world.  The particles cell
+// radiation emitted this step come from their staging slots, behind every live particle's index.
 __global__ void k_part_binsort(Ctx c) {
     if (blockIdx.x == 0) {
         for (int wi = threadIdx.x; wi < c.numWorlds; wi += blockDim.x) {
             const DevWorld& w = c.worlds[wi];
-            const int live = int(c.start[w.pbinOff + w.pnbx * w.pnby] - c.start[w.pbinOff]);
+            const int live = min(int(c.start[w.pbinOff + w.pnbx * w.pnby] - c.start[w.pbinOff]), w.partCap);
             c.ctr[wi].numParticles = live;
             if (live > 0) atomicAdd(&c.sc->liveParticles, live);
         }
@@ -411,6 +436,7 @@ __global__ void k_part_binsort(Ctx c) {
         c.B.plast[k] = c.A.plast[src];
         c.B.pslot[k] = c.A.pslot[src];
         c.B.pflags[k] = c.A.pflags[src];
+        c.B.pid[k] = c.A.pid[src];
     }
 }
 

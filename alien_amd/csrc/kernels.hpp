@@ -20,9 +20,11 @@ struct LaunchShape {
     int candGrid = 16;     // grid-stride kernels over fusion candidates
     bool hasFluid = false, hasCollision = false;  // motion modes present in the set
     bool hasBondWork = true;  // some active world has (or can get by fusion) a bonded or barrier cell
-    // some world has live energy particles (nothing on this path creates particles, so once all
-    // are absorbed or merged the particle kernels, which would find no work, are not launched)
+    // some world has live energy particles or radiates (only cell radiation creates particles, so
+    // once all are absorbed or merged in a set without it the particle kernels, which would find
+    // no work, are not launched)
     bool hasParticles = true;
+    bool hasRadiation = false;  // some active world radiates (DevParams::radiates)
     int livingBlocks = 0;           // cell blocks of the worlds whose living states live on the device
     bool hasInnerFriction = false;  // some active world with bond work has innerFriction > 0
 };

@@ -40,10 +40,11 @@ __device__ __forceinline__ void cell_bin_range(const Ctx& c, const DevWorld& w, 
     s = w.cellOff + int(c.start[w.binOff + b] - base);
     e = w.cellOff + int(c.start[w.binOff + b + 1] - base);
 }
+// (clamped to the world's particle capacity: a map beyond it is reported by the particle scatter)
 __device__ __forceinline__ void part_bin_range(const Ctx& c, const DevWorld& w, int b, int& s, int& e) {
     uint32_t base = c.start[w.pbinOff];
-    s = w.partOff + int(c.start[w.pbinOff + b] - base);
-    e = w.partOff + int(c.start[w.pbinOff + b + 1] - base);
+    s = w.partOff + min(int(c.start[w.pbinOff + b] - base), w.partCap);
+    e = w.partOff + min(int(c.start[w.pbinOff + b + 1] - base), w.partCap);
 }
 
 // cell bin of a position (global bin index), and its bin coordinates

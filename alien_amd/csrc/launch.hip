@@ -20,8 +20,11 @@ void launch_neighbor(const Ctx& c, const LaunchShape& sh, hipStream_t s, bool fl
 void launch_particles_pre(const Ctx& c, const LaunchShape& sh, hipStream_t s, Recorder* rec);
 void launch_particles_post(const Ctx& c, const LaunchShape& sh, hipStream_t s, Recorder* rec);
 void launch_bonds(const Ctx& c, const LaunchShape& sh, hipStream_t s, Recorder* rec);
+void launch_radiation(const Ctx& c, const LaunchShape& sh, hipStream_t s, Recorder* rec);
 
 void launch_step(const Ctx& c, const LaunchShape& sh, hipStream_t s, Recorder* rec, const SideStream* side) {
+    // cell radiation comes first: the particles it emits enter this step's particle map
+    launch_radiation(c, sh, s, rec);
     // without bond work the end-of-step kernel would only advance the timesteps: the map scan does
     // it
     if (rec) rec->mark("map_scan");

@@ -248,16 +248,26 @@ int alien_settings_read(const char* json, size_t length, alien_world_desc* desc,
         r.num(S + "cell.fusion velocity", p.cellFusionVelocity);
         r.num(S + "cell.max binding energy", p.cellMaxBindingEnergy);
         r.colors(S + "radiation.absorption", p.radiationAbsorption);
+        // cell radiation (:203-268): radiationCellAgeStrength is "radiation.factor"
+        r.colors(S + "radiation.factor", p.radiationCellAgeStrength);
+        r.num(S + "radiation.probability", p.radiationProb);
+        r.num(S + "radiation.velocity multiplier", p.radiationVelocityMultiplier);
+        r.num(S + "radiation.velocity perturbation", p.radiationVelocityPerturbation);
+        r.colors(S + "high radiation.min cell energy", p.highRadiationMinCellEnergy);
+        r.colors(S + "high radiation.factor", p.highRadiationFactor);
+        for (int c = 0; c < ALIEN_MAX_COLORS; ++c)
+            r.num(S + "radiation.min cell age[" + std::to_string(c) + "]", p.radiationMinCellAge[c]);
         // processes in the file that the physics path does not simulate
         uint32_t u = 0;
         if (r.value(S + "spots.num spots", 0.0) > 0.0) u |= ALIEN_SETTINGS_ZONES;
         if (r.value(S + "particle sources.num sources", 0.0) > 0.0) u |= ALIEN_SETTINGS_RADIATION_SOURCES;
         for (int c = 0; c < ALIEN_MAX_COLORS; ++c) {
             const std::string k = "[" + std::to_string(c) + "]";
-            // radiation emission: radiationCellAgeStrength (ZoneValues.h:17) and highRadiationFactor
-            // (SimulationParameters.h:85), both read by CellProcessor::radiation (:725-773)
-            if (r.value(S + "radiation.factor" + k, 0.00002) > 0.0 || r.value(S + "high radiation.factor" + k, 0.0) > 0.0)
-                u |= ALIEN_SETTINGS_RADIATION;
+            // the absorption penalties (SimulationParameters.h:83-84, ZoneValues.h:15-16)
+            for (const char* pen : {"radiation.absorption velocity penalty", "radiation.absorption low velocity penalty",
+                                    "radiation.absorption low connection penalty",
+                                    "radiation.absorption low genome complexity penalty"})
+                if (r.value(S + pen + k, 0.0) != 0.0) u |= ALIEN_SETTINGS_RADIATION;
             if (r.value(S + "cell.death probability" + k, 0.001) > 0.0) u |= ALIEN_SETTINGS_CELL_DEATH;
             if (r.value(S + "particle.split energy" + k, 3.4e38) < 3.0e38) u |= ALIEN_SETTINGS_PARTICLE_SPLIT;
         }
@@ -305,6 +315,14 @@ int alien_settings_write(const alien_world_desc* desc, const alien_params* param
         put(root, S + "cell.max binding energy", fmt(std::isinf(p.cellMaxBindingEnergy) ? 3.40282346638528859811704183484516925440e38
                                                                                        : double(p.cellMaxBindingEnergy)));
         colors(S + "radiation.absorption", p.radiationAbsorption);
+        colors(S + "radiation.factor", p.radiationCellAgeStrength);
+        put(root, S + "radiation.probability", fmt(p.radiationProb));
+        put(root, S + "radiation.velocity multiplier", fmt(p.radiationVelocityMultiplier));
+        put(root, S + "radiation.velocity perturbation", fmt(p.radiationVelocityPerturbation));
+        colors(S + "high radiation.min cell energy", p.highRadiationMinCellEnergy);
+        colors(S + "high radiation.factor", p.highRadiationFactor);
+        for (int c = 0; c < ALIEN_MAX_COLORS; ++c)
+            put(root, S + "radiation.min cell age[" + std::to_string(c) + "]", std::to_string(p.radiationMinCellAge[c]));
         // what this path keeps fixed, so that ALIEN reads the file back to the same physics
         put(root, S + "cell.death consequences", "0");
         put(root, S + "spots.num spots", "0");
@@ -312,8 +330,6 @@ int alien_settings_write(const alien_world_desc* desc, const alien_params* param
         put(root, S + "rigidity", fmt(0.0));
         for (int c = 0; c < ALIEN_MAX_COLORS; ++c) {
             const std::string k = "[" + std::to_string(c) + "]";
-            put(root, S + "radiation.factor" + k, fmt(0.0));
-            put(root, S + "high radiation.factor" + k, fmt(0.0));
             put(root, S + "cell.death probability" + k, fmt(0.0));
         }
         std::string out;

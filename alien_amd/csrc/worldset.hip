@@ -33,7 +33,15 @@ void validate_params(const alien_params& p) {
     for (int c = 0; c < kColors; ++c) {
         if (!(p.cellMaxForce[c] >= 0.0f)) throw Error(ALIEN_E_ARG, "cellMaxForce must not be negative");
         if (std::isnan(p.cellMinEnergy[c])) throw Error(ALIEN_E_ARG, "cellMinEnergy must be a number");
+        if (!(p.highRadiationFactor[c] >= 0.0f) || !(p.radiationCellAgeStrength[c] >= 0.0f) ||
+            std::isinf(p.highRadiationFactor[c]) || std::isinf(p.radiationCellAgeStrength[c]))
+            throw Error(ALIEN_E_ARG, "radiation factors must be finite and not negative");
+        if (std::isnan(p.highRadiationMinCellEnergy[c]))
+            throw Error(ALIEN_E_ARG, "highRadiationMinCellEnergy must be a number");
     }
+    if (!(p.radiationProb >= 0.0f && p.radiationProb <= 1.0f)) throw Error(ALIEN_E_ARG, "radiationProb must lie in [0, 1]");
+    if (!std::isfinite(p.radiationVelocityMultiplier) || !std::isfinite(p.radiationVelocityPerturbation))
+        throw Error(ALIEN_E_ARG, "radiation velocity multiplier and perturbation must be finite");
     if (!(p.timestepSize > 0.0f)) throw Error(ALIEN_E_ARG, "timestepSize must be positive");
     if (p.motionType == ALIEN_MOTION_FLUID && !(p.smoothingLength > 0.0f))
         throw Error(ALIEN_E_ARG, "smoothingLength must be positive");
@@ -111,6 +119,16 @@ DevParams to_dev_params(const alien_params& p) {
         d.maxForceSq[c] = std::isinf(p.cellMaxForce[c]) ? std::numeric_limits<float>::infinity()
                                                         : sqrt_threshold(p.cellMaxForce[c]);
         d.minEnergy[c] = p.cellMinEnergy[c];
+    }
+    d.radiates = radiates(p) ? 1 : 0;
+    d.radProb = p.radiationProb;
+    d.radVelMul = p.radiationVelocityMultiplier;
+    d.radVelPert = p.radiationVelocityPerturbation;
+    for (int c = 0; c < kColors; ++c) {
+        d.radMinAge[c] = p.radiationMinCellAge[c];
+        d.highRadFactor[c] = p.highRadiationFactor[c];
+        d.highRadMinEnergy[c] = p.highRadiationMinCellEnergy[c];
+        d.radAgeStrength[c] = p.radiationCellAgeStrength[c];
     }
     d.invH = 1.0f / d.smoothing;
     d.invH2 = 1.0f / (d.smoothing * d.smoothing);
@@ -216,6 +234,7 @@ WorldImage WorldImage::fromTO(const alien_world_desc& desc, const alien_params& 
     img.plast.assign(np, kNone);
     img.pslot.resize(np);
     img.pflags.resize(np);
+    img.pid.resize(np);
     h.particleId.resize(np);
     h.particleSelected.resize(np);
     for (int i = 0; i < np; ++i) {
@@ -229,14 +248,16 @@ WorldImage WorldImage::fromTO(const alien_world_desc& desc, const alien_params& 
         img.pen[i] = p.energy;
         img.pslot[i] = i;
         img.pflags[i] = uint8_t(p.color & kParticleColorMask);
+        img.pid[i] = p.id;
         h.particleId[i] = p.id;
         h.particleSelected[i] = p.selected;
     }
     img.numParticles = np;
     // Living states need tracking on the device only if some cell is not trivially Ready: a state
-    // other than Ready, an activation count, or (non-barrier) energy below cellMinEnergy.  Energy
-    // only grows on this path, so a world without any of these keeps every cell Ready with no
-    // count for good, and its ages follow from the number of steps.
+    // other than Ready, an activation count, (non-barrier) energy below cellMinEnergy, or cell
+    // radiation, the one process of this path that takes energy from cells.  A world without any
+    // of these keeps every cell Ready with no count for good (its energies only grow).
+    img.livingWork = radiates(params);
     for (int i = 0; i < n && !img.livingWork; ++i) {
         const alien_cell_to& c = cells[i];
         img.livingWork = c.livingState != 0 || c.activationTime != 0 ||
@@ -247,6 +268,7 @@ WorldImage WorldImage::fromTO(const alien_world_desc& desc, const alien_params& 
 
 void WorldImage::updateLivingWork() {
     const HostWorld& h = host;
+    if (radiates(h.params)) livingWork = true;
     for (int i = 0; i < h.numCells && !livingWork; ++i) {
         const int sl = slot[size_t(i)];
         const uint32_t f = flags[size_t(i)];
@@ -300,7 +322,7 @@ void WorldImage::toTO(std::vector<alien_cell_to>& cells, std::vector<alien_parti
     }
     const int np = numParticles;
     std::vector<std::pair<uint64_t, int>> pkeys(static_cast<size_t>(np));
-    for (int i = 0; i < np; ++i) pkeys[size_t(i)] = {h.particleId[pslot[i]], i};
+    for (int i = 0; i < np; ++i) pkeys[size_t(i)] = {pid[i], i};
     std::sort(pkeys.begin(), pkeys.end());
     std::vector<int> po(np);
     for (int k = 0; k < np; ++k) po[k] = pkeys[size_t(k)].second;
@@ -308,14 +330,14 @@ void WorldImage::toTO(std::vector<alien_cell_to>& cells, std::vector<alien_parti
     for (int k = 0; k < np; ++k) {
         const int i = po[k];
         alien_particle_to& p = particles[k];
-        p.id = h.particleId[pslot[i]];
+        p.id = pid[i];
         p.energy = pen[i];
         p.pos[0] = ppos[i].x;
         p.pos[1] = ppos[i].y;
         p.vel[0] = pvel[i].x;
         p.vel[1] = pvel[i].y;
         p.color = uint8_t(pflags[i] & kParticleColorMask);
-        p.selected = h.particleSelected[pslot[i]];
+        p.selected = pslot[i] >= 0 ? h.particleSelected[pslot[i]] : 0;
     }
 }
 
@@ -417,7 +439,7 @@ void WorldSet::synchronize() { check_hip(hipStreamSynchronize(stream_), "hipStre
 uint64_t WorldSet::estimateBytes(uint64_t cells, uint64_t particles, int W, int H, int binSize) {
     const uint64_t perCell = 2 * (8 + 8 + 4 + 4 + 4 + kMaxBonds * 12) + 8 + 4 * 4 + 8 + 3 * 8 + 2 * kMaxBonds * 8 + 4 +
                              2 + 8 + 4 * 16 + 4 + (8 + 8 + 4 + 4) + 4 + 4;
-    const uint64_t perParticle = 2 * (8 + 8 + 4 + 4 + 4 + 1) + 6 * 4;
+    const uint64_t perParticle = 2 * (8 + 8 + 4 + 4 + 4 + 1 + 8) + 6 * 4;
     const uint64_t bins = uint64_t(std::max(1, W / binSize)) * uint64_t(std::max(1, H / binSize)) +
                           uint64_t(std::max(1, W / kParticleBin)) * uint64_t(std::max(1, H / kParticleBin));
     // per bin: two histograms, two offset arrays, two mover-list heads and deltas (cell bins), scan
@@ -515,7 +537,12 @@ void WorldSet::build(const std::vector<Source>& sources) {
         w.numCells = h.numCells;
         w.binOff = int(bins);
         w.partOff = int(partOff);
-        w.partCap = live;
+        // room for the particles cell radiation emits (one per cell and step at most)
+        const int64_t cap = live + (radiates(h.params) ? int64_t(2 * kRadiationSteps) * h.numCells : 0);
+        if (partOff + cap > kMaxSetParticles)
+            throw Error(ALIEN_E_CAPACITY, "the particle room of the radiating worlds exceeds the particles of one world set");
+        w.partCap = int(cap);
+        w.ageTs = ts - h.stepsSinceSet;
         w.bondWork = work ? 1 : 0;
         w.livingWork = living ? 1 : 0;
         w.seed = h.desc.randomSeed;
@@ -523,7 +550,7 @@ void WorldSet::build(const std::vector<Source>& sources) {
         stripChoice.push_back(strip);
         stageChoice.push_back(stage);
         cellOff += h.numCells;
-        partOff += live;
+        partOff += w.partCap;
         bins += int64_t(w.nbx) * w.nby;
         if (bins + int64_t(w.pnbx) * w.pnby >= (int64_t(1) << 31) - 2)
             throw Error(ALIEN_E_CAPACITY, "too many spatial bins in one world set");
@@ -546,7 +573,12 @@ void WorldSet::build(const std::vector<Source>& sources) {
     WorldSet& me = *this;
     invalidateGraphs();
 
-    const int NC = int(std::max<int64_t>(cellOff, 1)), NP = int(std::max<int64_t>(partOff, 1));
+    bool anyRadiates = false;
+    for (const auto& h : hosts) anyRadiates = anyRadiates || radiates(h.params);
+    const int NC = int(std::max<int64_t>(cellOff, 1));
+    // the worlds' particle ranges, then (with a radiating world) one staging slot per cell
+    partStage_ = anyRadiates ? int(std::max<int64_t>(partOff, 1)) : 0;
+    const int NP = anyRadiates ? partStage_ + NC : int(std::max<int64_t>(partOff, 1));
     const int NB = int(std::max<int64_t>(bins, 1));
     cellCap_ = NC;
     partCap_ = NP;
@@ -573,10 +605,12 @@ void WorldSet::build(const std::vector<Source>& sources) {
         h.plast = static_cast<int*>(A(sizeof(int) * NP));
         h.pslot = static_cast<int*>(A(sizeof(int) * NP));
         h.pflags = static_cast<uint8_t*>(A(NP));
+        h.pid = static_cast<unsigned long long*>(A(sizeof(unsigned long long) * NP));
     }
     energy_ = static_cast<float*>(A(sizeof(float) * NC));
     stiff_ = static_cast<float*>(A(sizeof(float) * NC));
     cid_ = static_cast<unsigned long long*>(A(sizeof(unsigned long long) * NC));
+    ageBase_ = static_cast<uint32_t*>(A(sizeof(uint32_t) * NC));
     living_[0] = static_cast<uint8_t*>(A(NC));
     living_[1] = static_cast<uint8_t*>(A(NC));
     activation_ = static_cast<uint32_t*>(A(sizeof(uint32_t) * NC));
@@ -648,6 +682,7 @@ void WorldSet::build(const std::vector<Source>& sources) {
             h2d(stiff_ + w.cellOff, im.stiff.data(), sizeof(float) * n);
             static_assert(sizeof(unsigned long long) == sizeof(uint64_t), "cell ids");
             h2d(cid_ + w.cellOff, im.host.cellId.data(), sizeof(uint64_t) * n);
+            h2d(ageBase_ + w.cellOff, im.host.age.data(), sizeof(uint32_t) * n);
             h2d(living_[0] + w.cellOff, im.host.livingState.data(), n);
             h2d(activation_ + w.cellOff, im.host.activationTime.data(), sizeof(uint32_t) * n);
             h2d(creature_ + w.cellOff, im.host.creatureId.data(), sizeof(uint32_t) * n);
@@ -657,6 +692,8 @@ void WorldSet::build(const std::vector<Source>& sources) {
             h2d(D.plast + w.partOff, im.plast.data(), sizeof(int) * np);
             h2d(D.pslot + w.partOff, im.pslot.data(), sizeof(int) * np);
             h2d(D.pflags + w.partOff, im.pflags.data(), np);
+            static_assert(sizeof(unsigned long long) == sizeof(uint64_t), "particle ids");
+            h2d(D.pid + w.partOff, im.pid.data(), sizeof(uint64_t) * np);
             // the staging vectors must outlive the async copies
             check_hip(hipStreamSynchronize(stream_), "hipStreamSynchronize");
         } else {
@@ -678,6 +715,7 @@ void WorldSet::build(const std::vector<Source>& sources) {
             d2d(energy_ + w.cellOff, o.energy_ + ow.cellOff, sizeof(float) * n);
             d2d(stiff_ + w.cellOff, o.stiff_ + ow.cellOff, sizeof(float) * n);
             d2d(cid_ + w.cellOff, o.cid_ + ow.cellOff, sizeof(unsigned long long) * n);
+            d2d(ageBase_ + w.cellOff, o.ageBase_ + ow.cellOff, sizeof(uint32_t) * n);
             d2d(living_[0] + w.cellOff, o.living_[o.parity_] + ow.cellOff, n);
             d2d(activation_ + w.cellOff, o.activation_ + ow.cellOff, sizeof(uint32_t) * n);
             d2d(creature_ + w.cellOff, o.creature_ + ow.cellOff, sizeof(uint32_t) * n);
@@ -687,6 +725,7 @@ void WorldSet::build(const std::vector<Source>& sources) {
             d2d(D.plast + w.partOff, S.plast + ow.partOff, sizeof(int) * np);
             d2d(D.pslot + w.partOff, S.pslot + ow.partOff, sizeof(int) * np);
             d2d(D.pflags + w.partOff, S.pflags + ow.partOff, np);
+            d2d(D.pid + w.partOff, S.pid + ow.partOff, sizeof(unsigned long long) * np);
         }
     }
     check_hip(hipStreamSynchronize(stream_), "hipStreamSynchronize");
@@ -701,7 +740,8 @@ void WorldSet::build(const std::vector<Source>& sources) {
     stripChoice_ = std::move(stripChoice);
     stageChoice_ = std::move(stageChoice);
     chunk_shape(worlds_, params_, chunkRows_);
-    shape_.hasParticles = partOff > 0;
+    shape_.hasParticles = anyRadiates;
+    for (int lp : liveParticles) shape_.hasParticles = shape_.hasParticles || lp > 0;
     parity_ = 0;
     shape_.candGrid = 16;
     // one particle block per workgroup (blocks past a world's live particles exit at once)
@@ -735,6 +775,8 @@ void WorldSet::build(const std::vector<Source>& sources) {
     launch_fill_u32(reinterpret_cast<uint32_t*>(delFlag_), 0u, (size_t(NC) + 3) / 4, stream_);
     launch_fill_u32(reinterpret_cast<uint32_t*>(mark_), 0u, (size_t(NC) + 3) / 4, stream_);
     launch_fill_u32(reinterpret_cast<uint32_t*>(buf_[1].pflags), 0u, (size_t(NP) + 3) / 4, stream_);
+    if (partStage_ > 0)  // no particle waits in a staging slot
+        launch_fill_u32(reinterpret_cast<uint32_t*>(pbinOf_ + partStage_), 0xffffffffu, size_t(NC), stream_);
     check_hip(hipGetLastError(), "fill kernels");
     buildTables();
     launch_seed_bins(makeCtx(parity_), shape_, stream_);
@@ -900,10 +942,20 @@ void WorldSet::buildTables() {
     shape_.pbinBlocks = int(pbins.size());
     shape_.tileBlocks = int(tileTable.size());
     numPartBlocks_ = int(parts.size());
+    numCellBlocks_ = int(cells.size());
     shape_.hasFluid = shape_.hasCollision = false;
     for (const DevParams& p : params_) (p.motion == 0 ? shape_.hasFluid : shape_.hasCollision) = true;
     updateInnerFriction();
+    updateRadiation();
     invalidateGraphs();
+}
+
+void WorldSet::updateRadiation() {
+    bool on = false;
+    for (size_t w = 0; w < params_.size(); ++w) on = on || (active_[w] && params_[w].radiates);
+    if (on != shape_.hasRadiation) invalidateGraphs();
+    shape_.hasRadiation = on;
+    if (on) shape_.hasParticles = true;  // (particles can appear)
 }
 
 // the inner-friction kernel runs when an active world with bond work has innerFriction > 0
@@ -930,6 +982,10 @@ void WorldSet::setBlockSize(int threads) {
 void WorldSet::setActive(int w, bool on) {
     if (active_.at(w) == uint8_t(on)) return;
     active_[w] = on;
+    if (on) {
+        synchronize();
+        anchorAges(w);
+    }
     buildTables();
     invalidateGraphs();
 }
@@ -1004,6 +1060,8 @@ Ctx WorldSet::makeCtx(int parity) const {
     c.energy = energy_;
     c.stiff = stiff_;
     c.cid = cid_;
+    c.ageBase = ageBase_;
+    c.partStage = partStage_;
     c.living = living_[parity];
     c.livingNext = living_[parity ^ 1];
     c.activation = activation_;
@@ -1048,6 +1106,8 @@ Ctx WorldSet::makeCtx(int parity) const {
     c.partBlocks = partBlocks_;
     c.tileBlocks = tileBlocks_;
     c.numPartBlocks = numPartBlocks_;
+    c.numCellBlocks = numCellBlocks_;
+    c.cellBlockCells = shape_.cellThreads;
     return c;
 }
 
@@ -1081,6 +1141,17 @@ void WorldSet::setTimestep(int w, uint64_t t) {
     check_hip(hipMemcpy(&c, dCtr_ + w, sizeof(c), hipMemcpyDeviceToHost), "D2H");
     c.timestep = t;
     check_hip(hipMemcpy(dCtr_ + w, &c, sizeof(c), hipMemcpyHostToDevice), "H2D");
+    anchorAges(w);
+}
+
+// The device derives a cell's age from the world's timestep (ageBase + timestep - ageTs); after
+// the timestep is set, or while the world was inactive (frozen, its ages too, though the step
+// advances every world's timestep), the anchor moves so that the ages stay base + steps simulated.
+void WorldSet::anchorAges(int w) {
+    DevCounters c{};
+    check_hip(hipMemcpy(&c, dCtr_ + w, sizeof(c), hipMemcpyDeviceToHost), "D2H");
+    worlds_.at(w).ageTs = c.timestep - hosts_.at(w).stepsSinceSet;
+    check_hip(hipMemcpy(dWorlds_ + w, &worlds_[w], sizeof(DevWorld), hipMemcpyHostToDevice), "H2D");
 }
 
 void WorldSet::setParams(int w, const alien_params& p) {
@@ -1088,7 +1159,8 @@ void WorldSet::setParams(int w, const alien_params& p) {
     if (bin_size_for(p) != bin_size_for(hosts_.at(w).params))
         throw Error(ALIEN_E_STATE, "the bin grid changes with these parameters: build a fresh set around them");
     if (!livingWork_.at(w) && needs_fresh_set(hosts_.at(w).params, p))
-        throw Error(ALIEN_E_STATE, "cellMinEnergy changes: build a fresh set around the new parameters");
+        throw Error(ALIEN_E_STATE,
+                    "cellMinEnergy changes or cell radiation starts: build a fresh set around the new parameters");
     const bool wasFluid = params_.at(w).motion == 0;
     hosts_.at(w).params = p;
     params_.at(w) = to_dev_params(p);
@@ -1108,6 +1180,7 @@ void WorldSet::setParams(int w, const alien_params& p) {
         invalidateGraphs();
     }
     updateInnerFriction();
+    updateRadiation();  // (the room for emitted particles is made before the next step)
 }
 
 // A world whose living states are not tracked on the device stays trivially Ready only while no
@@ -1116,6 +1189,15 @@ void WorldSet::setParams(int w, const alien_params& p) {
 bool needs_fresh_set(const alien_params& before, const alien_params& after) {
     for (int c = 0; c < kColors; ++c)
         if (before.cellMinEnergy[c] != after.cellMinEnergy[c]) return true;
+    return radiates(after) && !radiates(before);
+}
+
+// CellProcessor::radiation takes energy from cells only when a draw below radiationProb can meet
+// a positive factor
+bool radiates(const alien_params& p) {
+    if (!(p.radiationProb > 0.0f)) return false;
+    for (int c = 0; c < kColors; ++c)
+        if (p.highRadiationFactor[c] > 0.0f || p.radiationCellAgeStrength[c] > 0.0f) return true;
     return false;
 }
 
@@ -1133,6 +1215,11 @@ void WorldSet::checkErrors() {
         sc.errors = 0;
         check_hip(hipMemcpy(dSc_, &sc, sizeof(sc), hipMemcpyHostToDevice), "H2D");
         throw Error(ALIEN_E_STATE, "map build: a bin tile's offset was not published in time");
+    }
+    if (sc.errors & 4) {
+        sc.errors = 0;
+        check_hip(hipMemcpy(dSc_, &sc, sizeof(sc), hipMemcpyHostToDevice), "H2D");
+        throw Error(ALIEN_E_CAPACITY, "a world's energy particles exceeded its particle capacity");
     }
     const int wantCand = std::min(2048, (std::min(sc.candidates, candCap_) + 255) / 256);
     if (wantCand > 2 * shape_.candGrid) {
@@ -1169,17 +1256,24 @@ void WorldSet::step(uint64_t n) {
         }
         return (count == 1 ? graph_ : graphMulti_)[parity_];
     };
+    uint64_t roomUntil = 0;  // with cell radiation: steps of this call covered by the last room check
     for (uint64_t s = 0; s < n;) {
+        if (shape_.hasRadiation && s >= roomUntil) {
+            const uint64_t k = std::min<uint64_t>(n - s, kRadiationSteps);
+            ensureParticleRoom(k);  // (a growth rebuilds the tables; the graphs follow)
+            roomUntil = s + k;
+        }
         // while particles live, every 32 steps their count is read without waiting for the device:
         // once none is left (seen a check later), the particle kernels go
-        if (shape_.hasParticles && s > 0 && s % 32 == 0) checkParticlesAsync();
+        if (shape_.hasParticles && !shape_.hasRadiation && s > 0 && s % 32 == 0) checkParticlesAsync();
         uint64_t done = 1;
         if (timing_) {
             launch_step(makeCtx(parity_), shape_, stream_, recorder_.get());
             check_hip(hipGetLastError(), "kernel launch");
         } else if (useGraph_) {
             // several steps per graph while they fit before the end and the next particle check
-            const uint64_t toCheck = shape_.hasParticles ? 32 - s % 32 : n - s;
+            const uint64_t toCheck =
+                shape_.hasRadiation ? roomUntil - s : (shape_.hasParticles ? 32 - s % 32 : n - s);
             if (n - s >= kStepsPerGraph && toCheck >= kStepsPerGraph) done = kStepsPerGraph;
             check_hip(hipGraphLaunch(graphOf(done), stream_), "hipGraphLaunch");
         } else {
@@ -1202,7 +1296,7 @@ void WorldSet::checkParticlesAsync() {
         const hipError_t q = hipEventQuery(liveReady_);
         if (q == hipSuccess) {
             livePending_ = false;
-            if (*liveHost_ == 0) {
+            if (*liveHost_ == 0 && !shape_.hasRadiation) {
                 shape_.hasParticles = false;
                 invalidateGraphs();
                 return;
@@ -1226,10 +1320,113 @@ void WorldSet::dropParticlesIfNone() {
     livePending_ = false;  // (a read in flight has landed; the count below is newer)
     StepCounters sc{};
     check_hip(hipMemcpy(&sc, dSc_, sizeof(sc), hipMemcpyDeviceToHost), "D2H");
-    if (sc.liveParticles == 0) {
+    if (sc.liveParticles == 0 && !shape_.hasRadiation) {
         shape_.hasParticles = false;
         invalidateGraphs();
     }
+}
+
+void WorldSet::ensureParticleRoom(uint64_t steps) {
+    if (!shape_.hasRadiation) return;
+    synchronize();
+    std::vector<DevCounters> ctr(worlds_.size());
+    check_hip(hipMemcpy(ctr.data(), dCtr_, sizeof(DevCounters) * ctr.size(), hipMemcpyDeviceToHost), "D2H");
+    bool grow = partStage_ == 0;
+    std::vector<int64_t> caps(worlds_.size());
+    for (size_t w = 0; w < worlds_.size(); ++w) {
+        caps[w] = worlds_[w].partCap;
+        if (!active_[w] || !params_[w].radiates) continue;
+        const int64_t need = int64_t(ctr[w].numParticles) + int64_t(steps) * worlds_[w].numCells;
+        if (need > caps[w]) {
+            caps[w] = 2 * need;
+            grow = true;
+        }
+    }
+    if (grow) growParticles(caps);
+}
+
+void WorldSet::release(void* p) {
+    for (size_t k = 0; k < allocs_.size(); ++k)
+        if (allocs_[k].p == p) {
+            (void)hipFree(p);
+            deviceBytes_ -= allocs_[k].bytes;
+            allocs_.erase(allocs_.begin() + std::ptrdiff_t(k));
+            return;
+        }
+}
+
+void WorldSet::growParticles(const std::vector<int64_t>& caps) {
+    synchronize();
+    int64_t total = 0;
+    for (int64_t c : caps) total += c;
+    if (total > kMaxSetParticles)
+        throw Error(ALIEN_E_CAPACITY, "the particle room of the radiating worlds exceeds the particles of one world set");
+    std::vector<DevCounters> ctr(worlds_.size());
+    check_hip(hipMemcpy(ctr.data(), dCtr_, sizeof(DevCounters) * ctr.size(), hipMemcpyDeviceToHost), "D2H");
+    const int stage = int(std::max<int64_t>(total, 1));
+    const int NP = stage + cellCap_;
+    // the particle arrays of both buffers and the per-particle step arrays, reallocated; the
+    // current state (and its next-step bins) moves to the new world offsets
+    HotBuffers nb[2];
+    for (int b = 0; b < 2; ++b) {
+        nb[b] = buf_[b];
+        nb[b].ppos = static_cast<float2*>(alloc(sizeof(float2) * NP));
+        nb[b].pvel = static_cast<float2*>(alloc(sizeof(float2) * NP));
+        nb[b].pen = static_cast<float*>(alloc(sizeof(float) * NP));
+        nb[b].plast = static_cast<int*>(alloc(sizeof(int) * NP));
+        nb[b].pslot = static_cast<int*>(alloc(sizeof(int) * NP));
+        nb[b].pflags = static_cast<uint8_t*>(alloc(NP));
+        nb[b].pid = static_cast<unsigned long long*>(alloc(sizeof(unsigned long long) * NP));
+    }
+    int* pbinOf = static_cast<int*>(alloc(sizeof(int) * NP));
+    const HotBuffers& S = buf_[parity_];
+    HotBuffers& D = nb[parity_];
+    int64_t off = 0;
+    auto d2d = [&](void* dst, const void* src, size_t bytes) {
+        if (bytes) check_hip(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, stream_), "hipMemcpy D2D");
+    };
+    for (size_t w = 0; w < worlds_.size(); ++w) {
+        DevWorld& dw = worlds_[w];
+        const size_t np = size_t(ctr[w].numParticles);
+        const int from = dw.partOff, to = int(off);
+        d2d(D.ppos + to, S.ppos + from, sizeof(float2) * np);
+        d2d(D.pvel + to, S.pvel + from, sizeof(float2) * np);
+        d2d(D.pen + to, S.pen + from, sizeof(float) * np);
+        d2d(D.plast + to, S.plast + from, sizeof(int) * np);
+        d2d(D.pslot + to, S.pslot + from, sizeof(int) * np);
+        d2d(D.pflags + to, S.pflags + from, np);
+        d2d(D.pid + to, S.pid + from, sizeof(unsigned long long) * np);
+        d2d(pbinOf + to, pbinOf_ + from, sizeof(int) * np);
+        dw.partOff = to;
+        dw.partCap = int(caps[w]);
+        off += caps[w];
+    }
+    check_hip(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+    for (int b = 0; b < 2; ++b) {
+        for (void* p : {static_cast<void*>(buf_[b].ppos), static_cast<void*>(buf_[b].pvel), static_cast<void*>(buf_[b].pen),
+                        static_cast<void*>(buf_[b].plast), static_cast<void*>(buf_[b].pslot),
+                        static_cast<void*>(buf_[b].pflags), static_cast<void*>(buf_[b].pid)})
+            release(p);
+        buf_[b] = nb[b];
+    }
+    for (void* p : {static_cast<void*>(pbinOf_), static_cast<void*>(psorted_), static_cast<void*>(want_),
+                    static_cast<void*>(fclaim_), static_cast<void*>(absorbCell_)})
+        release(p);
+    pbinOf_ = pbinOf;
+    psorted_ = static_cast<int*>(alloc(sizeof(int) * NP));
+    want_ = static_cast<int*>(alloc(sizeof(int) * NP));
+    fclaim_ = static_cast<int*>(alloc(sizeof(int) * NP));
+    absorbCell_ = static_cast<int*>(alloc(sizeof(int) * NP));
+    partCap_ = NP;
+    partStage_ = stage;
+    launch_fill_u32(reinterpret_cast<uint32_t*>(fclaim_), uint32_t(kClaimFree), size_t(NP), stream_);
+    launch_fill_u32(reinterpret_cast<uint32_t*>(buf_[parity_ ^ 1].pflags), 0u, (size_t(NP) + 3) / 4, stream_);
+    launch_fill_u32(reinterpret_cast<uint32_t*>(pbinOf_ + partStage_), 0xffffffffu, size_t(cellCap_), stream_);
+    check_hip(hipGetLastError(), "fill kernels");
+    check_hip(hipMemcpy(dWorlds_, worlds_.data(), sizeof(DevWorld) * worlds_.size(), hipMemcpyHostToDevice), "H2D");
+    shape_.partGrid = int(std::max<int64_t>(1, std::min<int64_t>(65536, (total + shape_.partThreads - 1) / shape_.partThreads)));
+    buildTables();  // (particle blocks of the new capacities; the graphs are captured again)
+    synchronize();
 }
 
 void WorldSet::extract(int wi, WorldImage& img) {
@@ -1293,12 +1490,14 @@ void WorldSet::extract(int wi, WorldImage& img) {
     img.plast.resize(np);
     img.pslot.resize(np);
     img.pflags.resize(np);
+    img.pid.resize(np);
     d2h(img.ppos.data(), S.ppos + w.partOff, sizeof(float2) * np);
     d2h(img.pvel.data(), S.pvel + w.partOff, sizeof(float2) * np);
     d2h(img.pen.data(), S.pen + w.partOff, sizeof(float) * np);
     d2h(img.plast.data(), S.plast + w.partOff, sizeof(int) * np);
     d2h(img.pslot.data(), S.pslot + w.partOff, sizeof(int) * np);
     d2h(img.pflags.data(), S.pflags + w.partOff, np);
+    d2h(img.pid.data(), S.pid + w.partOff, sizeof(uint64_t) * np);
     // particles killed in the last step are still in the live range until the next sort
     int keep = 0;
     for (int i = 0; i < np; ++i) {
@@ -1309,6 +1508,7 @@ void WorldSet::extract(int wi, WorldImage& img) {
         img.plast[keep] = img.plast[i];
         img.pslot[keep] = img.pslot[i];
         img.pflags[keep] = img.pflags[i];
+        img.pid[keep] = img.pid[i];
         ++keep;
     }
     img.ppos.resize(keep);
@@ -1317,6 +1517,7 @@ void WorldSet::extract(int wi, WorldImage& img) {
     img.plast.resize(keep);
     img.pslot.resize(keep);
     img.pflags.resize(keep);
+    img.pid.resize(keep);
     img.numParticles = keep;
 }
 

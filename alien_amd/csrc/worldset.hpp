@@ -37,6 +37,9 @@ void validate_world(const alien_world_desc& d, const alien_params& p);
 void check_set_size(uint64_t cells, uint64_t particles);
 // whether new parameters for a world without device-tracked living states need a fresh set
 bool needs_fresh_set(const alien_params& before, const alien_params& after);
+// whether the parameters let cell radiation take energy from cells (radiationProb > 0 and some
+// colour's high-radiation factor or age strength positive)
+bool radiates(const alien_params& p);
 DevParams to_dev_params(const alien_params& p);
 int choose_strip(const std::vector<float2>& pos, int W, int H, int binSize, bool twoRowStages, int& rowsPerStage);
 void strip_shape(DevWorld& w, int strip, int rowsPerStage);
@@ -79,6 +82,7 @@ struct WorldImage {
     std::vector<float> pen;
     std::vector<int> plast, pslot;
     std::vector<uint8_t> pflags;
+    std::vector<uint64_t> pid;  // particle ids
 
     // from transfer objects (ObjectFactory.cuh:46-86: positions wrapped, density 1)
     static WorldImage fromTO(const alien_world_desc& desc, const alien_params& params, const alien_cell_to* cells,
@@ -174,6 +178,15 @@ private:
     // the same without a host synchronisation: acts on the count read at the previous call (if it
     // has arrived) and starts reading the current one
     void checkParticlesAsync();
+    // Cell radiation adds at most one particle per cell and step: before `steps` more steps, every
+    // radiating world gets room for its live particles plus steps times its cells (twice that when
+    // it grows), and the set gets its staging area (a host synchronisation).
+    void ensureParticleRoom(uint64_t steps);
+    // moves the particles to arrays with the given capacities per world (and a staging area)
+    void growParticles(const std::vector<int64_t>& caps);
+    void release(void* p);  // frees one allocation of alloc()
+    void updateRadiation();  // shape_.hasRadiation from the active worlds' parameters
+    void anchorAges(int w);  // worlds_[w].ageTs from its timestep and steps simulated
 
     hipStream_t stream_ = nullptr;
     SideStream side_;  // particle kernels beside the neighbour pass
@@ -200,6 +213,9 @@ private:
     float* energy_ = nullptr;
     float* stiff_ = nullptr;
     unsigned long long* cid_ = nullptr;
+    uint32_t* ageBase_ = nullptr;  // cell ages by slot at each world's ageTs
+    int partStage_ = 0;            // first staging slot of radiation-emitted particles (0: none)
+    int numCellBlocks_ = 0;
     uint8_t* living_[2] = {nullptr, nullptr};  // living states by slot, alternating with the parity
     uint32_t* activation_ = nullptr;
     uint32_t* creature_ = nullptr;
@@ -247,6 +263,9 @@ private:
     // kStepsPerGraph steps from parity 0 / 1 (an even count: the parity is the same after them);
     // consecutive graph launches leave the device idle for ~20 us between them
     static constexpr uint64_t kStepsPerGraph = 4;
+    // steps between the particle-room checks of a set with cell radiation; a radiating world is
+    // built with room for twice that many steps of emissions
+    static constexpr uint64_t kRadiationSteps = 4;
     hipGraphExec_t graphMulti_[2] = {nullptr, nullptr};
 
     class EventRecorder;

@@ -86,8 +86,10 @@ typedef struct alien_particle_to {
  * Physics subset of SimulationParameters (source/EngineInterface/SimulationParameters.h:18-206,
  * Motion.h:7-43, SimulationParametersZoneValues.h:10-24).  alien_params_default() fills ALIEN's
  * defaults.  The processes of the reference that this path does not implement are fixed at these
- * values (no field here): radiation emission off, particle splitting and transformation off,
- * numZones 0, rigidity 0, cellDeathConsequences None, cellDeathProbability 0.
+ * values (no field here): radiation sources none (numRadiationSources 0), external energy off,
+ * particle splitting and transformation off, numZones 0, rigidity 0, cellDeathConsequences None,
+ * cellDeathProbability 0.  Cell radiation (CellProcessor::radiation, CellProcessor.cuh:725-773)
+ * is simulated: its fields close the struct.
  */
 typedef struct alien_params {
     float timestepSize;                /* SimulationParameters.h:61, default 1.0           */
@@ -108,6 +110,13 @@ typedef struct alien_params {
     float radiationAbsorption[ALIEN_MAX_COLORS];    /* ZoneValues.h:14, 1.0             */
     float cellMaxForce[ALIEN_MAX_COLORS];           /* ZoneValues.h:19, 0.8             */
     float cellMinEnergy[ALIEN_MAX_COLORS];          /* ZoneValues.h:20, 50.0            */
+    float radiationProb;               /* SimulationParameters.h:79, default 0.03, in [0,1] */
+    float radiationVelocityMultiplier; /* SimulationParameters.h:80, default 1.0           */
+    float radiationVelocityPerturbation; /* SimulationParameters.h:81, default 0.5         */
+    int32_t radiationMinCellAge[ALIEN_MAX_COLORS];        /* SimulationParameters.h:82, 0  */
+    float highRadiationFactor[ALIEN_MAX_COLORS];          /* SimulationParameters.h:85, 0  */
+    float highRadiationMinCellEnergy[ALIEN_MAX_COLORS];   /* SimulationParameters.h:86, 500 */
+    float radiationCellAgeStrength[ALIEN_MAX_COLORS];     /* ZoneValues.h:17, 0.00002     */
 } alien_params;
 
 /* World geometry and capacity (GeneralSettings worldSizeX/worldSizeY), and the key of the world's
@@ -318,7 +327,7 @@ uint64_t alien_server_plan_admission(uint64_t budgetBytes, uint64_t oldSetBytes,
  * the `unsimulated` output of alien_settings_read). */
 #define ALIEN_SETTINGS_ZONES 0x1u                   /* spots (numZones > 0)                       */
 #define ALIEN_SETTINGS_RADIATION_SOURCES 0x2u       /* particle sources                           */
-#define ALIEN_SETTINGS_RADIATION 0x4u               /* cell radiation / high radiation factors    */
+#define ALIEN_SETTINGS_RADIATION 0x4u               /* radiation absorption penalties             */
 #define ALIEN_SETTINGS_CELL_DEATH 0x8u              /* death consequences / death probability     */
 #define ALIEN_SETTINGS_PARTICLE_SPLIT 0x10u         /* finite particle split energy               */
 #define ALIEN_SETTINGS_RIGIDITY 0x20u               /* rigidity != 0                              */

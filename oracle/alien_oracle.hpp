@@ -37,6 +37,12 @@
 //       that becomes Dying become Detaching unless they are or become Dying themselves (the
 //       reference's atomicExch order is resolved that way).
 //   R10 dead particles are removed with a stable compaction.
+//   R18 cell radiation (CellProcessor.cuh:725-773) runs over the cells in the order of the last
+//       step's map (the cells' order at the start of the step), before the step's maps are built;
+//       the particles it creates join the step's particle map behind the live ones, in that order
+//       (the reference appends them with atomic slot grabs).  A created particle's id is
+//       kCreatedParticleId | timestep << 32 | index of the emitting cell (the reference takes ids
+//       from a shared atomic counter).
 //   R12 SPH sums gathered per cell in visiting order.
 //   R13 collision neighbours use the uncorrected (seam-blind) distance like Map.cuh:189.
 // Three rules change only the rounding of the FP32 build (in exact arithmetic they are the
@@ -214,6 +220,42 @@ inline void philox4x32_10(uint32_t c[4], uint32_t k0, uint32_t k1) {
     }
 }
 constexpr uint32_t kStreamForceDecay = 1;  // CellProcessor.cuh:367-386
+// CellProcessor::radiation (CellProcessor.cuh:725-773): the radiation draw, the energy fraction,
+// the perturbation angle
+constexpr uint32_t kStreamRadiation = 2, kStreamRadiationEnergy = 3, kStreamRadiationAngle = 4;
+constexpr uint64_t kCreatedParticleId = uint64_t(1) << 63;
+
+// Math::unitVectorOfAngle (Math.cuh:134-138): {sin a, -cos a}, a in degrees in [0, 360).  The
+// reference calls sinf/cosf; here the sine and cosine are their Taylor series (to x^13 and x^14,
+// Horner form, coefficients 1/k! rounded to float) of the angle reduced to a quarter turn, so that
+// the FP32 build and the GPU compute the same bits.
+template <class R> inline V2<R> unitVectorOfAngle(R deg) {
+    int q = int(deg / R(90));
+    q = q < 0 ? 0 : (q > 3 ? 3 : q);
+    const R x = (deg - R(90) * R(q)) * R(0.017453292f);
+    const R x2 = x * x;
+    const R sn = x * (R(1) + x2 * (R(-0.16666667f) +
+                                   x2 * (R(0.008333334f) +
+                                         x2 * (R(-0.0001984127f) +
+                                               x2 * (R(2.7557319e-06f) + x2 * (R(-2.5052108e-08f) + x2 * R(1.6059044e-10f)))))));
+    const R cs = R(1) + x2 * (R(-0.5f) +
+                              x2 * (R(0.041666668f) +
+                                    x2 * (R(-0.0013888889f) +
+                                          x2 * (R(2.4801588e-05f) +
+                                                x2 * (R(-2.7557319e-07f) + x2 * (R(2.0876757e-09f) + x2 * R(-1.1470746e-11f)))))));
+    R s = sn, c = cs;
+    if (q == 1) {
+        s = cs;
+        c = -sn;
+    } else if (q == 2) {
+        s = -sn;
+        c = -cs;
+    } else if (q == 3) {
+        s = -cs;
+        c = sn;
+    }
+    return {s, -c};
+}
 inline float uniformDraw(uint64_t seed, uint64_t timestep, uint64_t id, uint32_t purpose) {
     uint32_t c[4] = {uint32_t(id), uint32_t(id >> 32), uint32_t(timestep), purpose};
     philox4x32_10(c, uint32_t(seed), uint32_t(seed >> 32));
@@ -292,6 +334,10 @@ template <class R> struct Params {
     R radiationAbsorption[ALIEN_MAX_COLORS];
     R cellMaxForce[ALIEN_MAX_COLORS];
     R cellMinEnergy[ALIEN_MAX_COLORS];
+    R radiationProb, radiationVelocityMultiplier, radiationVelocityPerturbation;
+    int radiationMinCellAge[ALIEN_MAX_COLORS];
+    R highRadiationFactor[ALIEN_MAX_COLORS], highRadiationMinCellEnergy[ALIEN_MAX_COLORS];
+    R radiationCellAgeStrength[ALIEN_MAX_COLORS];
     static Params from(const alien_params& p) {
         Params q;
         q.timestepSize = R(p.timestepSize);
@@ -313,7 +359,14 @@ template <class R> struct Params {
             q.radiationAbsorption[c] = R(p.radiationAbsorption[c]);
             q.cellMaxForce[c] = R(p.cellMaxForce[c]);
             q.cellMinEnergy[c] = R(p.cellMinEnergy[c]);
+            q.radiationMinCellAge[c] = p.radiationMinCellAge[c];
+            q.highRadiationFactor[c] = R(p.highRadiationFactor[c]);
+            q.highRadiationMinCellEnergy[c] = R(p.highRadiationMinCellEnergy[c]);
+            q.radiationCellAgeStrength[c] = R(p.radiationCellAgeStrength[c]);
         }
+        q.radiationProb = R(p.radiationProb);
+        q.radiationVelocityMultiplier = R(p.radiationVelocityMultiplier);
+        q.radiationVelocityPerturbation = R(p.radiationVelocityPerturbation);
         return q;
     }
 };
@@ -382,6 +435,8 @@ public:
             throw std::invalid_argument("cellMaxForceDecayProb must lie in [0, 1]");
         if (prm.motionType != ALIEN_MOTION_FLUID && prm.motionType != ALIEN_MOTION_COLLISION)
             throw std::invalid_argument("bad motionType");
+        if (!(prm.radiationProb >= 0.0f && prm.radiationProb <= 1.0f))
+            throw std::invalid_argument("radiationProb must lie in [0, 1]");
         rawParams = prm;
         p = Params<R>::from(prm);
         densityResetPending = true;  // SimulationCudaFacade.cu:667-669
@@ -1113,6 +1168,51 @@ public:
         }
     }
 
+    // ---- cell radiation: CellProcessor.cuh:725-773, RadiationProcessor.cuh:248-364 (R9, R18) ----
+    // A non-barrier cell radiates when its draw is below radiationProb: its factor is
+    // highRadiationFactor above highRadiationMinCellEnergy plus radiationCellAgeStrength above
+    // radiationMinCellAge; it loses 2 u E factor / radiationProb (capped at E - 1, and only if
+    // E > 1), carried off by a particle with the cell's velocity times the multiplier plus a unit
+    // vector of random angle times the perturbation, placed 1.5 ahead of the cell along that
+    // velocity and one velocity back; a particle below NEAR_ZERO energy is not created (its energy
+    // is lost).  No radiation sources, no external energy.
+    void radiation() {
+        if (!(p.radiationProb > R(0))) return;
+        for (size_t i = 0; i < cells.size(); ++i) {
+            auto& c = cells[i];
+            if (c.barrier) continue;
+            if (!(R(uniformDraw(seed, timestep, c.id, kStreamRadiation)) < p.radiationProb)) continue;
+            const int col = c.color % ALIEN_MAX_COLORS;
+            R factor = R(0);
+            if (c.energy > p.highRadiationMinCellEnergy[col]) factor = factor + p.highRadiationFactor[col];
+            if (int64_t(c.age) > int64_t(p.radiationMinCellAge[col])) factor = factor + p.radiationCellAgeStrength[col];
+            if (!(factor > R(0))) continue;
+            R loss = c.energy * factor;
+            loss = loss / p.radiationProb;
+            loss = R(2) * loss * R(uniformDraw(seed, timestep, c.id, kStreamRadiationEnergy));
+            if (!(c.energy > R(1))) continue;
+            const V2<R> u = unitVectorOfAngle(R(uniformDraw(seed, timestep, c.id, kStreamRadiationAngle)) * R(360));
+            const V2<R> vel = c.vel * p.radiationVelocityMultiplier + u * p.radiationVelocityPerturbation;
+            V2<R> pos = c.pos + normalized(vel) * R(1.5f) - vel;
+            torus.correctPosition(pos);
+            if (loss > c.energy - R(1)) loss = c.energy - R(1);
+            if (loss > K<R>::NEAR_ZERO) {
+                Particle<R> q{};
+                q.id = kCreatedParticleId | ((timestep & 0x7fffffffull) << 32) | uint64_t(uint32_t(i));
+                q.pos = pos;
+                q.vel = vel;
+                q.energy = loss;
+                q.color = uint8_t(col);
+                q.selected = 0;
+                q.lastAbsorbedCellId = 0;
+                q.hasLast = false;
+                q.alive = true;
+                parts.push_back(q);
+            }
+            c.energy = c.energy - loss;
+        }
+    }
+
     // ---- inner friction: CellProcessor.cuh:678-705 (R11) ----------------------------------------
     void innerFriction() {
         const R f = p.innerFriction;
@@ -1202,6 +1302,7 @@ public:
             densityResetPending = false;
         }
         const bool considerAngles = (timestep % 3 == 0);  // SimulationKernelsLauncher.cu:31
+        radiation();  // (R18: in the cells' order of the last map, before the maps)
         sortCells();
         sortParticles();
         const int n = int(cells.size());

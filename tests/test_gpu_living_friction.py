@@ -76,8 +76,8 @@ def test_force_decay_bit_exact(motion):
 
 
 def test_alien_default_parameters_bit_exact():
-    """ALIEN's defaults (inner friction 0.3, force decay 0.2 above 0.8) on the 512x512 soft-body
-    world, which round 1 of this path refused with ALIEN_E_ARG."""
+    """ALIEN's defaults (inner friction 0.3, force decay 0.2 above 0.8, cell radiation with
+    probability 0.03 and age strength 0.00002) on the 512x512 soft-body world."""
     base = worlds.plain_soft_body()
     p = abi.alien_default_params(motionType=MOTION_COLLISION, cellMaxCollisionDistance=1.3, friction=0.001)
     w = worlds.World("plain_soft_body_alien_defaults", base.size, p, base.cells, base.particles, 30, base.seed)

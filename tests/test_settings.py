@@ -43,9 +43,22 @@ def test_read_alien_settings_file():
         assert p.cellMinEnergy[k] == f32(cell[f"min energy[{k}]"])
         assert p.radiationAbsorption[k] == f32(sp["radiation"][f"absorption[{k}]"])
     assert p.innerFriction == f32(0.3)  # not part of ALIEN's settings files: the default stays
-    # the file switches on zones (3 spots), particle sources, cell radiation and cell death
+    # cell radiation (AuxiliaryDataParserService.cpp:203-268), simulated
+    rad, high = sp["radiation"], sp["high radiation"]
+    assert p.radiationProb == f32(rad["probability"]) == f32(0.03)
+    assert p.radiationVelocityMultiplier == f32(rad["velocity multiplier"])
+    assert p.radiationVelocityPerturbation == f32(rad["velocity perturbation"])
+    for k in range(7):
+        assert p.radiationCellAgeStrength[k] == f32(rad[f"factor[{k}]"])
+        assert p.highRadiationFactor[k] == f32(high[f"factor[{k}]"])
+        assert p.highRadiationMinCellEnergy[k] == f32(high[f"min cell energy[{k}]"])
+        assert p.radiationMinCellAge[k] == int(float(rad.get(f"min cell age[{k}]", "0")))
+    assert p.radiationCellAgeStrength[1] == f32(0.000093) and p.highRadiationFactor[0] == f32(0.000478)
+    # the file switches on zones (3 spots), particle sources and cell death; its absorption
+    # penalties are 0
     assert int(sp["spots"]["num spots"]) == 3
-    expect = abi.SETTINGS_ZONES | abi.SETTINGS_RADIATION | abi.SETTINGS_CELL_DEATH
+    assert all(float(v) == 0.0 for key, v in rad.items() if "penalty" in key)
+    expect = abi.SETTINGS_ZONES | abi.SETTINGS_CELL_DEATH
     if int(sp["particle sources"]["num sources"]) > 0:
         expect |= abi.SETTINGS_RADIATION_SOURCES
     assert unsim == expect
@@ -58,6 +71,13 @@ def test_write_read_round_trip(motion):
     for k in range(7):
         p.radiationAbsorption[k] = 0.1 * (k + 1)
         p.cellMinEnergy[k] = 40.0 + k
+        p.radiationMinCellAge[k] = 10 * k
+        p.highRadiationFactor[k] = 0.001 * k
+        p.highRadiationMinCellEnergy[k] = 300.0 + k
+        p.radiationCellAgeStrength[k] = 0.00001 * (k + 1)
+    p.radiationProb = 0.05
+    p.radiationVelocityMultiplier = 0.75
+    p.radiationVelocityPerturbation = 0.25
     text = engine.write_settings((640, 480), p, timestep=1234)
     tree = json.loads(text)  # valid JSON, every leaf a string like ALIEN's
     assert tree["general"]["world size"]["x"] == "640"
