@@ -48,6 +48,9 @@ def parse():
     ap.add_argument("--batch-worlds", type=int, default=256)
     ap.add_argument("--batch-steps", type=int, default=500,
                     help="steps per call of the batched-worlds leg of the default line (config 3: 500)")
+    ap.add_argument("--radiation", action="store_true",
+                    help="large fluid world with ALIEN's default cell radiation (probability 0.03, age strength "
+                         "0.00002 per colour): the cost of the process, not the headline workload")
     ap.add_argument("--no-batch", action="store_true",
                     help="default line without its batched-worlds leg (batched vs per-world sequential)")
     ap.add_argument("--serving-seconds", type=float, default=60.0, help="arrival window of the serving run")
@@ -275,6 +278,9 @@ def main():
 
     world = worlds.large_fluid(size=(args.size, args.size), num_cells=args.cells, num_particles=args.particles,
                                steps=args.steps)
+    if args.radiation:
+        for c in range(7):
+            world.params.radiationCellAgeStrength[c] = 0.00002
     n = len(world.cells)
     t0 = time.perf_counter()
     sim = engine.Simulation.from_world(world)  # host arrays -> device (set_data)
@@ -335,7 +341,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": "large_fluid", "world": f"{args.size}x{args.size}", "cells": n,
                    "particles": args.particles, "motion": "fluid (SPH, radius 1.3)", "seed": world.seed,
-                   "device": device},
+                   "device": device, "cell_radiation": "ALIEN defaults" if args.radiation else "off"},
         "roofline": roofline,
         # the whole step against HBM: algorithmic bytes of all cells and particles per step
         "step_bandwidth_GBs": step_bytes / (elapsed / args.steps) / 1e9,
