@@ -106,6 +106,7 @@ struct DevWorld {
     // timestep at which the cells' base ages (ageBase) were taken: a non-barrier cell ages by one
     // every step, so its age at the start of step t is ageBase + (t - ageTs)
     unsigned long long ageTs;
+    int active;  // 0: left out of every launch (frozen)
 };
 
 // per-world device counters
@@ -229,10 +230,14 @@ struct Ctx {
     const int2* bondBlocks;  // the same over the worlds with bond work
     const int2* binBlocks;   // per block: (world, first global bin index) over cell bins
     const int2* pbinBlocks;  // per block: (world, first global bin index) over particle bins
-    const int2* partBlocks;  // per block: (world, first global particle index)
+    // per block of partBlockParticles live particles: (world, first global particle index), over
+    // the active worlds' live ranges; written by the host when the tables are built and by the
+    // particle bin sort for the ranges it publishes (*numPartBlocks entries)
+    int2* partBlocks;
+    int* numPartBlocks;
+    int partBlockParticles;
     const int2* tileBlocks;  // per block: (world, tile index), -1 for padding blocks
     const int2* livingBlocks;  // per block: (world, first global cell index) over livingWork worlds
-    int numPartBlocks;
     int numCellBlocks, cellBlockCells;  // entries of cellBlocks, cells per entry
 };
 

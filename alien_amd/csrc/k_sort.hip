@@ -418,6 +418,18 @@ __global__ void k_part_binsort(Ctx c) {
             c.ctr[wi].numParticles = live;
             if (live > 0) atomicAdd(&c.sc->liveParticles, live);
         }
+        __syncthreads();
+        // the table of live particle blocks, world by world
+        const int ppb = c.partBlockParticles;
+        int base = 0;
+        for (int wi = 0; wi < c.numWorlds; ++wi) {
+            const DevWorld& w = c.worlds[wi];
+            if (!w.active) continue;
+            const int nb = (c.ctr[wi].numParticles + ppb - 1) / ppb;
+            for (int j = int(threadIdx.x); j < nb; j += int(blockDim.x)) c.partBlocks[base + j] = make_int2(wi, w.partOff + j * ppb);
+            base += nb;
+        }
+        if (threadIdx.x == 0) *c.numPartBlocks = base;
     }
     int2 blk = c.pbinBlocks[blockIdx.x];
     const DevWorld w = c.worlds[blk.x];

@@ -133,9 +133,11 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t& t
     return res;
 }
 
-// grid-stride iteration over the particle block table, live particles only
+// grid-stride iteration over the blocks of live particles (the table the particle bin sort writes
+// for the live ranges it publishes)
 template <class F> __device__ __forceinline__ void for_live_particles(const Ctx& c, F&& f) {
-    for (int e = blockIdx.x; e < c.numPartBlocks; e += gridDim.x) {
+    const int n = *c.numPartBlocks;
+    for (int e = blockIdx.x; e < n; e += gridDim.x) {
         const int2 blk = c.partBlocks[e];
         const int live = c.ctr[blk.x].numParticles;
         const DevWorld& w = c.worlds[blk.x];

@@ -187,6 +187,27 @@ WorldImage WorldImage::fromTO(const alien_world_desc& desc, const alien_params& 
     img.bang.assign(size_t(kMaxBonds) * n, 0.0f);
     img.energy.resize(n);
     img.stiff.resize(n);
+    // Slots (the cells' stable identities, which index their cold state) are given in spatial
+    // order, bin by bin of the cell map: the map keeps cells in bin order, so the cold state of
+    // neighbouring map entries then lies close in memory.  Only locality depends on it.
+    {
+        const int bs = bin_size_for(params);
+        const int nbx = std::max(1, W / bs), nby = std::max(1, H / bs);
+        std::vector<int> key(n), cnt(size_t(nbx) * nby + 1, 0);
+        for (int i = 0; i < n; ++i) {
+            float x = cells[i].pos[0], y = cells[i].pos[1];
+            int k = 0;
+            if (std::isfinite(x) && std::isfinite(y)) {
+                host_wrap(x, y, W, H);
+                const int bx = std::min(int(x) / bs, nbx - 1), by = std::min(int(y) / bs, nby - 1);
+                k = std::max(0, by) * nbx + std::max(0, bx);
+            }
+            key[i] = k;
+            ++cnt[size_t(k) + 1];
+        }
+        for (size_t b = 1; b < cnt.size(); ++b) cnt[b] += cnt[b - 1];
+        for (int i = 0; i < n; ++i) img.slot[i] = cnt[size_t(key[i])]++;
+    }
     for (int i = 0; i < n; ++i) {
         const alien_cell_to& c = cells[i];
         if (c.numConnections > kMaxBonds) throw Error(ALIEN_E_ARG, "numConnections above 6");
@@ -199,7 +220,7 @@ WorldImage WorldImage::fromTO(const alien_world_desc& desc, const alien_params& 
         uint32_t maxConn = c.maxConnections > 15 ? 15u : c.maxConnections;
         img.flags[i] = uint32_t(c.numConnections) | (maxConn << kMaxConnShift) | (uint32_t(c.color) << kColorShift) |
                        (c.barrier ? kBarrierBit : 0u);
-        img.slot[i] = i;
+        const int sl = img.slot[i];
         for (int b = 0; b < c.numConnections; ++b) {
             int t = c.connections[b].cellIndex;
             if (t < 0 || t >= n) throw Error(ALIEN_E_ARG, "bond cellIndex out of range");
@@ -207,8 +228,8 @@ WorldImage WorldImage::fromTO(const alien_world_desc& desc, const alien_params& 
             img.brest[size_t(b) * n + i] = c.connections[b].distance;
             img.bang[size_t(b) * n + i] = c.connections[b].angleFromPrevious;
         }
-        img.energy[i] = c.energy;
-        img.stiff[i] = c.stiffness;
+        img.energy[sl] = c.energy;
+        img.stiff[sl] = c.stiffness;
         for (int b = 0; b < c.numConnections; ++b) {  // bonds come in pairs a->b, b->a (TOs.cuh:30-35)
             const int t = c.connections[b].cellIndex;
             const alien_cell_to& o = cells[t];
@@ -217,16 +238,16 @@ WorldImage WorldImage::fromTO(const alien_world_desc& desc, const alien_params& 
             for (int k = 0; k < c.numConnections; ++k) same += c.connections[k].cellIndex == t;
             if (t == i || back != 1 || same != 1) throw Error(ALIEN_E_ARG, "bonds must be symmetric and unique");
         }
-        h.cellId[i] = c.id;
-        h.age[i] = c.age;
-        h.creatureId[i] = c.creatureId;
-        h.mutationId[i] = c.mutationId;
-        h.activationTime[i] = c.activationTime;
-        h.livingState[i] = c.livingState;
-        h.cellFunction[i] = c.cellFunction;
-        h.ancestorMutationId[i] = c.ancestorMutationId;
-        h.selected[i] = c.selected;
-        h.genomeComplexity[i] = c.genomeComplexity;
+        h.cellId[sl] = c.id;
+        h.age[sl] = c.age;
+        h.creatureId[sl] = c.creatureId;
+        h.mutationId[sl] = c.mutationId;
+        h.activationTime[sl] = c.activationTime;
+        h.livingState[sl] = c.livingState;
+        h.cellFunction[sl] = c.cellFunction;
+        h.ancestorMutationId[sl] = c.ancestorMutationId;
+        h.selected[sl] = c.selected;
+        h.genomeComplexity[sl] = c.genomeComplexity;
     }
     img.ppos.resize(np);
     img.pvel.resize(np);
@@ -543,6 +564,7 @@ void WorldSet::build(const std::vector<Source>& sources) {
             throw Error(ALIEN_E_CAPACITY, "the particle room of the radiating worlds exceeds the particles of one world set");
         w.partCap = int(cap);
         w.ageTs = ts - h.stepsSinceSet;
+        w.active = 1;
         w.bondWork = work ? 1 : 0;
         w.livingWork = living ? 1 : 0;
         w.seed = h.desc.randomSeed;
@@ -745,7 +767,6 @@ void WorldSet::build(const std::vector<Source>& sources) {
     parity_ = 0;
     shape_.candGrid = 16;
     // one particle block per workgroup (blocks past a world's live particles exit at once)
-    shape_.partGrid = int(std::max<int64_t>(1, std::min<int64_t>(65536, (partOff + shape_.partThreads - 1) / shape_.partThreads)));
 
     std::vector<DevCounters> ctr(worlds_.size());
     for (size_t i = 0; i < worlds_.size(); ++i) {
@@ -883,6 +904,7 @@ void chunk_shape(std::vector<DevWorld>& worlds, const std::vector<DevParams>& pa
 }
 
 void WorldSet::buildTables() {
+    if (stream_) synchronize();  // (the tables and the live counts read below are in use until then)
     for (auto& a : tableAllocs_) (void)hipFree(a.p);
     tableAllocs_.clear();
     std::vector<int2> cells, bonds, bins, pbins, parts, living;
@@ -895,7 +917,6 @@ void WorldSet::buildTables() {
             for (int s = 0; s < w.numCells; s += ct) bonds.push_back(make_int2(wi, w.cellOff + s));
         if (livingWork_[wi])
             for (int s = 0; s < w.numCells; s += ct) living.push_back(make_int2(wi, w.cellOff + s));
-        for (int s = 0; s < w.partCap; s += pt) parts.push_back(make_int2(wi, w.partOff + s));
         for (int s = 0; s < w.nbx * w.nby; s += bt) bins.push_back(make_int2(wi, w.binOff + s));
         for (int s = 0; s < w.pnbx * w.pnby; s += bt) pbins.push_back(make_int2(wi, w.pbinOff + s));
     }
@@ -925,11 +946,34 @@ void WorldSet::buildTables() {
             if (t < T) tileTable[b] = tiles[t];
         }
     }
+    // particle blocks: a table sized for every active world's particle capacity, holding the
+    // blocks of the live particles (the particle bin sort rewrites it every step); the grid of the
+    // particle kernels follows the live particles (and the cells that can radiate)
+    std::vector<DevCounters> ctr(worlds_.size());
+    if (!worlds_.empty())
+        check_hip(hipMemcpy(ctr.data(), dCtr_, sizeof(DevCounters) * ctr.size(), hipMemcpyDeviceToHost), "D2H");
+    size_t partCapBlocks = 0;
+    int64_t live = 0, radCells = 0;
+    for (int wi = 0; wi < int(worlds_.size()); ++wi) {
+        if (!active_[wi]) continue;
+        const DevWorld& w = worlds_[wi];
+        partCapBlocks += size_t((w.partCap + pt - 1) / pt);
+        for (int s = 0; s < ctr[wi].numParticles; s += pt) parts.push_back(make_int2(wi, w.partOff + s));
+        live += ctr[wi].numParticles;
+        if (params_[wi].radiates) radCells += w.numCells;
+    }
+    {
+        std::vector<int2> table(std::max<size_t>(partCapBlocks, 1), make_int2(0, 0));
+        std::copy(parts.begin(), parts.end(), table.begin());
+        partBlocks_ = up(table);
+        const std::vector<int2> count(1, make_int2(int(parts.size()), 0));
+        numPartBlocks_ = reinterpret_cast<int*>(up(count));
+    }
+    shape_.partGrid = int(std::max<int64_t>(64, std::min<int64_t>(65536, (live + pt - 1) / pt + (radCells / 8 + pt - 1) / pt)));
     cellBlocks_ = up(cells);
     bondBlocks_ = up(bonds);
     binBlocks_ = up(bins);
     pbinBlocks_ = up(pbins);
-    partBlocks_ = up(parts);
     tileBlocks_ = up(tileTable);
     livingBlocks_ = up(living);
     shape_.livingBlocks = int(living.size());
@@ -937,11 +981,10 @@ void WorldSet::buildTables() {
     shape_.bondBlocks = int(bonds.size());
     // the bond kernels run when an active world can have a bonded or barrier cell
     shape_.hasBondWork = !bonds.empty();
-    shape_.partBlocks = int(parts.size());
+    shape_.partBlocks = int(partCapBlocks);
     shape_.binBlocks = int(bins.size());
     shape_.pbinBlocks = int(pbins.size());
     shape_.tileBlocks = int(tileTable.size());
-    numPartBlocks_ = int(parts.size());
     numCellBlocks_ = int(cells.size());
     shape_.hasFluid = shape_.hasCollision = false;
     for (const DevParams& p : params_) (p.motion == 0 ? shape_.hasFluid : shape_.hasCollision) = true;
@@ -982,6 +1025,8 @@ void WorldSet::setBlockSize(int threads) {
 void WorldSet::setActive(int w, bool on) {
     if (active_.at(w) == uint8_t(on)) return;
     active_[w] = on;
+    worlds_[w].active = on ? 1 : 0;
+    check_hip(hipMemcpy(dWorlds_ + w, &worlds_[w], sizeof(DevWorld), hipMemcpyHostToDevice), "H2D");
     if (on) {
         synchronize();
         anchorAges(w);
@@ -1106,6 +1151,7 @@ Ctx WorldSet::makeCtx(int parity) const {
     c.partBlocks = partBlocks_;
     c.tileBlocks = tileBlocks_;
     c.numPartBlocks = numPartBlocks_;
+    c.partBlockParticles = shape_.partThreads;
     c.numCellBlocks = numCellBlocks_;
     c.cellBlockCells = shape_.cellThreads;
     return c;
@@ -1424,7 +1470,6 @@ void WorldSet::growParticles(const std::vector<int64_t>& caps) {
     launch_fill_u32(reinterpret_cast<uint32_t*>(pbinOf_ + partStage_), 0xffffffffu, size_t(cellCap_), stream_);
     check_hip(hipGetLastError(), "fill kernels");
     check_hip(hipMemcpy(dWorlds_, worlds_.data(), sizeof(DevWorld) * worlds_.size(), hipMemcpyHostToDevice), "H2D");
-    shape_.partGrid = int(std::max<int64_t>(1, std::min<int64_t>(65536, (total + shape_.partThreads - 1) / shape_.partThreads)));
     buildTables();  // (particle blocks of the new capacities; the graphs are captured again)
     synchronize();
 }

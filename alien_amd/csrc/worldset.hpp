@@ -248,7 +248,7 @@ private:
     int2 *cellBlocks_ = nullptr, *bondBlocks_ = nullptr, *binBlocks_ = nullptr, *pbinBlocks_ = nullptr;
     int2* partBlocks_ = nullptr;
     int2* tileBlocks_ = nullptr;
-    int numPartBlocks_ = 0;
+    int* numPartBlocks_ = nullptr;  // (device) entries of partBlocks_
     std::vector<uint8_t> bondWork_;  // per world: a bonded or barrier cell exists or can appear by fusion
     std::vector<uint8_t> active_;    // per world: included in the launch tables
     std::vector<int> stripChoice_;   // per world: strip width chosen from its cell density
