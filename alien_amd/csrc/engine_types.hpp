@@ -147,6 +147,16 @@ struct HotBuffers {
     unsigned long long* pid;  // particle id
 };
 
+// a particle emitted by cell radiation, in its staging slot (one record: one cache line touched to
+// write it and one to read it)
+struct __align__(16) StagedParticle {
+    float2 pos;  // pre-move position
+    float2 vel;
+    float energy;
+    uint32_t color;
+    unsigned long long id;
+};
+
 struct Ctx {
     int numWorlds;
     int cellCap;  // total cells (= sum of numCells)
@@ -166,6 +176,7 @@ struct Ctx {
     // behind the worlds' particle ranges (partStage = 0: no staging area), and enter the step's
     // particle map like the live particles.
     int partStage;
+    StagedParticle* staged;  // [cellCap]: the particle of staging slot partStage + i at staged[i]
     // living states of the worlds with livingWork, cold by slot: `living` holds the state at the
     // start of the step, `livingNext` the other buffer (the step's phases alternate between the
     // two and leave the end-of-step state in `livingNext`, the next step's `living`)

@@ -12,6 +12,11 @@
 
 namespace alien {
 
+// start-of-step position of particle map entry src (a live particle or a staging slot)
+__device__ __forceinline__ f2 pre_move_pos(const Ctx& c, int src) {
+    return c.partStage > 0 && src >= c.partStage ? to_f2(c.staged[src - c.partStage].pos) : to_f2(c.A.ppos[src]);
+}
+
 // The particle map winner of the slot of position p (R2): the first particle of the bin whose
 // pre-move slot matches; particle k fuses into it when it is another particle closer than sqrt(0.5)
 // after the move.  Returns the target or -1.
@@ -23,7 +28,7 @@ __device__ __forceinline__ int particle_target(const Ctx& c, const DevWorld& w, 
     int t = -1;
     for (int m = s; m < e; ++m) {
         int qx, qy;
-        slot_of(to_f2(c.A.ppos[c.psorted[m]]), w.W, w.H, qx, qy);
+        slot_of(pre_move_pos(c, c.psorted[m]), w.W, w.H, qx, qy);
         if (qx == sx && qy == sy) {
             t = m;
             break;
@@ -182,15 +187,14 @@ __global__ void k_radiation(Ctx c) {
     const f2 p = torus_pos(sub(add(to_f2(c.A.pos[i]), mul(unit(v), 1.5f)), v), w.W, w.H);
     if (loss > e - 1.0f) loss = e - 1.0f;
     if (loss > kNearZero) {
-        const int st = c.partStage + i;
-        c.A.ppos[st] = to_float2(p);
-        c.A.pvel[st] = to_float2(v);
-        c.A.pen[st] = loss;
-        c.A.plast[st] = kNone;
-        c.A.pslot[st] = kNone;
-        c.A.pflags[st] = uint8_t(col & kParticleColorMask);
-        c.A.pid[st] = kCreatedParticleId | ((ts & 0x7fffffffull) << 32) | (unsigned long long)uint32_t(i - w.cellOff);
-        pbin_for_next_step(c, w, st, p, c.count);
+        StagedParticle r;
+        r.pos = to_float2(p);
+        r.vel = to_float2(v);
+        r.energy = loss;
+        r.color = uint32_t(col) & kParticleColorMask;
+        r.id = kCreatedParticleId | ((ts & 0x7fffffffull) << 32) | (unsigned long long)uint32_t(i - w.cellOff);
+        c.staged[i] = r;
+        pbin_for_next_step(c, w, c.partStage + i, p, c.count);
     }
     c.energy[sl] = e - loss;
 }

@@ -334,21 +334,20 @@ __global__ void k_part_scatter(Ctx c) {
         if (gb < 0) return;
         scatter_particle(c, w, gb, i);
     });
-    // the particles cell radiation emitted this step, from their staging slots (cleared once read)
-    if (c.partStage > 0)
-        for (int e = blockIdx.x; e < c.numCellBlocks; e += gridDim.x) {
-            const int2 blk = c.cellBlocks[e];
-            const DevWorld& w = c.worlds[blk.x];
-            for (int t = int(threadIdx.x); t < c.cellBlockCells; t += int(blockDim.x)) {
-                const int i = blk.y + t;
-                if (i >= w.cellOff + w.numCells) break;
-                const int st = c.partStage + i;
-                const int gb = c.pbinOf[st];
-                if (gb < 0) continue;
-                c.pbinOf[st] = -1;
-                scatter_particle(c, w, gb, st);
-            }
-        }
+}
+
+// the particles cell radiation emitted this step, from their staging slots (cleared once read); one
+// workgroup per cell block
+__global__ void k_part_scatter_staged(Ctx c) {
+    const int2 blk = c.cellBlocks[blockIdx.x];
+    const DevWorld& w = c.worlds[blk.x];
+    const int i = blk.y + int(threadIdx.x);
+    if (i >= w.cellOff + w.numCells) return;
+    const int st = c.partStage + i;
+    const int gb = c.pbinOf[st];
+    if (gb < 0) return;
+    c.pbinOf[st] = -1;
+    scatter_particle(c, w, gb, st);
 }
 
 // sort the entries of one bin ascending (R1 stable order), small bins in registers
@@ -406,10 +405,9 @@ __global__ void k_cell_binsort(Ctx c) {
     sort_bin<true>(c.sorted, s, e, c.newidx);
 }
 
-// Particle map in logical order, and the movement of the bin's particles (RadiationProcessor.cuh:
-// 61-70: position += velocity * dt on the torus) into the end-of-step arrays at their map
-// positions; also publishes the new number of live particles per world.  The particles cell
-// radiation emitted this step come from their staging slots, behind every live particle's index.
+// Particle map in logical order (the particles cell radiation emitted this step, from their staging
+// slots, sort behind every live particle's index); also publishes the new number of live
+// particles per world and the table of live particle blocks.
 __global__ void k_part_binsort(Ctx c) {
     if (blockIdx.x == 0) {
         for (int wi = threadIdx.x; wi < c.numWorlds; wi += blockDim.x) {
@@ -438,18 +436,51 @@ __global__ void k_part_binsort(Ctx c) {
     int s, e;
     part_bin_range(c, w, gb - w.pbinOff, s, e);
     sort_bin<false>(c.psorted, s, e, nullptr);
-    const float dt = c.params[blk.x].dt;
-    for (int k = s; k < e; ++k) {
+}
+
+// The movement of the particles in map order (RadiationProcessor.cuh:61-70: position += velocity *
+// dt on the torus) into the end-of-step arrays, one thread per particle of the new live ranges; a
+// particle emitted this step comes from its staging record.
+__global__ void k_part_move(Ctx c) {
+    // (the start-of-step and end-of-step arrays are distinct)
+    const float2* __restrict__ appos = c.A.ppos;
+    const float2* __restrict__ apvel = c.A.pvel;
+    const float* __restrict__ apen = c.A.pen;
+    const int* __restrict__ aplast = c.A.plast;
+    const int* __restrict__ apslot = c.A.pslot;
+    const uint8_t* __restrict__ apflags = c.A.pflags;
+    const unsigned long long* __restrict__ apid = c.A.pid;
+    float2* __restrict__ bppos = c.B.ppos;
+    float2* __restrict__ bpvel = c.B.pvel;
+    float* __restrict__ bpen = c.B.pen;
+    int* __restrict__ bplast = c.B.plast;
+    int* __restrict__ bpslot = c.B.pslot;
+    uint8_t* __restrict__ bpflags = c.B.pflags;
+    unsigned long long* __restrict__ bpid = c.B.pid;
+    for_live_particles(c, [&](int wi, const DevWorld& w, int k) {
+        const float dt = c.params[wi].dt;
         const int src = c.psorted[k];
-        const f2 v = to_f2(c.A.pvel[src]);
-        c.B.ppos[k] = to_float2(torus_pos(add(to_f2(c.A.ppos[src]), mul(v, dt)), w.W, w.H));
-        c.B.pvel[k] = to_float2(v);
-        c.B.pen[k] = c.A.pen[src];
-        c.B.plast[k] = c.A.plast[src];
-        c.B.pslot[k] = c.A.pslot[src];
-        c.B.pflags[k] = c.A.pflags[src];
-        c.B.pid[k] = c.A.pid[src];
-    }
+        if (c.partStage > 0 && src >= c.partStage) {
+            const StagedParticle r = c.staged[src - c.partStage];
+            const f2 v = to_f2(r.vel);
+            bppos[k] = to_float2(torus_pos(add(to_f2(r.pos), mul(v, dt)), w.W, w.H));
+            bpvel[k] = r.vel;
+            bpen[k] = r.energy;
+            bplast[k] = kNone;
+            bpslot[k] = kNone;
+            bpflags[k] = uint8_t(r.color);
+            bpid[k] = r.id;
+            return;
+        }
+        const f2 v = to_f2(apvel[src]);
+        bppos[k] = to_float2(torus_pos(add(to_f2(appos[src]), mul(v, dt)), w.W, w.H));
+        bpvel[k] = to_float2(v);
+        bpen[k] = apen[src];
+        bplast[k] = aplast[src];
+        bpslot[k] = apslot[src];
+        bpflags[k] = apflags[src];
+        bpid[k] = apid[src];
+    });
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -492,7 +523,10 @@ void launch_part_scatter(const Ctx& c, const LaunchShape& sh, hipStream_t s) {
 void launch_part_sort(const Ctx& c, const LaunchShape& sh, hipStream_t s) {
     if (!sh.hasParticles) return;
     if (sh.partBlocks) hipLaunchKernelGGL(k_part_scatter, dim3(sh.partGrid), dim3(sh.partThreads), 0, s, c);
+    if (c.partStage > 0 && sh.cellBlocks)
+        hipLaunchKernelGGL(k_part_scatter_staged, dim3(sh.cellBlocks), dim3(sh.cellThreads), 0, s, c);
     if (sh.pbinBlocks) hipLaunchKernelGGL(k_part_binsort, dim3(sh.pbinBlocks), dim3(sh.binThreads), 0, s, c);
+    if (sh.partBlocks) hipLaunchKernelGGL(k_part_move, dim3(sh.partGrid), dim3(sh.partThreads), 0, s, c);
 }
 
 void launch_cell_binsort(const Ctx& c, const LaunchShape& sh, hipStream_t s) {

@@ -633,6 +633,7 @@ void WorldSet::build(const std::vector<Source>& sources) {
     stiff_ = static_cast<float*>(A(sizeof(float) * NC));
     cid_ = static_cast<unsigned long long*>(A(sizeof(unsigned long long) * NC));
     ageBase_ = static_cast<uint32_t*>(A(sizeof(uint32_t) * NC));
+    staged_ = partStage_ > 0 ? static_cast<StagedParticle*>(A(sizeof(StagedParticle) * NC)) : nullptr;
     living_[0] = static_cast<uint8_t*>(A(NC));
     living_[1] = static_cast<uint8_t*>(A(NC));
     activation_ = static_cast<uint32_t*>(A(sizeof(uint32_t) * NC));
@@ -1107,6 +1108,7 @@ Ctx WorldSet::makeCtx(int parity) const {
     c.cid = cid_;
     c.ageBase = ageBase_;
     c.partStage = partStage_;
+    c.staged = staged_;
     c.living = living_[parity];
     c.livingNext = living_[parity ^ 1];
     c.activation = activation_;
@@ -1465,6 +1467,7 @@ void WorldSet::growParticles(const std::vector<int64_t>& caps) {
     absorbCell_ = static_cast<int*>(alloc(sizeof(int) * NP));
     partCap_ = NP;
     partStage_ = stage;
+    if (!staged_) staged_ = static_cast<StagedParticle*>(alloc(sizeof(StagedParticle) * size_t(cellCap_)));
     launch_fill_u32(reinterpret_cast<uint32_t*>(fclaim_), uint32_t(kClaimFree), size_t(NP), stream_);
     launch_fill_u32(reinterpret_cast<uint32_t*>(buf_[parity_ ^ 1].pflags), 0u, (size_t(NP) + 3) / 4, stream_);
     launch_fill_u32(reinterpret_cast<uint32_t*>(pbinOf_ + partStage_), 0xffffffffu, size_t(cellCap_), stream_);

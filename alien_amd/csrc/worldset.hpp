@@ -215,6 +215,7 @@ private:
     unsigned long long* cid_ = nullptr;
     uint32_t* ageBase_ = nullptr;  // cell ages by slot at each world's ageTs
     int partStage_ = 0;            // first staging slot of radiation-emitted particles (0: none)
+    StagedParticle* staged_ = nullptr;  // [cellCap] with a staging area
     int numCellBlocks_ = 0;
     uint8_t* living_[2] = {nullptr, nullptr};  // living states by slot, alternating with the parity
     uint32_t* activation_ = nullptr;

@@ -100,3 +100,21 @@ def test_batch_with_radiation_equals_alone(graph):
         ref = from_world(w, threads=CPU_THREADS)
         ref.step(13)
         assert_identical(b.get_data(i), ref.get_data(), f"batch world {i} ({w.name}) graph={graph}")
+
+
+def test_server_with_radiation_equals_alone():
+    """Radiating jobs through the job server (set rebuilds by device copies as jobs come and go,
+    worlds frozen between chunks): each snapshot equal to the oracle's run of the job alone."""
+    ws = [radiating(worlds.random_mix(451, motion=MOTION_FLUID, num_particles=60), strength=0.003, prob=0.3),
+          worlds.random_mix(452),
+          radiating(living_mix(453, MOTION_COLLISION), strength=0.002, prob=0.5, absorb=0.0)]
+    steps = [21, 6, 14]
+    srv = engine.Server(budget_bytes=1 << 30, chunk_steps=4)
+    jobs = [srv.submit(w.size, w.params, w.cells, w.particles, s, seed=w.seed) for w, s in zip(ws, steps)]
+    for j in jobs:
+        srv.wait(j, timeout=120)
+    for j, w, s in zip(jobs, ws, steps):
+        ref = from_world(w, threads=CPU_THREADS)
+        ref.step(s)
+        assert_identical(srv.snapshot(j), ref.get_data(), f"server job {w.name}")
+    srv.close()
