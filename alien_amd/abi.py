@@ -106,6 +106,7 @@ class Params(ctypes.Structure):
         ("highRadiationFactor", ctypes.c_float * MAX_COLORS),
         ("highRadiationMinCellEnergy", ctypes.c_float * MAX_COLORS),
         ("radiationCellAgeStrength", ctypes.c_float * MAX_COLORS),
+        ("particleSplitEnergy", ctypes.c_float * MAX_COLORS),
     ]
 
     def copy(self):
@@ -158,6 +159,7 @@ def default_params(**overrides):
         p.highRadiationFactor[c] = 0.0
         p.highRadiationMinCellEnergy[c] = 500.0
         p.radiationCellAgeStrength[c] = 0.0
+        p.particleSplitEnergy[c] = float("inf")
     p.radiationProb = 0.03
     p.radiationVelocityMultiplier = 1.0
     p.radiationVelocityPerturbation = 0.5

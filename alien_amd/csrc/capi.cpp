@@ -121,6 +121,7 @@ void alien_params_default(alien_params* p) {
         p->highRadiationFactor[c] = 0.0f;
         p->highRadiationMinCellEnergy[c] = 500.0f;
         p->radiationCellAgeStrength[c] = 0.00002f;
+        p->particleSplitEnergy[c] = std::numeric_limits<float>::infinity();
     }
     p->radiationProb = 0.03f;
     p->radiationVelocityMultiplier = 1.0f;

@@ -171,6 +171,9 @@ constexpr uint32_t kStreamForceDecay = 1;  // checkForces, CellProcessor.cuh:367
 // CellProcessor::radiation (CellProcessor.cuh:725-773): whether the cell radiates this step, the
 // fraction of the energy loss drawn, the direction of the velocity perturbation
 constexpr uint32_t kStreamRadiation = 2, kStreamRadiationEnergy = 3, kStreamRadiationAngle = 4;
+// RadiationProcessor::splitting (RadiationProcessor.cuh:172-204): whether the particle splits, the
+// direction of the split
+constexpr uint32_t kStreamSplit = 5, kStreamSplitAngle = 6;
 __host__ __device__ inline float uniform_draw(unsigned long long seed, unsigned long long timestep,
                                               unsigned long long id, uint32_t purpose) {
     uint32_t c0 = uint32_t(id), c1 = uint32_t(id >> 32), c2 = uint32_t(timestep), c3 = purpose;

@@ -44,9 +44,12 @@ constexpr uint32_t kDetachedBit = 1u << 17;
 // particle flags byte
 constexpr uint8_t kParticleColorMask = 0x7F;
 constexpr uint8_t kParticleDead = 0x80;
-// ids of the particles cell radiation creates: this bit, the timestep (bits 32-62) and the local
-// index of the emitting cell (bits 0-31); ids given to set_data must not have it
+// ids of the particles cell radiation creates: this bit, the timestep (bits 32-61) and the local
+// index of the emitting cell (bits 0-31)
 constexpr unsigned long long kCreatedParticleId = 1ull << 63;
+// with this bit too: the particle came from a split (timestep in bits 32-61, index of the particle
+// that split in bits 0-31)
+constexpr unsigned long long kSplitParticleId = 1ull << 62;
 
 __host__ __device__ inline int flags_num_conn(uint32_t f) { return int(f & kNumConnMask); }
 __host__ __device__ inline int flags_max_conn(uint32_t f) { return int((f >> kMaxConnShift) & 0xFu); }
@@ -82,6 +85,8 @@ struct DevParams {
     // cell radiation (CellProcessor.cuh:725-773); radiates: radiationProb > 0 and some colour has
     // a positive high-radiation factor or age strength
     int radiates;
+    int splits;                  // some colour has a finite particleSplitEnergy
+    float splitEnergy[kColors];  // particleSplitEnergy (RadiationProcessor::splitting)
     float radProb, radVelMul, radVelPert;
     int radMinAge[kColors];
     float highRadFactor[kColors], highRadMinEnergy[kColors], radAgeStrength[kColors];
@@ -153,9 +158,12 @@ struct __align__(16) StagedParticle {
     float2 pos;  // pre-move position
     float2 vel;
     float energy;
-    uint32_t color;
+    uint32_t color;  // | kStagedPending for a split half not yet in the particle arrays
     unsigned long long id;
 };
+// a split half waits in its record from the end of one step to the movement of the next (and is
+// part of the world's state in between: get_data, statistics and rebuilt sets include it)
+constexpr uint32_t kStagedPending = 0x100u;
 
 struct Ctx {
     int numWorlds;
@@ -177,6 +185,12 @@ struct Ctx {
     // particle map like the live particles.
     int partStage;
     StagedParticle* staged;  // [cellCap]: the particle of staging slot partStage + i at staged[i]
+    // Particles split off at the end of a step: staging slot splitStage + k for the particle at k
+    // (0: no split staging area), record splitStaged[k]; they enter the next step's map behind the
+    // particles that lived and before that step's radiation
+    int splitStage;
+    StagedParticle* splitStaged;
+    int timestepAdvanced;  // 1 if the step's map scan advanced the worlds' timesteps
     // living states of the worlds with livingWork, cold by slot: `living` holds the state at the
     // start of the step, `livingNext` the other buffer (the step's phases alternate between the
     // two and leave the end-of-step state in `livingNext`, the next step's `living`)

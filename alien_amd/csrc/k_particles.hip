@@ -14,7 +14,9 @@ namespace alien {
 
 // start-of-step position of particle map entry src (a live particle or a staging slot)
 __device__ __forceinline__ f2 pre_move_pos(const Ctx& c, int src) {
-    return c.partStage > 0 && src >= c.partStage ? to_f2(c.staged[src - c.partStage].pos) : to_f2(c.A.ppos[src]);
+    if (c.partStage > 0 && src >= c.partStage) return to_f2(c.staged[src - c.partStage].pos);
+    if (c.splitStage > 0 && src >= c.splitStage) return to_f2(c.splitStaged[src - c.splitStage].pos);
+    return to_f2(c.A.ppos[src]);
 }
 
 // The particle map winner of the slot of position p (R2): the first particle of the bin whose
@@ -116,6 +118,42 @@ __global__ void k_part_apply(Ctx c) {
 }
 
 // absorption (:108-165); also bins the surviving particles for the next step
+// RadiationProcessor::splitting (RadiationProcessor.cuh:172-204), after absorption: with
+// probability 0.01 (R9 draw keyed by the particle id) a particle above particleSplitEnergy of its
+// colour halves its energy and splits in two, 1/5 of a random unit vector u either side of its
+// position, their velocities its own -/+ u radiationVelocityPerturbation / (E/2 + 1).  The new half
+// goes to the split staging slot of the particle's index and into the next step's histogram, so
+// that the next map sorts it behind the particles that lived, in index order (R18); its id is
+// kCreatedParticleId | kSplitParticleId | timestep << 32 | local index.
+__device__ __forceinline__ void split_particle(const Ctx& c, const DevWorld& w, int wi, int k) {
+    const DevParams& P = c.params[wi];
+    const unsigned long long id = c.B.pid[k];
+    const unsigned long long ts = c.ctr[wi].timestep - (c.timestepAdvanced ? 1ull : 0ull);
+    if (!(uniform_draw(w.seed, ts, id, kStreamSplit) < 0.01f)) return;
+    const int col = c.B.pflags[k] & kParticleColorMask;
+    float e = c.B.pen[k];
+    if (!(e > P.splitEnergy[col % kColors])) return;
+    e = e * 0.5f;
+    f2 u = to_f2(unit_vector_of_angle(uniform_draw(w.seed, ts, id, kStreamSplitAngle) * 360.0f));
+    const f2 p = to_f2(c.B.ppos[k]);
+    const f2 d = {u.x / 5.0f, u.y / 5.0f};
+    const f2 other = torus_pos(add(p, d), w.W, w.H);
+    const f2 self = torus_pos(sub(p, d), w.W, w.H);
+    u = mul(u, P.radVelPert / (e + 1.0f));
+    const f2 v = to_f2(c.B.pvel[k]);
+    c.B.pen[k] = e;
+    c.B.ppos[k] = to_float2(self);
+    c.B.pvel[k] = to_float2(sub(v, u));
+    StagedParticle r;
+    r.pos = to_float2(other);
+    r.vel = to_float2(add(v, u));
+    r.energy = e;
+    r.color = uint32_t(col) | kStagedPending;
+    r.id = kCreatedParticleId | kSplitParticleId | ((ts & 0x3fffffffull) << 32) | (unsigned long long)uint32_t(k - w.partOff);
+    c.splitStaged[k] = r;
+    pbin_for_next_step(c, w, c.splitStage + k, other, c.countNext);
+}
+
 __global__ void k_part_absorb(Ctx c) {
     for_live_particles(c, [&](int wi, const DevWorld& w, int k) {
         const int cm = c.absorbCell[k];
@@ -138,10 +176,12 @@ __global__ void k_part_absorb(Ctx c) {
             else
                 c.B.plast[k] = cslot;
         }
-        if (c.B.pflags[k] & kParticleDead)
+        if (c.B.pflags[k] & kParticleDead) {
             c.pbinOf[k] = -1;
-        else
-            pbin_for_next_step(c, w, k, to_f2(c.B.ppos[k]), c.countNext);
+            return;
+        }
+        if (c.splitStage > 0 && c.params[wi].splits) split_particle(c, w, wi, k);
+        pbin_for_next_step(c, w, k, to_f2(c.B.ppos[k]), c.countNext);
     });
 }
 
@@ -192,7 +232,7 @@ __global__ void k_radiation(Ctx c) {
         r.vel = to_float2(v);
         r.energy = loss;
         r.color = uint32_t(col) & kParticleColorMask;
-        r.id = kCreatedParticleId | ((ts & 0x7fffffffull) << 32) | (unsigned long long)uint32_t(i - w.cellOff);
+        r.id = kCreatedParticleId | ((ts & 0x3fffffffull) << 32) | (unsigned long long)uint32_t(i - w.cellOff);
         c.staged[i] = r;
         pbin_for_next_step(c, w, c.partStage + i, p, c.count);
     }

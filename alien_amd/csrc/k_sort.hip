@@ -38,6 +38,14 @@ __global__ void k_part_bin(Ctx c) {
             c.pbinOf[i] = -1;
         else
             pbin_for_next_step(c, w, i, to_f2(c.A.ppos[i]), c.count);
+        // a half split off particle i at the end of the last step
+        if (c.splitStage > 0) {
+            const StagedParticle& r = c.splitStaged[i];
+            if (r.color & kStagedPending)
+                pbin_for_next_step(c, w, c.splitStage + i, to_f2(r.pos), c.count);
+            else
+                c.pbinOf[c.splitStage + i] = -1;
+        }
     });
 }
 
@@ -330,9 +338,17 @@ __device__ __forceinline__ void scatter_particle(const Ctx& c, const DevWorld& w
 
 __global__ void k_part_scatter(Ctx c) {
     for_live_particles(c, [&](int, const DevWorld& w, int i) {
-        int gb = c.pbinOf[i];
-        if (gb < 0) return;
-        scatter_particle(c, w, gb, i);
+        const int gb = c.pbinOf[i];
+        if (gb >= 0) scatter_particle(c, w, gb, i);
+        // the particle split off particle i at the end of the last step (cleared once read)
+        if (c.splitStage > 0) {
+            const int st = c.splitStage + i;
+            const int g2 = c.pbinOf[st];
+            if (g2 >= 0) {
+                c.pbinOf[st] = -1;
+                scatter_particle(c, w, g2, st);
+            }
+        }
     });
 }
 
@@ -460,15 +476,17 @@ __global__ void k_part_move(Ctx c) {
     for_live_particles(c, [&](int wi, const DevWorld& w, int k) {
         const float dt = c.params[wi].dt;
         const int src = c.psorted[k];
-        if (c.partStage > 0 && src >= c.partStage) {
-            const StagedParticle r = c.staged[src - c.partStage];
+        const bool rad = c.partStage > 0 && src >= c.partStage;
+        if (rad || (c.splitStage > 0 && src >= c.splitStage)) {
+            const StagedParticle r = rad ? c.staged[src - c.partStage] : c.splitStaged[src - c.splitStage];
+            if (!rad) c.splitStaged[src - c.splitStage].color = r.color & ~kStagedPending;  // (in the arrays now)
             const f2 v = to_f2(r.vel);
             bppos[k] = to_float2(torus_pos(add(to_f2(r.pos), mul(v, dt)), w.W, w.H));
             bpvel[k] = r.vel;
             bpen[k] = r.energy;
             bplast[k] = kNone;
             bpslot[k] = kNone;
-            bpflags[k] = uint8_t(r.color);
+            bpflags[k] = uint8_t(r.color & kParticleColorMask);
             bpid[k] = r.id;
             return;
         }

@@ -25,6 +25,7 @@ struct LaunchShape {
     // no work, are not launched)
     bool hasParticles = true;
     bool hasRadiation = false;  // some active world radiates (DevParams::radiates)
+    bool hasSplitting = false;  // some active world's particles can split (DevParams::splits)
     int livingBlocks = 0;           // cell blocks of the worlds whose living states live on the device
     bool hasInnerFriction = false;  // some active world with bond work has innerFriction > 0
 };

@@ -133,8 +133,8 @@ __global__ void __launch_bounds__(kStatThreads) k_statistics(Ctx c, int wi, int 
     const DevWorld w = c.worlds[wi];
     const bool cells = int(blockIdx.x) < cellBlocks;
     const int idx = (cells ? int(blockIdx.x) : int(blockIdx.x) - cellBlocks) * kStatThreads + int(threadIdx.x);
-    int col = -1;
-    double e = 0.0;
+    int col = -1, col2 = -1;
+    double e = 0.0, e2 = 0.0;
     if (cells) {
         if (idx < w.numCells) {
             const int i = w.cellOff + idx;
@@ -149,12 +149,20 @@ __global__ void __launch_bounds__(kStatThreads) k_statistics(Ctx c, int wi, int 
             col = int(pf & kParticleColorMask) % kColors;
             e = double(c.A.pen[j]);
         }
+        // a half split off it at the end of the last step, waiting for the next map
+        if (c.splitStage > 0) {
+            const StagedParticle r = c.splitStaged[j];
+            if (r.color & kStagedPending) {
+                col2 = int(r.color & kParticleColorMask) % kColors;
+                e2 = double(r.energy);
+            }
+        }
     }
     const int lane = int(threadIdx.x) & 63, wave = int(threadIdx.x) >> 6;
 #pragma unroll
     for (int k = 0; k < kColors; ++k) {
-        int n = col == k ? 1 : 0;
-        double s = col == k ? e : 0.0;
+        int n = (col == k ? 1 : 0) + (col2 == k ? 1 : 0);
+        double s = (col == k ? e : 0.0) + (col2 == k ? e2 : 0.0);
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1) {
             n += __shfl_down(n, o, 64);

@@ -7,6 +7,7 @@
 // subset of such a file into alien_params / alien_world_desc, reports which processes of the file
 // this path does not simulate, and writes the subset back in the same layout.
 #include <cmath>
+#include <limits>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -257,6 +258,10 @@ int alien_settings_read(const char* json, size_t length, alien_world_desc* desc,
         r.colors(S + "high radiation.factor", p.highRadiationFactor);
         for (int c = 0; c < ALIEN_MAX_COLORS; ++c)
             r.num(S + "radiation.min cell age[" + std::to_string(c) + "]", p.radiationMinCellAge[c]);
+        // particle splitting (:662); ALIEN writes its "unbounded" default as the largest float
+        r.colors(S + "particle.split energy", p.particleSplitEnergy);
+        for (float& e : p.particleSplitEnergy)
+            if (e >= 3.0e38f) e = std::numeric_limits<float>::infinity();
         // processes in the file that the physics path does not simulate
         uint32_t u = 0;
         if (r.value(S + "spots.num spots", 0.0) > 0.0) u |= ALIEN_SETTINGS_ZONES;
@@ -269,7 +274,6 @@ int alien_settings_read(const char* json, size_t length, alien_world_desc* desc,
                                     "radiation.absorption low genome complexity penalty"})
                 if (r.value(S + pen + k, 0.0) != 0.0) u |= ALIEN_SETTINGS_RADIATION;
             if (r.value(S + "cell.death probability" + k, 0.001) > 0.0) u |= ALIEN_SETTINGS_CELL_DEATH;
-            if (r.value(S + "particle.split energy" + k, 3.4e38) < 3.0e38) u |= ALIEN_SETTINGS_PARTICLE_SPLIT;
         }
         if (r.value(S + "cell.death consequences", 2.0) != 0.0) u |= ALIEN_SETTINGS_CELL_DEATH;
         if (r.value(S + "rigidity", 0.0) != 0.0) u |= ALIEN_SETTINGS_RIGIDITY;
@@ -321,8 +325,12 @@ int alien_settings_write(const alien_world_desc* desc, const alien_params* param
         put(root, S + "radiation.velocity perturbation", fmt(p.radiationVelocityPerturbation));
         colors(S + "high radiation.min cell energy", p.highRadiationMinCellEnergy);
         colors(S + "high radiation.factor", p.highRadiationFactor);
-        for (int c = 0; c < ALIEN_MAX_COLORS; ++c)
+        for (int c = 0; c < ALIEN_MAX_COLORS; ++c) {
             put(root, S + "radiation.min cell age[" + std::to_string(c) + "]", std::to_string(p.radiationMinCellAge[c]));
+            const float e = p.particleSplitEnergy[c];
+            put(root, S + "particle.split energy[" + std::to_string(c) + "]",
+                fmt(std::isinf(e) && e > 0 ? 3.40282346638528859811704183484516925440e38 : double(e)));
+        }
         // what this path keeps fixed, so that ALIEN reads the file back to the same physics
         put(root, S + "cell.death consequences", "0");
         put(root, S + "spots.num spots", "0");

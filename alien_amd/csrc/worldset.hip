@@ -38,6 +38,7 @@ void validate_params(const alien_params& p) {
             throw Error(ALIEN_E_ARG, "radiation factors must be finite and not negative");
         if (std::isnan(p.highRadiationMinCellEnergy[c]))
             throw Error(ALIEN_E_ARG, "highRadiationMinCellEnergy must be a number");
+        if (std::isnan(p.particleSplitEnergy[c])) throw Error(ALIEN_E_ARG, "particleSplitEnergy must be a number");
     }
     if (!(p.radiationProb >= 0.0f && p.radiationProb <= 1.0f)) throw Error(ALIEN_E_ARG, "radiationProb must lie in [0, 1]");
     if (!std::isfinite(p.radiationVelocityMultiplier) || !std::isfinite(p.radiationVelocityPerturbation))
@@ -124,7 +125,9 @@ DevParams to_dev_params(const alien_params& p) {
     d.radProb = p.radiationProb;
     d.radVelMul = p.radiationVelocityMultiplier;
     d.radVelPert = p.radiationVelocityPerturbation;
+    d.splits = splits(p) ? 1 : 0;
     for (int c = 0; c < kColors; ++c) {
+        d.splitEnergy[c] = p.particleSplitEnergy[c];
         d.radMinAge[c] = p.radiationMinCellAge[c];
         d.highRadFactor[c] = p.highRadiationFactor[c];
         d.highRadMinEnergy[c] = p.highRadiationMinCellEnergy[c];
@@ -479,6 +482,8 @@ void WorldSet::clearAll() {
     parity_ = 0;
 }
 
+int stageLayout(int64_t caps, int cells, bool radiation, bool splitting, int& splitStage, int& radStage);
+
 void WorldSet::build(const std::vector<Source>& sources) {
     // collect the new layout first (sources may live in this very set's old buffers only via a
     // different WorldSet object)
@@ -559,7 +564,9 @@ void WorldSet::build(const std::vector<Source>& sources) {
         w.binOff = int(bins);
         w.partOff = int(partOff);
         // room for the particles cell radiation emits (one per cell and step at most)
-        const int64_t cap = live + (radiates(h.params) ? int64_t(2 * kRadiationSteps) * h.numCells : 0);
+        // (and for particle splitting, one split per particle and step at most)
+        const int64_t cap = (live + (radiates(h.params) ? int64_t(2 * kRadiationSteps) * h.numCells : 0)) *
+                            (splits(h.params) ? 2 : 1);
         if (partOff + cap > kMaxSetParticles)
             throw Error(ALIEN_E_CAPACITY, "the particle room of the radiating worlds exceeds the particles of one world set");
         w.partCap = int(cap);
@@ -595,12 +602,13 @@ void WorldSet::build(const std::vector<Source>& sources) {
     WorldSet& me = *this;
     invalidateGraphs();
 
-    bool anyRadiates = false;
-    for (const auto& h : hosts) anyRadiates = anyRadiates || radiates(h.params);
+    bool anyRadiates = false, anySplits = false;
+    for (const auto& h : hosts) {
+        anyRadiates = anyRadiates || radiates(h.params);
+        anySplits = anySplits || splits(h.params);
+    }
     const int NC = int(std::max<int64_t>(cellOff, 1));
-    // the worlds' particle ranges, then (with a radiating world) one staging slot per cell
-    partStage_ = anyRadiates ? int(std::max<int64_t>(partOff, 1)) : 0;
-    const int NP = anyRadiates ? partStage_ + NC : int(std::max<int64_t>(partOff, 1));
+    const int NP = stageLayout(partOff, NC, anyRadiates, anySplits, splitStage_, partStage_);
     const int NB = int(std::max<int64_t>(bins, 1));
     cellCap_ = NC;
     partCap_ = NP;
@@ -634,6 +642,7 @@ void WorldSet::build(const std::vector<Source>& sources) {
     cid_ = static_cast<unsigned long long*>(A(sizeof(unsigned long long) * NC));
     ageBase_ = static_cast<uint32_t*>(A(sizeof(uint32_t) * NC));
     staged_ = partStage_ > 0 ? static_cast<StagedParticle*>(A(sizeof(StagedParticle) * NC)) : nullptr;
+    splitStaged_ = splitStage_ > 0 ? static_cast<StagedParticle*>(A(sizeof(StagedParticle) * size_t(splitStage_))) : nullptr;
     living_[0] = static_cast<uint8_t*>(A(NC));
     living_[1] = static_cast<uint8_t*>(A(NC));
     activation_ = static_cast<uint32_t*>(A(sizeof(uint32_t) * NC));
@@ -677,7 +686,14 @@ void WorldSet::build(const std::vector<Source>& sources) {
     absorbCell_ = static_cast<int*>(A(sizeof(int) * NP));
     cclaim_ = static_cast<int*>(A(sizeof(int) * NC));
 
-    // copy world states into buffer 0
+    // copy world states into buffer 0 (and, from other sets, the split halves still waiting)
+    struct PendingCopy {
+        size_t world;
+        int fromOff;
+        const StagedParticle* from;
+        int count;
+    };
+    std::vector<PendingCopy> pendingCopies;
     HotBuffers& D = buf_[0];
     for (size_t wi = 0; wi < sources.size(); ++wi) {
         const auto& src = sources[wi];
@@ -749,6 +765,7 @@ void WorldSet::build(const std::vector<Source>& sources) {
             d2d(D.pslot + w.partOff, S.pslot + ow.partOff, sizeof(int) * np);
             d2d(D.pflags + w.partOff, S.pflags + ow.partOff, np);
             d2d(D.pid + w.partOff, S.pid + ow.partOff, sizeof(unsigned long long) * np);
+            pendingCopies.push_back({wi, int(ow.partOff), o.splitStaged_, np});
         }
     }
     check_hip(hipStreamSynchronize(stream_), "hipStreamSynchronize");
@@ -799,6 +816,16 @@ void WorldSet::build(const std::vector<Source>& sources) {
     launch_fill_u32(reinterpret_cast<uint32_t*>(buf_[1].pflags), 0u, (size_t(NP) + 3) / 4, stream_);
     if (partStage_ > 0)  // no particle waits in a staging slot
         launch_fill_u32(reinterpret_cast<uint32_t*>(pbinOf_ + partStage_), 0xffffffffu, size_t(NC), stream_);
+    if (splitStage_ > 0) {
+        launch_fill_u32(reinterpret_cast<uint32_t*>(pbinOf_ + splitStage_), 0xffffffffu, size_t(splitStage_), stream_);
+        // no split half is pending but those copied below
+        launch_fill_u32(reinterpret_cast<uint32_t*>(splitStaged_), 0u, size_t(splitStage_) * sizeof(StagedParticle) / 4, stream_);
+        for (const PendingCopy& pc : pendingCopies)
+            if (pc.from && pc.count > 0)
+                check_hip(hipMemcpyAsync(splitStaged_ + worlds_[pc.world].partOff, pc.from + pc.fromOff,
+                                         sizeof(StagedParticle) * size_t(pc.count), hipMemcpyDeviceToDevice, stream_),
+                          "hipMemcpy D2D");
+    }
     check_hip(hipGetLastError(), "fill kernels");
     buildTables();
     launch_seed_bins(makeCtx(parity_), shape_, stream_);
@@ -995,11 +1022,28 @@ void WorldSet::buildTables() {
 }
 
 void WorldSet::updateRadiation() {
-    bool on = false;
-    for (size_t w = 0; w < params_.size(); ++w) on = on || (active_[w] && params_[w].radiates);
-    if (on != shape_.hasRadiation) invalidateGraphs();
+    bool on = false, split = false;
+    for (size_t w = 0; w < params_.size(); ++w) {
+        on = on || (active_[w] && params_[w].radiates);
+        split = split || (active_[w] && params_[w].splits);
+    }
+    if (on != shape_.hasRadiation || split != shape_.hasSplitting) invalidateGraphs();
     shape_.hasRadiation = on;
+    shape_.hasSplitting = split;
     if (on) shape_.hasParticles = true;  // (particles can appear)
+}
+
+// Index space of the particles: the worlds' ranges [0, caps), then the split staging slots (one per
+// particle slot) with a world that can split, then the radiation staging slots (one per cell) with
+// a radiating world.  Returns its size.
+int stageLayout(int64_t caps, int cells, bool radiation, bool splitting, int& splitStage, int& radStage) {
+    const int64_t c = std::max<int64_t>(caps, 1);
+    int64_t end = c;
+    splitStage = splitting ? int(end) : 0;
+    if (splitting) end += c;
+    radStage = radiation ? int(end) : 0;
+    if (radiation) end += cells;
+    return int(end);
 }
 
 // the inner-friction kernel runs when an active world with bond work has innerFriction > 0
@@ -1109,6 +1153,9 @@ Ctx WorldSet::makeCtx(int parity) const {
     c.ageBase = ageBase_;
     c.partStage = partStage_;
     c.staged = staged_;
+    c.splitStage = splitStage_;
+    c.splitStaged = splitStaged_;
+    c.timestepAdvanced = shape_.hasBondWork ? 0 : 1;
     c.living = living_[parity];
     c.livingNext = living_[parity ^ 1];
     c.activation = activation_;
@@ -1174,6 +1221,13 @@ int WorldSet::liveParticles(int w) {
                   "D2H");
     int live = 0;
     for (uint8_t x : f) live += (x & kParticleDead) ? 0 : 1;
+    if (splitStage_ > 0 && np > 0) {  // and the split halves waiting for the next map
+        std::vector<StagedParticle> rec(static_cast<size_t>(np));
+        check_hip(hipMemcpy(rec.data(), splitStaged_ + worlds_.at(w).partOff, sizeof(StagedParticle) * size_t(np),
+                            hipMemcpyDeviceToHost),
+                  "D2H");
+        for (const StagedParticle& r : rec) live += (r.color & kStagedPending) ? 1 : 0;
+    }
     return live;
 }
 
@@ -1238,6 +1292,13 @@ bool needs_fresh_set(const alien_params& before, const alien_params& after) {
     for (int c = 0; c < kColors; ++c)
         if (before.cellMinEnergy[c] != after.cellMinEnergy[c]) return true;
     return radiates(after) && !radiates(before);
+}
+
+// RadiationProcessor::splitting acts on a particle above a finite particleSplitEnergy of its colour
+bool splits(const alien_params& p) {
+    for (int c = 0; c < kColors; ++c)
+        if (!std::isinf(p.particleSplitEnergy[c]) || p.particleSplitEnergy[c] < 0.0f) return true;
+    return false;
 }
 
 // CellProcessor::radiation takes energy from cells only when a draw below radiationProb can meet
@@ -1306,8 +1367,9 @@ void WorldSet::step(uint64_t n) {
     };
     uint64_t roomUntil = 0;  // with cell radiation: steps of this call covered by the last room check
     for (uint64_t s = 0; s < n;) {
-        if (shape_.hasRadiation && s >= roomUntil) {
-            const uint64_t k = std::min<uint64_t>(n - s, kRadiationSteps);
+        if ((shape_.hasRadiation || shape_.hasSplitting) && s >= roomUntil) {
+            // (with splitting the live particles can double every step: room is made per step)
+            const uint64_t k = std::min<uint64_t>(n - s, shape_.hasSplitting ? 1 : kRadiationSteps);
             ensureParticleRoom(k);  // (a growth rebuilds the tables; the graphs follow)
             roomUntil = s + k;
         }
@@ -1321,7 +1383,8 @@ void WorldSet::step(uint64_t n) {
         } else if (useGraph_) {
             // several steps per graph while they fit before the end and the next particle check
             const uint64_t toCheck =
-                shape_.hasRadiation ? roomUntil - s : (shape_.hasParticles ? 32 - s % 32 : n - s);
+                (shape_.hasRadiation || shape_.hasSplitting) ? roomUntil - s
+                                                             : (shape_.hasParticles ? 32 - s % 32 : n - s);
             if (n - s >= kStepsPerGraph && toCheck >= kStepsPerGraph) done = kStepsPerGraph;
             check_hip(hipGraphLaunch(graphOf(done), stream_), "hipGraphLaunch");
         } else {
@@ -1375,16 +1438,19 @@ void WorldSet::dropParticlesIfNone() {
 }
 
 void WorldSet::ensureParticleRoom(uint64_t steps) {
-    if (!shape_.hasRadiation) return;
+    if (!shape_.hasRadiation && !shape_.hasSplitting) return;
     synchronize();
     std::vector<DevCounters> ctr(worlds_.size());
     check_hip(hipMemcpy(ctr.data(), dCtr_, sizeof(DevCounters) * ctr.size(), hipMemcpyDeviceToHost), "D2H");
-    bool grow = partStage_ == 0;
+    bool grow = (shape_.hasRadiation && partStage_ == 0) || (shape_.hasSplitting && splitStage_ == 0);
     std::vector<int64_t> caps(worlds_.size());
     for (size_t w = 0; w < worlds_.size(); ++w) {
         caps[w] = worlds_[w].partCap;
-        if (!active_[w] || !params_[w].radiates) continue;
-        const int64_t need = int64_t(ctr[w].numParticles) + int64_t(steps) * worlds_[w].numCells;
+        const bool rad = params_[w].radiates != 0, spl = params_[w].splits != 0;
+        if (!active_[w] || !(rad || spl)) continue;
+        // (steps is 1 with splitting)
+        int64_t need = int64_t(ctr[w].numParticles) + (rad ? int64_t(steps) * worlds_[w].numCells : 0);
+        if (spl) need *= 2;
         if (need > caps[w]) {
             caps[w] = 2 * need;
             grow = true;
@@ -1411,8 +1477,13 @@ void WorldSet::growParticles(const std::vector<int64_t>& caps) {
         throw Error(ALIEN_E_CAPACITY, "the particle room of the radiating worlds exceeds the particles of one world set");
     std::vector<DevCounters> ctr(worlds_.size());
     check_hip(hipMemcpy(ctr.data(), dCtr_, sizeof(DevCounters) * ctr.size(), hipMemcpyDeviceToHost), "D2H");
-    const int stage = int(std::max<int64_t>(total, 1));
-    const int NP = stage + cellCap_;
+    bool anyRadiates = false, anySplits = false;
+    for (const DevParams& q : params_) {
+        anyRadiates = anyRadiates || q.radiates;
+        anySplits = anySplits || q.splits;
+    }
+    int splitStage = 0, radStage = 0;
+    const int NP = stageLayout(total, cellCap_, anyRadiates, anySplits, splitStage, radStage);
     // the particle arrays of both buffers and the per-particle step arrays, reallocated; the
     // current state (and its next-step bins) moves to the new world offsets
     HotBuffers nb[2];
@@ -1427,6 +1498,13 @@ void WorldSet::growParticles(const std::vector<int64_t>& caps) {
         nb[b].pid = static_cast<unsigned long long*>(alloc(sizeof(unsigned long long) * NP));
     }
     int* pbinOf = static_cast<int*>(alloc(sizeof(int) * NP));
+    // no particle waits in a new staging slot, but for the split particles of the last step, which
+    // move with the particles they split from
+    launch_fill_u32(reinterpret_cast<uint32_t*>(pbinOf), 0xffffffffu, size_t(NP), stream_);
+    StagedParticle* splitStaged =
+        splitStage > 0 ? static_cast<StagedParticle*>(alloc(sizeof(StagedParticle) * size_t(splitStage))) : nullptr;
+    if (splitStaged)
+        launch_fill_u32(reinterpret_cast<uint32_t*>(splitStaged), 0u, size_t(splitStage) * sizeof(StagedParticle) / 4, stream_);
     const HotBuffers& S = buf_[parity_];
     HotBuffers& D = nb[parity_];
     int64_t off = 0;
@@ -1445,6 +1523,10 @@ void WorldSet::growParticles(const std::vector<int64_t>& caps) {
         d2d(D.pflags + to, S.pflags + from, np);
         d2d(D.pid + to, S.pid + from, sizeof(unsigned long long) * np);
         d2d(pbinOf + to, pbinOf_ + from, sizeof(int) * np);
+        if (splitStage_ > 0 && splitStage > 0) {
+            d2d(pbinOf + splitStage + to, pbinOf_ + splitStage_ + from, sizeof(int) * np);
+            d2d(splitStaged + to, splitStaged_ + from, sizeof(StagedParticle) * np);
+        }
         dw.partOff = to;
         dw.partCap = int(caps[w]);
         off += caps[w];
@@ -1458,19 +1540,20 @@ void WorldSet::growParticles(const std::vector<int64_t>& caps) {
         buf_[b] = nb[b];
     }
     for (void* p : {static_cast<void*>(pbinOf_), static_cast<void*>(psorted_), static_cast<void*>(want_),
-                    static_cast<void*>(fclaim_), static_cast<void*>(absorbCell_)})
-        release(p);
+                    static_cast<void*>(fclaim_), static_cast<void*>(absorbCell_), static_cast<void*>(splitStaged_)})
+        if (p) release(p);
+    splitStaged_ = splitStaged;
     pbinOf_ = pbinOf;
     psorted_ = static_cast<int*>(alloc(sizeof(int) * NP));
     want_ = static_cast<int*>(alloc(sizeof(int) * NP));
     fclaim_ = static_cast<int*>(alloc(sizeof(int) * NP));
     absorbCell_ = static_cast<int*>(alloc(sizeof(int) * NP));
     partCap_ = NP;
-    partStage_ = stage;
-    if (!staged_) staged_ = static_cast<StagedParticle*>(alloc(sizeof(StagedParticle) * size_t(cellCap_)));
+    partStage_ = radStage;
+    splitStage_ = splitStage;
+    if (partStage_ > 0 && !staged_) staged_ = static_cast<StagedParticle*>(alloc(sizeof(StagedParticle) * size_t(cellCap_)));
     launch_fill_u32(reinterpret_cast<uint32_t*>(fclaim_), uint32_t(kClaimFree), size_t(NP), stream_);
     launch_fill_u32(reinterpret_cast<uint32_t*>(buf_[parity_ ^ 1].pflags), 0u, (size_t(NP) + 3) / 4, stream_);
-    launch_fill_u32(reinterpret_cast<uint32_t*>(pbinOf_ + partStage_), 0xffffffffu, size_t(cellCap_), stream_);
     check_hip(hipGetLastError(), "fill kernels");
     check_hip(hipMemcpy(dWorlds_, worlds_.data(), sizeof(DevWorld) * worlds_.size(), hipMemcpyHostToDevice), "H2D");
     buildTables();  // (particle blocks of the new capacities; the graphs are captured again)
@@ -1546,6 +1629,15 @@ void WorldSet::extract(int wi, WorldImage& img) {
     d2h(img.pslot.data(), S.pslot + w.partOff, sizeof(int) * np);
     d2h(img.pflags.data(), S.pflags + w.partOff, np);
     d2h(img.pid.data(), S.pid + w.partOff, sizeof(uint64_t) * np);
+    // the halves split off at the end of the last step, which enter the map at the next step,
+    // follow the particles that lived (the order the next map gives them)
+    std::vector<StagedParticle> halves;
+    if (splitStage_ > 0 && np > 0) {
+        std::vector<StagedParticle> rec(static_cast<size_t>(np));
+        d2h(rec.data(), splitStaged_ + w.partOff, sizeof(StagedParticle) * size_t(np));
+        for (const StagedParticle& r : rec)
+            if (r.color & kStagedPending) halves.push_back(r);
+    }
     // particles killed in the last step are still in the live range until the next sort
     int keep = 0;
     for (int i = 0; i < np; ++i) {
@@ -1566,7 +1658,16 @@ void WorldSet::extract(int wi, WorldImage& img) {
     img.pslot.resize(keep);
     img.pflags.resize(keep);
     img.pid.resize(keep);
-    img.numParticles = keep;
+    for (const StagedParticle& r : halves) {
+        img.ppos.push_back(r.pos);
+        img.pvel.push_back(r.vel);
+        img.pen.push_back(r.energy);
+        img.plast.push_back(kNone);
+        img.pslot.push_back(kNone);
+        img.pflags.push_back(uint8_t(r.color & kParticleColorMask));
+        img.pid.push_back(r.id);
+    }
+    img.numParticles = int(img.ppos.size());
 }
 
 alien_totals WorldSet::totals(int wi) {

@@ -40,6 +40,8 @@ bool needs_fresh_set(const alien_params& before, const alien_params& after);
 // whether the parameters let cell radiation take energy from cells (radiationProb > 0 and some
 // colour's high-radiation factor or age strength positive)
 bool radiates(const alien_params& p);
+// whether some colour has a finite particleSplitEnergy (particles can split)
+bool splits(const alien_params& p);
 DevParams to_dev_params(const alien_params& p);
 int choose_strip(const std::vector<float2>& pos, int W, int H, int binSize, bool twoRowStages, int& rowsPerStage);
 void strip_shape(DevWorld& w, int strip, int rowsPerStage);
@@ -216,6 +218,8 @@ private:
     uint32_t* ageBase_ = nullptr;  // cell ages by slot at each world's ageTs
     int partStage_ = 0;            // first staging slot of radiation-emitted particles (0: none)
     StagedParticle* staged_ = nullptr;  // [cellCap] with a staging area
+    int splitStage_ = 0;                     // first split staging slot (0: none)
+    StagedParticle* splitStaged_ = nullptr;  // [splitStage_] with a split staging area
     int numCellBlocks_ = 0;
     uint8_t* living_[2] = {nullptr, nullptr};  // living states by slot, alternating with the parity
     uint32_t* activation_ = nullptr;

@@ -87,9 +87,10 @@ typedef struct alien_particle_to {
  * Motion.h:7-43, SimulationParametersZoneValues.h:10-24).  alien_params_default() fills ALIEN's
  * defaults.  The processes of the reference that this path does not implement are fixed at these
  * values (no field here): radiation sources none (numRadiationSources 0), external energy off,
- * particle splitting and transformation off, numZones 0, rigidity 0, cellDeathConsequences None,
+ * particle transformation off, numZones 0, rigidity 0, cellDeathConsequences None,
  * cellDeathProbability 0.  Cell radiation (CellProcessor::radiation, CellProcessor.cuh:725-773)
- * is simulated: its fields close the struct.
+ * and particle splitting (RadiationProcessor::splitting, RadiationProcessor.cuh:172-204) are
+ * simulated: their fields close the struct.
  */
 typedef struct alien_params {
     float timestepSize;                /* SimulationParameters.h:61, default 1.0           */
@@ -117,6 +118,7 @@ typedef struct alien_params {
     float highRadiationFactor[ALIEN_MAX_COLORS];          /* SimulationParameters.h:85, 0  */
     float highRadiationMinCellEnergy[ALIEN_MAX_COLORS];   /* SimulationParameters.h:86, 500 */
     float radiationCellAgeStrength[ALIEN_MAX_COLORS];     /* ZoneValues.h:17, 0.00002     */
+    float particleSplitEnergy[ALIEN_MAX_COLORS];          /* SimulationParameters.h:114, +inf */
 } alien_params;
 
 /* World geometry and capacity (GeneralSettings worldSizeX/worldSizeY), and the key of the world's
@@ -329,7 +331,7 @@ uint64_t alien_server_plan_admission(uint64_t budgetBytes, uint64_t oldSetBytes,
 #define ALIEN_SETTINGS_RADIATION_SOURCES 0x2u       /* particle sources                           */
 #define ALIEN_SETTINGS_RADIATION 0x4u               /* radiation absorption penalties             */
 #define ALIEN_SETTINGS_CELL_DEATH 0x8u              /* death consequences / death probability     */
-#define ALIEN_SETTINGS_PARTICLE_SPLIT 0x10u         /* finite particle split energy               */
+#define ALIEN_SETTINGS_PARTICLE_SPLIT 0x10u         /* (not reported: particle splitting is simulated) */
 #define ALIEN_SETTINGS_RIGIDITY 0x20u               /* rigidity != 0                              */
 #define ALIEN_SETTINGS_PARTICLE_TRANSFORMATION 0x40u
 #define ALIEN_SETTINGS_ABSORPTION_CONTROL 0x80u     /* features.additional absorption control     */

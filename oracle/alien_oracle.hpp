@@ -42,7 +42,10 @@
 //       the particles it creates join the step's particle map behind the live ones, in that order
 //       (the reference appends them with atomic slot grabs).  A created particle's id is
 //       kCreatedParticleId | timestep << 32 | index of the emitting cell (the reference takes ids
-//       from a shared atomic counter).
+//       from a shared atomic counter).  Particle splitting (RadiationProcessor.cuh:172-204) runs
+//       over the particles in map order after absorption; the halves it creates join the next
+//       step's particle map behind the particles that lived, in that order, before that step's
+//       radiation; their ids are kCreatedParticleId | kSplitParticleId | timestep << 32 | index.
 //   R12 SPH sums gathered per cell in visiting order.
 //   R13 collision neighbours use the uncorrected (seam-blind) distance like Map.cuh:189.
 // Three rules change only the rounding of the FP32 build (in exact arithmetic they are the
@@ -223,7 +226,9 @@ constexpr uint32_t kStreamForceDecay = 1;  // CellProcessor.cuh:367-386
 // CellProcessor::radiation (CellProcessor.cuh:725-773): the radiation draw, the energy fraction,
 // the perturbation angle
 constexpr uint32_t kStreamRadiation = 2, kStreamRadiationEnergy = 3, kStreamRadiationAngle = 4;
-constexpr uint64_t kCreatedParticleId = uint64_t(1) << 63;
+// RadiationProcessor::splitting (RadiationProcessor.cuh:172-204): the split draw, the split direction
+constexpr uint32_t kStreamSplit = 5, kStreamSplitAngle = 6;
+constexpr uint64_t kCreatedParticleId = uint64_t(1) << 63, kSplitParticleId = uint64_t(1) << 62;
 
 // Math::unitVectorOfAngle (Math.cuh:134-138): {sin a, -cos a}, a in degrees in [0, 360).  The
 // reference calls sinf/cosf; here the sine and cosine are their Taylor series (to x^13 and x^14,
@@ -338,6 +343,7 @@ template <class R> struct Params {
     int radiationMinCellAge[ALIEN_MAX_COLORS];
     R highRadiationFactor[ALIEN_MAX_COLORS], highRadiationMinCellEnergy[ALIEN_MAX_COLORS];
     R radiationCellAgeStrength[ALIEN_MAX_COLORS];
+    R particleSplitEnergy[ALIEN_MAX_COLORS];
     static Params from(const alien_params& p) {
         Params q;
         q.timestepSize = R(p.timestepSize);
@@ -363,6 +369,7 @@ template <class R> struct Params {
             q.highRadiationFactor[c] = R(p.highRadiationFactor[c]);
             q.highRadiationMinCellEnergy[c] = R(p.highRadiationMinCellEnergy[c]);
             q.radiationCellAgeStrength[c] = R(p.radiationCellAgeStrength[c]);
+            q.particleSplitEnergy[c] = R(p.particleSplitEnergy[c]);
         }
         q.radiationProb = R(p.radiationProb);
         q.radiationVelocityMultiplier = R(p.radiationVelocityMultiplier);
@@ -903,6 +910,37 @@ public:
                 P.lastAbsorbedCellId = C.id;
             }
         }
+        // splitting (RadiationProcessor.cuh:172-204): with probability 0.01 a particle above the
+        // split energy of its colour halves its energy and splits, the halves 1/5 of a random unit
+        // vector u either side, their velocities -/+ u perturbation / (E/2 + 1)
+        for (size_t k = 0; k < np; ++k) {
+            if (!parts[k].alive) continue;
+            const uint64_t id = parts[k].id;
+            if (!(R(uniformDraw(seed, timestep, id, kStreamSplit)) < R(0.01f))) continue;
+            if (!(parts[k].energy > p.particleSplitEnergy[parts[k].color % ALIEN_MAX_COLORS])) continue;
+            Particle<R> q = parts[k];
+            q.energy = q.energy * R(0.5f);
+            V2<R> u = unitVectorOfAngle(R(uniformDraw(seed, timestep, id, kStreamSplitAngle)) * R(360));
+            const V2<R> d = u / R(5);
+            V2<R> other = q.pos + d;
+            torus.correctPosition(other);
+            q.pos = q.pos - d;
+            torus.correctPosition(q.pos);
+            u = u * (p.radiationVelocityPerturbation / (q.energy + R(1)));
+            Particle<R> half{};
+            half.id = kCreatedParticleId | kSplitParticleId | ((timestep & 0x3fffffffull) << 32) | uint64_t(uint32_t(k));
+            half.pos = other;
+            half.vel = q.vel + u;
+            half.energy = q.energy;
+            half.color = q.color;
+            half.selected = 0;
+            half.lastAbsorbedCellId = 0;
+            half.hasLast = false;
+            half.alive = true;
+            q.vel = q.vel - u;
+            parts[k] = q;
+            parts.push_back(half);
+        }
     }
 
     // ---- connection forces: CellProcessor.cuh:408-491 (R7) -------------------------------------
@@ -1198,7 +1236,7 @@ public:
             if (loss > c.energy - R(1)) loss = c.energy - R(1);
             if (loss > K<R>::NEAR_ZERO) {
                 Particle<R> q{};
-                q.id = kCreatedParticleId | ((timestep & 0x7fffffffull) << 32) | uint64_t(uint32_t(i));
+                q.id = kCreatedParticleId | ((timestep & 0x3fffffffull) << 32) | uint64_t(uint32_t(i));
                 q.pos = pos;
                 q.vel = vel;
                 q.energy = loss;

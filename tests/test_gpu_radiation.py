@@ -10,6 +10,7 @@ from alien_amd import engine, worlds
 from alien_amd.abi import MOTION_COLLISION, MOTION_FLUID
 from oracle.pyoracle import OracleWorld, from_world
 
+from . import helpers as H
 from .test_gpu_living_friction import living_mix, run_checkpoints
 from .test_gpu_parity import CPU_THREADS, assert_identical, say
 
@@ -103,12 +104,14 @@ def test_batch_with_radiation_equals_alone(graph):
 
 
 def test_server_with_radiation_equals_alone():
-    """Radiating jobs through the job server (set rebuilds by device copies as jobs come and go,
-    worlds frozen between chunks): each snapshot equal to the oracle's run of the job alone."""
+    """Radiating and splitting jobs through the job server (set rebuilds by device copies as jobs
+    come and go, split halves waiting between steps carried along, worlds frozen between chunks):
+    each snapshot equal to the oracle's run of the job alone."""
     ws = [radiating(worlds.random_mix(451, motion=MOTION_FLUID, num_particles=60), strength=0.003, prob=0.3),
           worlds.random_mix(452),
-          radiating(living_mix(453, MOTION_COLLISION), strength=0.002, prob=0.5, absorb=0.0)]
-    steps = [21, 6, 14]
+          radiating(living_mix(453, MOTION_COLLISION), strength=0.002, prob=0.5, absorb=0.0),
+          splitting(worlds.random_mix(454, motion=MOTION_COLLISION, num_particles=500), 1.5)]
+    steps = [21, 6, 14, 26]
     srv = engine.Server(budget_bytes=1 << 30, chunk_steps=4)
     jobs = [srv.submit(w.size, w.params, w.cells, w.particles, s, seed=w.seed) for w, s in zip(ws, steps)]
     for j in jobs:
@@ -118,3 +121,49 @@ def test_server_with_radiation_equals_alone():
         ref.step(s)
         assert_identical(srv.snapshot(j), ref.get_data(), f"server job {w.name}")
     srv.close()
+
+
+def splitting(w, threshold):
+    p = w.params.copy()
+    for c in range(7):
+        p.particleSplitEnergy[c] = threshold
+    return worlds.World(w.name + "_split", w.size, p, w.cells, w.particles, w.steps, w.seed)
+
+
+@pytest.mark.parametrize("motion", [MOTION_FLUID, MOTION_COLLISION])
+def test_splitting_bit_exact(motion):
+    """Particles above the split energy split with probability 0.01 per step (RadiationProcessor.cuh:
+    172-204): the halves, staged at the end of a step, enter the next map; GPU equal to the oracle."""
+    w = splitting(worlds.random_mix(461 + motion, motion=motion, num_particles=600), 1.5)
+    say(f"splitting: {w.name} motion={motion}")
+    sim, _ = run_checkpoints(w, [1, 2, 7, 25, 45])
+    ids = sim.get_data()[1]["id"].astype(np.uint64)
+    assert ((ids & np.uint64(1 << 62)) != 0).sum() > 0
+
+
+def sparse_lattice(n=32, spacing=6.0):
+    """Cells on a sparse lattice, slow and aged: the particles they emit spread out and rarely fuse."""
+    c = H.cells(n * n)
+    k = np.arange(n * n)
+    c["pos"][:, 0] = 1.5 + spacing * (k % n)
+    c["pos"][:, 1] = 1.5 + spacing * (k // n)
+    c["vel"] = np.random.default_rng(5).normal(0, 0.05, size=(n * n, 2)).astype(np.float32)
+    c["age"] = 3
+    p = H.params(MOTION_COLLISION, cellMaxCollisionDistance=1.3)
+    return worlds.World("sparse_lattice", (int(n * spacing), int(n * spacing)), p, c, H.particles(0), 10, 9)
+
+
+def test_splitting_with_radiation_and_room_growth():
+    """Every cell radiates every step, nothing absorbs, particles split above a low energy: both
+    kinds of created particles, the per-step room check, and array growth while split halves wait
+    in their staging slots for the next step."""
+    base = sparse_lattice()
+    w = splitting(radiating(base, strength=0.01, prob=1.0, absorb=0.0), 0.05)
+    say(f"splitting with radiation: {w.name} cells={len(w.cells)}")
+    sim, _ = run_checkpoints(w, [1, 3, 10, 16])
+    parts = sim.get_data()[1]
+    ids = parts["id"].astype(np.uint64)
+    split = (ids & np.uint64(1 << 62)) != 0
+    assert split.sum() > 100 and ((ids >= np.uint64(CREATED)) & ~split).sum() > 1000
+    # the room the world was built with, 2 * 8 * cells, was outgrown: the arrays grew
+    assert 2 * (len(parts) + len(w.cells)) > 2 * 8 * len(w.cells)

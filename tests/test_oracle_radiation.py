@@ -206,3 +206,100 @@ def test_radiated_particles_are_absorbed_like_any_other(oracle_lib):
     total0 = e0["cellEnergy"] + e0["particleEnergy"]
     total1 = e1["cellEnergy"] + e1["particleEnergy"]
     assert math.isclose(total0, total1, rel_tol=1e-9)
+
+
+# --------------------------------------------------------------------------------------------
+# particle splitting (RadiationProcessor.cuh:172-204): after absorption, with probability 0.01
+# (draw(seed, timestep, particle id, 5)), a particle above particleSplitEnergy of its colour halves
+# its energy; the halves go 1/5 of the unit vector of angle 360 draw(..., 6) either side of its
+# (moved) position, with velocities v -/+ u radiationVelocityPerturbation / (E/2 + 1)
+
+SPLIT = 1 << 62
+
+
+def split_params(threshold, **kw):
+    p = rad_params(**kw)
+    for c in range(7):
+        p.particleSplitEnergy[c] = threshold
+    return p
+
+
+def first_split_step(pid, limit=5000):
+    return next(t for t in range(limit) if draw(SEED, t, pid, 5) < np.float32(0.01))
+
+
+@pytest.mark.parametrize("precision", [32, 64])
+def test_split_closed_form(oracle_lib, precision):
+    q = H.particles(1)
+    q["pos"][0] = (20.25, 30.75)
+    q["vel"][0] = (0.3, -0.2)
+    q["energy"] = 10.0
+    q["color"] = 3
+    pid = int(q["id"][0])
+    t = first_split_step(pid)
+    o = OracleWorld((64, 64), split_params(5.0), precision=precision, seed=SEED)
+    o.set_data(H.cells(0), q)
+    o.timestep = t
+    o.step(1)
+    _, parts = o.get_data()
+    assert len(parts) == 2
+    byid = {int(x["id"]): x for x in parts}
+    half = byid[CREATED | SPLIT | (t << 32) | 0]
+    orig = byid[pid]
+    a = math.radians(360.0 * draw(SEED, t, pid, 6))
+    u = (math.sin(a), -math.cos(a))
+    moved = (20.25 + 0.3, 30.75 - 0.2)
+    k = 0.5 / (5.0 + 1.0)
+    tol = 4e-6  # (both builds return float32 transfer objects)
+    for x, sign in ((orig, -1.0), (half, 1.0)):
+        assert float(x["energy"]) == 5.0 and x["color"] == 3
+        assert abs(float(x["pos"][0]) - (moved[0] + sign * u[0] / 5)) < tol
+        assert abs(float(x["pos"][1]) - (moved[1] + sign * u[1] / 5)) < tol
+        assert abs(float(x["vel"][0]) - (0.3 + sign * u[0] * k)) < tol
+        assert abs(float(x["vel"][1]) - (-0.2 + sign * u[1] * k)) < tol
+
+
+def test_which_particles_split(oracle_lib):
+    n = 30
+    q = H.particles(n * n)
+    k = np.arange(n * n)
+    q["pos"][:, 0] = 1.5 + 4.0 * (k % n)
+    q["pos"][:, 1] = 1.5 + 4.0 * (k // n)
+    q["energy"] = np.where(k % 3 == 0, 2.0, 8.0)  # every third below the threshold 2.0 (strict)
+    p = split_params(2.0)
+    for t in (0, 7):
+        o = OracleWorld((128, 128), p, seed=SEED)
+        o.set_data(H.cells(0), q)
+        o.timestep = t
+        o.step(1)
+        _, parts = o.get_data()
+        expect = sorted(int(q["id"][i]) for i in k
+                        if q["energy"][i] > 2.0 and draw(SEED, t, int(q["id"][i]), 5) < np.float32(0.01))
+        # the particles that split kept half their energy; each half is a new particle
+        halved = sorted(int(x["id"]) for x in parts if not int(x["id"]) & CREATED and x["energy"] == np.float32(4.0))
+        halves = [x for x in parts if int(x["id"]) & SPLIT]
+        assert halved == expect and len(halves) == len(expect) and len(parts) == n * n + len(expect)
+        assert all(x["energy"] == np.float32(4.0) for x in halves)
+    assert len(expect) > 0
+
+
+def test_split_and_radiation_together(oracle_lib):
+    # a particle that splits at step t among radiating cells: one half with the split id of step t
+    # (the halves, 0.4 apart, fuse again in the next step), and the radiated particles with theirs
+    c = lattice(2)
+    q = H.particles(1)
+    q["pos"][0] = (30.5, 30.5)
+    q["energy"] = 10.0
+    pid = int(q["id"][0])
+    t = first_split_step(pid)
+    p = split_params(5.0, age_strength=0.001, prob=1.0)
+    o = OracleWorld((48, 48), p, seed=SEED)
+    o.set_data(c, q)
+    o.timestep = t
+    o.step(1)
+    _, parts = o.get_data()
+    ids = [int(x["id"]) for x in parts]
+    # the particle is map entry 4, behind the four radiated ones (bins in row order)
+    assert [i for i in ids if i & SPLIT] == [CREATED | SPLIT | (t << 32) | 4]
+    radiated = [i for i in ids if i & CREATED and not i & SPLIT]
+    assert sorted(radiated) == [CREATED | (t << 32) | k for k in range(4)]
