@@ -392,6 +392,27 @@ __global__ void k_living_decay(Ctx c) {
     c.livingNext[sl] = next;
 }
 
+// A set without bond work has no bonded or barrier cell: aging with the activation count
+// (CellProcessor.cuh:561-594), the transitions (:596-676; UnderConstruction stays without an
+// Activating partner) and the decay to Dying (:775-858; no partner to mark Detaching) of the three
+// kernels above need no partner, and one pass per cell gives their end-of-step state.
+__global__ void k_living_unbonded(Ctx c) {
+    int x, wi;
+    if (!living_cell(c, x, wi)) return;
+    const DevWorld& w = c.worlds[wi];
+    const DevParams& P = c.params[wi];
+    const uint32_t fx = c.B.flags[x];
+    const int sl = w.cellOff + c.B.slot[x];
+    uint8_t st = c.living[sl];
+    if (!flags_barrier(fx)) {
+        const uint32_t act = c.activation[sl];
+        if (st == kReady && act > 0u) c.activation[sl] = act - 1u;
+    }
+    if (flags_barrier(fx) || st == kActivating || st == kReviving || st == kDetaching) st = kReady;
+    if (below_min_energy(c, w, P, fx, sl)) st = kDying;
+    c.livingNext[sl] = st;
+}
+
 void launch_living_transitions(const Ctx& c, const LaunchShape& sh, hipStream_t s, Recorder* rec) {
     if (!sh.livingBlocks) return;
     if (rec) rec->mark("living_mark_age");
@@ -657,8 +678,7 @@ void launch_bonds(const Ctx& c, const LaunchShape& sh, hipStream_t s, Recorder* 
     // no world can have a bonded or barrier cell this step (none has one and none can fuse): no
     // cell is staged, and the scan at the start of the step advanced the timesteps
     if (!sh.hasBondWork || sh.bondBlocks == 0) {
-        launch_living_transitions(c, sh, s, rec);
-        launch_living_decay(c, sh, s, rec);
+        if (sh.livingBlocks) L("living_unbonded", k_living_unbonded, dim3(sh.livingBlocks), dim3(sh.cellThreads), c);
         return;
     }
     const dim3 b(sh.cellThreads), cb(256);

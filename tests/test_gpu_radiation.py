@@ -167,3 +167,22 @@ def test_splitting_with_radiation_and_room_growth():
     assert split.sum() > 100 and ((ids >= np.uint64(CREATED)) & ~split).sum() > 1000
     # the room the world was built with, 2 * 8 * cells, was outgrown: the arrays grew
     assert 2 * (len(parts) + len(w.cells)) > 2 * 8 * len(w.cells)
+
+
+def test_living_states_without_bond_work():
+    """A radiating world whose cells have no bonds and no bond slots (no bond work): its living
+    states (every state, activation counts, decay to Dying as radiation drains cells below
+    cellMinEnergy) advance in one pass per cell, equal to the oracle's."""
+    base = sparse_lattice(n=24, spacing=4.0)
+    c = base.cells.copy()
+    rng = np.random.default_rng(481)
+    c["maxConnections"] = 0
+    c["livingState"] = rng.integers(0, 6, size=len(c)).astype(np.uint8)
+    c["activationTime"] = np.where(rng.random(len(c)) < 0.4, rng.integers(1, 6, size=len(c)), 0).astype(np.uint32)
+    c["energy"] = rng.uniform(45.0, 60.0, size=len(c)).astype(np.float32)
+    w = radiating(worlds.World("unbonded_living", base.size, base.params, c, base.particles, 10, base.seed),
+                  strength=0.02, prob=0.5)
+    say(f"living states without bond work: cells={len(c)}")
+    sim, _ = run_checkpoints(w, [1, 2, 6, 15])
+    st = sim.get_data()[0]["livingState"]
+    assert (st == 5).sum() > len(c) // 4 and (st == 0).any()
