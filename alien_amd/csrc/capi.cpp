@@ -885,8 +885,10 @@ int alien_server_submit(alien_server* s, const alien_world_desc* desc, const ali
         job->stepsRequested = steps;
         job->numCells = numCells;
         job->numParticles = numParticles;
-        job->bytes = WorldSet::estimateBytes(numCells, numParticles, desc->worldSizeX, desc->worldSizeY,
-                                             alien::bin_size_for(*params));
+        // (a radiating or splitting world is built with room for the particles it will create)
+        job->bytes = WorldSet::estimateBytes(
+            numCells, uint64_t(alien::particle_room(int64_t(numCells), int64_t(numParticles), *params)),
+            desc->worldSizeX, desc->worldSizeY, alien::bin_size_for(*params));
         if (job->bytes > s->budget) throw Error(ALIEN_E_BUDGET, "job does not fit the device memory budget");
         job->submitTime = nowSeconds();
         std::lock_guard<std::mutex> lk(s->mu);

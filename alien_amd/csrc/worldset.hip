@@ -563,10 +563,7 @@ void WorldSet::build(const std::vector<Source>& sources) {
         w.numCells = h.numCells;
         w.binOff = int(bins);
         w.partOff = int(partOff);
-        // room for the particles cell radiation emits (one per cell and step at most)
-        // (and for particle splitting, one split per particle and step at most)
-        const int64_t cap = (live + (radiates(h.params) ? int64_t(2 * kRadiationSteps) * h.numCells : 0)) *
-                            (splits(h.params) ? 2 : 1);
+        const int64_t cap = particle_room(h.numCells, live, h.params);
         if (partOff + cap > kMaxSetParticles)
             throw Error(ALIEN_E_CAPACITY, "the particle room of the radiating worlds exceeds the particles of one world set");
         w.partCap = int(cap);
@@ -1294,6 +1291,11 @@ bool needs_fresh_set(const alien_params& before, const alien_params& after) {
     return radiates(after) && !radiates(before);
 }
 
+int64_t particle_room(int64_t cells, int64_t particles, const alien_params& p) {
+    constexpr int64_t kRoomSteps = 8;  // twice the steps between the room checks of a radiating set
+    return (particles + (radiates(p) ? kRoomSteps * cells : 0)) * (splits(p) ? 2 : 1);
+}
+
 // RadiationProcessor::splitting acts on a particle above a finite particleSplitEnergy of its colour
 bool splits(const alien_params& p) {
     for (int c = 0; c < kColors; ++c)
@@ -1438,6 +1440,7 @@ void WorldSet::dropParticlesIfNone() {
 }
 
 void WorldSet::ensureParticleRoom(uint64_t steps) {
+    static_assert(2 * kRadiationSteps == 8, "particle_room builds radiating worlds with room for 8 steps");
     if (!shape_.hasRadiation && !shape_.hasSplitting) return;
     synchronize();
     std::vector<DevCounters> ctr(worlds_.size());

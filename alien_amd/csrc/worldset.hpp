@@ -42,6 +42,9 @@ bool needs_fresh_set(const alien_params& before, const alien_params& after);
 bool radiates(const alien_params& p);
 // whether some colour has a finite particleSplitEnergy (particles can split)
 bool splits(const alien_params& p);
+// particle slots a world is built with: its particles, plus room for 8 steps of cell radiation
+// (one particle per cell and step at most), doubled when its particles can split
+int64_t particle_room(int64_t cells, int64_t particles, const alien_params& p);
 DevParams to_dev_params(const alien_params& p);
 int choose_strip(const std::vector<float2>& pos, int W, int H, int binSize, bool twoRowStages, int& rowsPerStage);
 void strip_shape(DevWorld& w, int strip, int rowsPerStage);
