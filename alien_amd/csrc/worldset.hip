@@ -1038,8 +1038,11 @@ int stageLayout(int64_t caps, int cells, bool radiation, bool splitting, int& sp
     int64_t end = c;
     splitStage = splitting ? int(end) : 0;
     if (splitting) end += c;
-    radStage = radiation ? int(end) : 0;
+    radStage = radiation && end < (int64_t(1) << 31) ? int(end) : 0;
     if (radiation) end += cells;
+    // (particle indices, staging slots included, are 32-bit)
+    if (end >= (int64_t(1) << 31) - 1)
+        throw Error(ALIEN_E_CAPACITY, "the particle slots and staging slots of the world set exceed 32-bit indices");
     return int(end);
 }
 
